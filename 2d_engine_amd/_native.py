@@ -1,0 +1,267 @@
+"""ctypes binding of the p2d C ABI (include/p2d.h) -> libp2d.so.
+
+The HIP library is the only compute path: when libp2d.so is missing this
+module raises at import, and when no GPU is visible every compute call raises
+P2DError (status P2D_NO_DEVICE).  Arrays may be numpy (host memory) or torch
+tensors on the GPU (device memory, used in place); shapes follow the C ABI:
+cells (ny, nx), x-faces (ny, nx+1), y-faces (ny+1, nx), row-major.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libp2d.so")
+
+P2D_OK, P2D_NOT_CONVERGED, P2D_NAN, P2D_BAD_ARG, P2D_HIP_ERROR, P2D_NO_DEVICE = range(6)
+MEM_HOST, MEM_DEVICE = 0, 1
+FP64, MIXED = 0, 1
+PREC_F64, PREC_F32 = 0, 1
+
+# Kernels p2d_time_kernel can time (finest level).
+KERNEL_DIR_APPLY_F32 = 0
+KERNEL_DIR_APPLY_F64 = 1
+KERNEL_GS_F32 = 2
+KERNEL_CORRECT_RESID = 3
+
+EXPORTS = [
+    "p2d_config_default", "p2d_create", "p2d_destroy", "p2d_set_stream", "p2d_set_fluid",
+    "p2d_set_coefficients", "p2d_project", "p2d_get_pressure", "p2d_get_coefficients", "p2d_adjoint",
+    "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_status_string",
+]
+
+
+class P2DConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("h", ctypes.c_double),
+        ("precision", ctypes.c_int), ("mg_max_levels", ctypes.c_int),
+        ("mg_pre_sweeps", ctypes.c_int), ("mg_post_sweeps", ctypes.c_int),
+        ("rtol", ctypes.c_double), ("div_tol", ctypes.c_double),
+        ("max_iterations", ctypes.c_int), ("max_outer", ctypes.c_int),
+        ("inner_rtol", ctypes.c_double),
+    ]
+
+
+class P2DStats(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int), ("iterations", ctypes.c_int), ("outer", ctypes.c_int),
+        ("rel_residual", ctypes.c_double), ("max_abs_residual", ctypes.c_double),
+        ("max_div", ctypes.c_double), ("max_u", ctypes.c_double), ("max_v", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class P2DError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} ({status})")
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    cfgp, stp = ctypes.POINTER(P2DConfig), ctypes.POINTER(P2DStats)
+    sig = {
+        "p2d_config_default": (None, [cfgp, i, i, d]),
+        "p2d_create": (i, [cfgp, ctypes.POINTER(vp)]),
+        "p2d_destroy": (i, [vp]),
+        "p2d_set_stream": (i, [vp, vp]),
+        "p2d_set_fluid": (i, [vp, vp, d, d, d, i]),
+        "p2d_set_coefficients": (i, [vp, vp, vp, vp, vp, i]),
+        "p2d_project": (i, [vp, vp, vp, d, stp, i]),
+        "p2d_get_pressure": (i, [vp, vp, i, i]),
+        "p2d_get_coefficients": (i, [vp, vp, vp, vp, i]),
+        "p2d_adjoint": (i, [vp, vp, vp, vp, vp, vp, vp, stp, i]),
+        "p2d_apply_stencil": (i, [vp, vp, vp, i, i]),
+        "p2d_vcycle": (i, [vp, vp, vp, i, i]),
+        "p2d_mg_levels": (i, [vp]),
+        "p2d_time_kernel": (i, [vp, i, i, ctypes.POINTER(d)]),
+        "p2d_status_string": (ctypes.c_char_p, [i]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def status_string(status):
+    return lib.p2d_status_string(status).decode()
+
+
+def default_config(nx, ny, h):
+    c = P2DConfig()
+    lib.p2d_config_default(ctypes.byref(c), nx, ny, h)
+    return c
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+class _Buf:
+    """Pointer + memory kind for one argument; host numpy arrays are made contiguous copies when needed."""
+
+    def __init__(self, a, shape, dtype=np.float64, writable=False):
+        if a is None:
+            self.ptr, self.mem, self.arr = None, None, None
+            return
+        if _is_torch(a):
+            import torch
+            want = torch.float64 if dtype == np.float64 else torch.float32
+            if tuple(a.shape) != tuple(shape) or a.dtype != want or not a.is_contiguous():
+                raise ValueError(f"tensor must be contiguous {want} of shape {shape}, got {a.dtype} {tuple(a.shape)}")
+            self.mem = MEM_DEVICE if a.is_cuda else MEM_HOST
+            self.ptr, self.arr = a.data_ptr(), a
+            return
+        arr = np.asarray(a)
+        if arr.shape != tuple(shape):
+            raise ValueError(f"array must have shape {shape}, got {arr.shape}")
+        if writable:
+            if arr.dtype != dtype or not arr.flags["C_CONTIGUOUS"]:
+                raise ValueError(f"output array must be C-contiguous {np.dtype(dtype)}")
+        else:
+            arr = np.ascontiguousarray(arr, dtype=dtype)
+        self.mem, self.ptr, self.arr = MEM_HOST, arr.ctypes.data, arr
+
+
+def _mem(*bufs):
+    kinds = {b.mem for b in bufs if b.mem is not None}
+    if len(kinds) > 1:
+        raise ValueError("mixing host and device arrays in one call")
+    return kinds.pop() if kinds else MEM_HOST
+
+
+def _check(rc, what, allow=(P2D_OK,)):
+    if rc not in allow:
+        raise P2DError(rc, what)
+    return rc
+
+
+class Context:
+    """One projection context (p2d_ctx) for an nx x ny grid of spacing h."""
+
+    def __init__(self, nx, ny, h, precision=FP64, **cfg):
+        c = default_config(nx, ny, h)
+        c.precision = precision
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise TypeError(f"unknown config field {k}")
+            setattr(c, k, v)
+        self.cfg = c
+        self.nx, self.ny, self.h = nx, ny, h
+        self._h = ctypes.c_void_p()
+        _check(lib.p2d_create(ctypes.byref(c), ctypes.byref(self._h)), "p2d_create")
+
+    # shapes
+    @property
+    def cell_shape(self):
+        return (self.ny, self.nx)
+
+    @property
+    def u_shape(self):
+        return (self.ny, self.nx + 1)
+
+    @property
+    def v_shape(self):
+        return (self.ny + 1, self.nx)
+
+    def close(self):
+        if self._h:
+            lib.p2d_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle):
+        _check(lib.p2d_set_stream(self._h, stream_handle), "p2d_set_stream")
+
+    def set_fluid(self, phi, rho_minus, rho_plus, sigma):
+        b = _Buf(phi, self.cell_shape)
+        _check(lib.p2d_set_fluid(self._h, b.ptr, rho_minus, rho_plus, sigma, b.mem), "p2d_set_fluid")
+
+    def set_coefficients(self, beta_x, beta_y, phi, jc=None):
+        a, b, p, j = (_Buf(beta_x, self.u_shape), _Buf(beta_y, self.v_shape), _Buf(phi, self.cell_shape),
+                      _Buf(jc, self.cell_shape))
+        _check(lib.p2d_set_coefficients(self._h, a.ptr, b.ptr, p.ptr, j.ptr, _mem(a, b, p, j)),
+               "p2d_set_coefficients")
+
+    def project(self, u, v, dt=1.0, allow_not_converged=False):
+        """Project u, v in place (device tensors) or return projected copies (numpy). Returns (u, v, stats)."""
+        if not _is_torch(u):
+            u = np.array(u, dtype=np.float64, order="C", copy=True)
+            v = np.array(v, dtype=np.float64, order="C", copy=True)
+        bu, bv = _Buf(u, self.u_shape, writable=True), _Buf(v, self.v_shape, writable=True)
+        st = P2DStats()
+        allow = (P2D_OK, P2D_NOT_CONVERGED) if allow_not_converged else (P2D_OK,)
+        _check(lib.p2d_project(self._h, bu.ptr, bv.ptr, dt, ctypes.byref(st), _mem(bu, bv)), "p2d_project", allow)
+        return u, v, st.as_dict()
+
+    def pressure(self, mean_zero=True, out=None):
+        if out is None:
+            out = np.empty(self.cell_shape)
+        b = _Buf(out, self.cell_shape, writable=True)
+        _check(lib.p2d_get_pressure(self._h, b.ptr, 1 if mean_zero else 0, b.mem), "p2d_get_pressure")
+        return out
+
+    def coefficients(self):
+        bx, by, jc = np.empty(self.u_shape), np.empty(self.v_shape), np.empty(self.cell_shape)
+        _check(lib.p2d_get_coefficients(self._h, bx.ctypes.data, by.ctypes.data, jc.ctypes.data, MEM_HOST),
+               "p2d_get_coefficients")
+        return bx, by, jc
+
+    def adjoint(self, gu, gv, outs=None):
+        """Returns (du, dv, dbeta_x, dbeta_y, stats)."""
+        a, b = _Buf(gu, self.u_shape), _Buf(gv, self.v_shape)
+        if outs is None:
+            if a.mem == MEM_DEVICE:
+                import torch
+                dev = gu.device
+                outs = (torch.empty(self.u_shape, dtype=torch.float64, device=dev),
+                        torch.empty(self.v_shape, dtype=torch.float64, device=dev),
+                        torch.empty(self.u_shape, dtype=torch.float64, device=dev),
+                        torch.empty(self.v_shape, dtype=torch.float64, device=dev))
+            else:
+                outs = (np.empty(self.u_shape), np.empty(self.v_shape), np.empty(self.u_shape), np.empty(self.v_shape))
+        o = [_Buf(x, s, writable=True) for x, s in zip(outs, (self.u_shape, self.v_shape, self.u_shape, self.v_shape))]
+        st = P2DStats()
+        _check(lib.p2d_adjoint(self._h, a.ptr, b.ptr, o[0].ptr, o[1].ptr, o[2].ptr, o[3].ptr, ctypes.byref(st),
+                               _mem(a, b, *o)), "p2d_adjoint")
+        return outs[0], outs[1], outs[2], outs[3], st.as_dict()
+
+    def apply_stencil(self, x, prec=PREC_F64):
+        dt = np.float64 if prec == PREC_F64 else np.float32
+        bx = _Buf(x, self.cell_shape, dtype=dt)
+        y = np.empty(self.cell_shape, dtype=dt) if bx.mem == MEM_HOST else x.new_empty(x.shape)
+        by = _Buf(y, self.cell_shape, dtype=dt, writable=True)
+        _check(lib.p2d_apply_stencil(self._h, bx.ptr, by.ptr, prec, _mem(bx, by)), "p2d_apply_stencil")
+        return y
+
+    def vcycle(self, r_scaled, prec=PREC_F64):
+        dt = np.float64 if prec == PREC_F64 else np.float32
+        br = _Buf(r_scaled, self.cell_shape, dtype=dt)
+        z = np.empty(self.cell_shape, dtype=dt) if br.mem == MEM_HOST else r_scaled.new_empty(r_scaled.shape)
+        bz = _Buf(z, self.cell_shape, dtype=dt, writable=True)
+        _check(lib.p2d_vcycle(self._h, br.ptr, bz.ptr, prec, _mem(br, bz)), "p2d_vcycle")
+        return z
+
+    @property
+    def mg_levels(self):
+        return lib.p2d_mg_levels(self._h)
+
+    def time_kernel(self, which, reps=50):
+        ms = ctypes.c_double()
+        _check(lib.p2d_time_kernel(self._h, which, reps, ctypes.byref(ms)), "p2d_time_kernel")
+        return ms.value

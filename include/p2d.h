@@ -1,0 +1,143 @@
+/*
+ * p2d -- MI355X (gfx950) pressure projection for 2D_Engine, C ABI.
+ *
+ * Replaces the projection path of the reference:
+ *   PROJECTION_2D::Solve(dt)            PROJECTION_2D.h:235-257
+ *     DetermineProjectionDensity        PROJECTION_2D.h:259-393
+ *     DetermineDivergence               PROJECTION_2D.h:395-478
+ *     DetermineJumpConditionField       PROJECTION_2D.h:760-866 (+ LEVELSET_2D.h:898-990)
+ *     DeterminePressure                 PROJECTION_2D.h:480-509
+ *       POISSON_SOLVER::Solve           POISSON_SOLVER.cpp:127-145
+ *       BuildLinearSystemNodeJumpConditionVaribleCoefficient  POISSON_SOLVER.cpp:714-976
+ *       LINEAR_SOLVER::Solve (CG/PCG)   LINEAR_SOLVER.h:31-35, CG_METHOD.cpp:13-73, PCG_METHOD.cpp:176-383
+ *       VectorToGrid                    POISSON_SOLVER.cpp:1593-1603
+ *     UpdateVelocity                    PROJECTION_2D.h:754-758, 1231-1417
+ *
+ * Layout: nx x ny cells of spacing h (dx == dy, as POISSON_SOLVER.cpp:872 assumes),
+ * arrays i-fastest without ghosts: cell (i,j) -> i + j*nx; x-faces u (nx+1) x ny,
+ * face (i,j) -> i + j*(nx+1) between cells (i-1,j),(i,j); y-faces v nx x (ny+1),
+ * face (i,j) -> i + j*nx between cells (i,j-1),(i,j).  This is the reference's
+ * ARRAY_2D order (ARRAY_2D.h:140-146) with the ghost ring stripped; the MAC grids
+ * are EULERIAN_FLUID_SOLVER_2D.h:357 (u) and :363 (v).  Walls are no-flux
+ * (Neumann_Boundary_Condition, PROJECTION_2D.h:1010-1047): wall-normal faces are
+ * never modified.  "Minus" side = phi <= 0 (air in air_bubble_rising,
+ * PROJECTION_2D.h:330-343).
+ *
+ * Pointers are host or device memory as selected by `mem`.  A context is not
+ * thread-safe: call it from one thread (the reference runs the solve on thread 0
+ * only, PROJECTION_2D.h:250-254).  Every call is ordered on the context's stream
+ * and returns after its results are available.
+ */
+#ifndef P2D_H
+#define P2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the reference has none: it exit()s or returns silently, see
+ * LINEAR_SOLVER.cpp:12-40, CG_METHOD.cpp:38-73, LEVELSET_2D.h:869-872). */
+enum {
+  P2D_OK = 0,
+  P2D_NOT_CONVERGED = 1, /* iteration caps reached before the tolerances */
+  P2D_NAN = 2,           /* NaN/Inf or a breakdown (p.Ap <= 0) in the solve, or a zero level-set gradient */
+  P2D_BAD_ARG = 3,       /* invalid sizes, pointers or options */
+  P2D_HIP_ERROR = 4,     /* a HIP runtime call failed */
+  P2D_NO_DEVICE = 5      /* no usable gfx950 device */
+};
+
+enum { P2D_MEM_HOST = 0, P2D_MEM_DEVICE = 1 };
+enum { P2D_FP64 = 0, P2D_MIXED = 1 }; /* solver precision */
+enum { P2D_PREC_F64 = 0, P2D_PREC_F32 = 1 }; /* element type of test-hook buffers */
+
+typedef struct p2d_config {
+  int nx, ny;          /* cells per side (>= 2) */
+  double h;            /* cell size */
+  int precision;       /* P2D_FP64: FP64 MG-PCG; P2D_MIXED: FP32 MG-PCG inside FP64 refinement */
+  int mg_max_levels;   /* 0: coarsen until <= 64 cells or an odd size */
+  int mg_pre_sweeps;   /* red-black Gauss-Seidel sweeps before restriction (default 2) */
+  int mg_post_sweeps;  /* ... after prolongation (default 2; equal counts keep M symmetric) */
+  double rtol;         /* stop when ||r||_2 <= rtol ||b||_2 (0 disables); PROJECTION_2D's `tolerance` */
+  double div_tol;      /* stop when the predicted max|D u_proj| <= div_tol (0 disables) */
+  int max_iterations;  /* cap on inner Krylov iterations per solve (PROJECTION_2D's `max_iteration`) */
+  int max_outer;       /* cap on FP64 refinement passes */
+  double inner_rtol;   /* mixed: relative residual reduction of each FP32 inner solve */
+} p2d_config;
+
+typedef struct p2d_stats {
+  int status;               /* P2D_* code of the solve */
+  int iterations;           /* inner Krylov iterations, all passes */
+  int outer;                /* FP64 refinement passes */
+  double rel_residual;      /* final true ||b - A p||_2 / ||b||_2 (FP64) */
+  double max_abs_residual;  /* final true ||b - A p||_inf (FP64, unscaled operator) */
+  double max_div;           /* max |D u_proj| over cells, FP64 (after the update) */
+  double max_u, max_v;      /* max |u|, |v| after the update (PROJECTION_2D max_velocity_x/y) */
+} p2d_stats;
+
+typedef struct p2d_ctx p2d_ctx;
+
+/* Fill `cfg` with defaults for an nx x ny grid of spacing h. */
+void p2d_config_default(p2d_config* cfg, int nx, int ny, double h);
+
+/* PROJECTION_2D constructor + InitializeFromBlock (PROJECTION_2D.h:107-232):
+ * allocates every device buffer for the grid once. */
+int p2d_create(const p2d_config* cfg, p2d_ctx** out);
+int p2d_destroy(p2d_ctx* ctx);
+
+/* Run on a caller-owned HIP stream (hipStream_t); NULL restores the context's own. */
+int p2d_set_stream(p2d_ctx* ctx, void* hip_stream);
+
+/* Fluid state, the fields the caller writes before Solve (EULERIAN_FLUID_SOLVER_2D.h:506-515):
+ * water_levelset phi (nx*ny), air_density = rho_minus (phi <= 0),
+ * water_density = rho_plus (phi > 0), surface_tension sigma.  Face beta = 1/rho_f
+ * (POISSON_SOLVER.cpp:743-746) and the jump sigma*kappa (PROJECTION_2D.h:830-834)
+ * are rebuilt from these inside every p2d_project. */
+int p2d_set_fluid(p2d_ctx* ctx, const double* phi, double rho_minus, double rho_plus, double sigma, int mem);
+
+/* Explicit coefficients instead of (phi, rho, sigma): face beta (walls ignored),
+ * the level set (sides of the jump) and the jump on the solution jc (may be NULL = 0).
+ * Used by the adjoint's beta-gradient checks; p2d_project then skips assembly. */
+int p2d_set_coefficients(p2d_ctx* ctx, const double* beta_x, const double* beta_y, const double* phi,
+                         const double* jc, int mem);
+
+/* PROJECTION_2D::Solve(dt): u, v projected in place, pressure kept in the context. */
+int p2d_project(p2d_ctx* ctx, double* u, double* v, double dt, p2d_stats* stats, int mem);
+
+/* pressure_field after the last projection.  mean_zero = 1: mean removed;
+ * 0: shifted so p(0,0) = 0, the reference's pin (POISSON_SOLVER.cpp:789-807). */
+int p2d_get_pressure(p2d_ctx* ctx, double* p, int mean_zero, int mem);
+
+/* Coefficients built by the last p2d_project / p2d_set_coefficients (any may be NULL):
+ * beta_x ((nx+1)*ny), beta_y (nx*(ny+1)), jc (nx*ny) = sigma*kappa. */
+int p2d_get_coefficients(p2d_ctx* ctx, double* beta_x, double* beta_y, double* jc, int mem);
+
+/* Adjoint of the last projection (no reference counterpart).  Given g = dL/du_proj,
+ * dL/dv_proj on interior faces, returns dL/du_in, dL/dv_in and dL/dbeta_x, dL/dbeta_y
+ * (wall entries 0).  One extra solve with the same symmetric operator; only the forward
+ * pressure is kept from the forward pass.  Outputs may be NULL. */
+int p2d_adjoint(p2d_ctx* ctx, const double* gu, const double* gv, double* du, double* dv, double* dbeta_x,
+                double* dbeta_y, p2d_stats* stats, int mem);
+
+/* Test hook: y = A x with the consistent operator -D B G (no-flux walls, no pin,
+ * entries beta/h^2) of the current coefficients, in FP64 or FP32 (prec). */
+int p2d_apply_stencil(p2d_ctx* ctx, const void* x, void* y, int prec, int mem);
+
+/* Test hook: z = M r for the multigrid preconditioner of the current coefficients on the
+ * h^2-scaled operator (one V-cycle from z = 0), FP64 or FP32 (prec). */
+int p2d_vcycle(p2d_ctx* ctx, const void* r_scaled, void* z, int prec, int mem);
+
+/* Number of multigrid levels of the current configuration. */
+int p2d_mg_levels(p2d_ctx* ctx);
+
+/* Bench hook: average duration in ms of `reps` back-to-back launches of one kernel at
+ * the finest level, timed with HIP events on the context's stream.
+ * which: 0 = FP32 fused direction+stencil (the Krylov SpMV), 1 = FP64 of the same,
+ *        2 = FP32 red-black smoothing half-sweep, 3 = FP64 true residual. */
+int p2d_time_kernel(p2d_ctx* ctx, int which, int reps, double* avg_ms);
+
+const char* p2d_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* P2D_H */

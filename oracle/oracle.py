@@ -17,8 +17,8 @@ Parity pins (how each part is checked, see tests/test_oracle_*.py):
                    static drop (p_in - p_out = -a exactly, u_proj = 0)
   Krylov solvers   CG, IC(0)-PCG, Jacobi-PCG and MG-PCG agree on the mean-zero
                    solution; IC(0) factor reproduces A on its pattern
-  MG V-cycle       symmetry <Mx,y> = <x,My>, constant-mode exactness, two-grid
-                   convergence on the constant-coefficient Poisson problem
+  MG V-cycle       symmetry <Mx,y> = <x,My>, constants in its null space, bounded
+                   spectrum of M A and iteration counts (constant and 1000:1 coefficients)
   adjoint          central finite differences and the dot-product identity
 """
 import ctypes
@@ -66,7 +66,7 @@ def lib():
             "orc_update_consistent": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
             "orc_update_reference_literal": (None, [_i, _i, _d, _d, _pd, _pd, _d, _d, _pd, _pd, _pd]),
             "orc_mg_num_levels": (_i, [_i, _i, _i, _i]),
-            "orc_mg_create": (_vp, [_i, _i, _pd, _pd, _d, _i, _i, _i, _i, _i]),
+            "orc_mg_create": (_vp, [_i, _i, _pd, _pd, _i, _i, _i, _i, _i]),
             "orc_mg_destroy": (None, [_vp]),
             "orc_mg_levels": (_i, [_vp]),
             "orc_mg_vcycle": (None, [_vp, _pd, _pd]),
@@ -257,11 +257,11 @@ MG_DEFAULT_MODE = 3
 
 
 class MG:
-    def __init__(self, bx, by, pin, max_levels=0, bottom_max=64, nu_pre=2, nu_post=2, mode=MG_DEFAULT_MODE):
+    def __init__(self, bx, by, max_levels=0, bottom_max=64, nu_pre=2, nu_post=2, mode=MG_DEFAULT_MODE):
         self.bx, self.by = _f64(bx), _f64(by)
         ny, nx1 = self.bx.shape
         self.nx, self.ny = nx1 - 1, ny
-        self.h = lib().orc_mg_create(self.nx, self.ny, _p(self.bx), _p(self.by), pin, max_levels, bottom_max,
+        self.h = lib().orc_mg_create(self.nx, self.ny, _p(self.bx), _p(self.by), max_levels, bottom_max,
                                      nu_pre, nu_post, mode)
         if not self.h:
             raise RuntimeError("singular bottom operator")

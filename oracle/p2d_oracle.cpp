@@ -696,11 +696,13 @@ void orc_update_reference_literal(int nx, int ny, double h, double dt, const dou
 // MULTIGRID_METHOD is constant-coefficient and never wired into the projection).
 // CPU mirror of the GPU V-cycle, operation for operation, for parity of M r.
 //
-// Level operator (h^2-scaled): (A' x)_c = sum_f beta_f (x_c - x_nb) + [c==(0,0)] pin * x_c.
+// Level operator (h^2-scaled): (A' x)_c = sum_f beta_f (x_c - x_nb), singular on constants
+// (no-flux walls, no pin on any level).
 // Coarse faces: beta_H = (beta_a + beta_b)/2 over the two fine faces on the coarse face.
 // P = cell-centred bilinear (9,3,3,1)/16 with constant extension at walls; R = P^T.
 // Smoother: red-black Gauss-Seidel, pre (red, black) x nu, post (black, red) x nu.
-// Bottom: exact dense solve of the pinned coarsest operator.
+// Bottom: pseudo-inverse of the coarsest operator (constants in its null space), so the
+// V-cycle treats the constant mode exactly as the singular fine operator does.
 
 constexpr int MG_DEFAULT_MODE = 3;
 struct MGLevel {
@@ -710,17 +712,14 @@ struct MGLevel {
 };
 struct MG {
   std::vector<MGLevel> L;
-  double pin;
   int nu_pre, nu_post;
   int mode;                        // bit0: harmonic-normal coarse beta; bit1: operator-dependent P
   std::vector<double> bottom_inv;  // dense inverse of the coarsest operator
 };
 
-static double diag_at(const MGLevel& l, int i, int j, double pin) {
+static double diag_at(const MGLevel& l, int i, int j) {
   const int nx = l.nx;
-  double d = l.bx[FX(i, j, nx)] + l.bx[FX(i + 1, j, nx)] + l.by[FY(i, j, nx)] + l.by[FY(i, j + 1, nx)];
-  if (i == 0 && j == 0) d += pin;
-  return d;
+  return l.bx[FX(i, j, nx)] + l.bx[FX(i + 1, j, nx)] + l.by[FY(i, j, nx)] + l.by[FY(i, j + 1, nx)];
 }
 static double offsum(const MGLevel& l, const std::vector<double>& x, int i, int j) {
   const int nx = l.nx, ny = l.ny;
@@ -731,17 +730,17 @@ static double offsum(const MGLevel& l, const std::vector<double>& x, int i, int 
   if (j + 1 < ny) s += l.by[FY(i, j + 1, nx)] * x[C(i, j + 1, nx)];
   return s;
 }
-static void rb_sweep(MGLevel& l, int color, double pin) {
+static void rb_sweep(MGLevel& l, int color) {
   for (int j = 0; j < l.ny; ++j)
     for (int i = 0; i < l.nx; ++i) {
       if (((i + j) & 1) != color) continue;
-      l.x[C(i, j, l.nx)] = (l.r[C(i, j, l.nx)] + offsum(l, l.x, i, j)) / diag_at(l, i, j, pin);
+      l.x[C(i, j, l.nx)] = (l.r[C(i, j, l.nx)] + offsum(l, l.x, i, j)) / diag_at(l, i, j);
     }
 }
-static void residual(MGLevel& l, double pin) {
+static void residual(MGLevel& l) {
   for (int j = 0; j < l.ny; ++j)
     for (int i = 0; i < l.nx; ++i)
-      l.res[C(i, j, l.nx)] = l.r[C(i, j, l.nx)] - (diag_at(l, i, j, pin) * l.x[C(i, j, l.nx)] - offsum(l, l.x, i, j));
+      l.res[C(i, j, l.nx)] = l.r[C(i, j, l.nx)] - (diag_at(l, i, j) * l.x[C(i, j, l.nx)] - offsum(l, l.x, i, j));
 }
 // Interpolation weights.  Fine cell i (1-D) has parent P = i/2 and neighbour
 // Q = P +- 1 (towards the side of i inside its parent; clamped at walls).  The
@@ -879,10 +878,9 @@ static double coarse_face(const double* b, int stride, int k, int nf) {
   return 2.0 / r;
 }
 
-void* orc_mg_create(int nx, int ny, const double* bx, const double* by, double pin, int max_levels, int bottom_max,
-                    int nu_pre, int nu_post, int mode) {
+void* orc_mg_create(int nx, int ny, const double* bx, const double* by, int max_levels, int bottom_max, int nu_pre,
+                    int nu_post, int mode) {
   MG* mg = new MG;
-  mg->pin = pin;
   mg->nu_pre = nu_pre;
   mg->nu_post = nu_post;
   mg->mode = mode;
@@ -923,20 +921,28 @@ void* orc_mg_create(int nx, int ny, const double* bx, const double* by, double p
         }
     }
   }
-  // Bottom: dense operator -> inverse.
+  // Bottom: pseudo-inverse of the singular (no-flux) coarsest operator,
+  // A+ = (A + a 1 1^T)^-1 - 1 1^T / (a n^2), with a n = mean diagonal.
   MGLevel& b = mg->L.back();
   const int n = b.nx * b.ny;
   std::vector<double> A((size_t)n * n, 0.0);
+  double dsum = 0;
+  for (int j = 0; j < b.ny; ++j)
+    for (int i = 0; i < b.nx; ++i) dsum += diag_at(b, i, j);
+  const double alpha = dsum / n / n;
+  for (size_t q = 0; q < A.size(); ++q) A[q] = alpha;
   for (int j = 0; j < b.ny; ++j)
     for (int i = 0; i < b.nx; ++i) {
       const int c = (int)C(i, j, b.nx);
-      A[(size_t)c * n + c] = diag_at(b, i, j, pin);
-      if (i > 0) A[(size_t)c * n + c - 1] = -b.bx[FX(i, j, b.nx)];
-      if (i + 1 < b.nx) A[(size_t)c * n + c + 1] = -b.bx[FX(i + 1, j, b.nx)];
-      if (j > 0) A[(size_t)c * n + c - b.nx] = -b.by[FY(i, j, b.nx)];
-      if (j + 1 < b.ny) A[(size_t)c * n + c + b.nx] = -b.by[FY(i, j + 1, b.nx)];
+      A[(size_t)c * n + c] += diag_at(b, i, j);
+      if (i > 0) A[(size_t)c * n + c - 1] -= b.bx[FX(i, j, b.nx)];
+      if (i + 1 < b.nx) A[(size_t)c * n + c + 1] -= b.bx[FX(i + 1, j, b.nx)];
+      if (j > 0) A[(size_t)c * n + c - b.nx] -= b.by[FY(i, j, b.nx)];
+      if (j + 1 < b.ny) A[(size_t)c * n + c + b.nx] -= b.by[FY(i, j + 1, b.nx)];
     }
   if (!dense_inverse(n, A)) { delete mg; return nullptr; }
+  const double corr = 1.0 / (alpha * n * n);
+  for (size_t q = 0; q < A.size(); ++q) A[q] -= corr;
   mg->bottom_inv.swap(A);
   return mg;
 }
@@ -956,13 +962,13 @@ static void vcycle(MG* mg, int l) {
     return;
   }
   std::fill(lv.x.begin(), lv.x.end(), 0.0);
-  for (int s = 0; s < mg->nu_pre; ++s) { rb_sweep(lv, 0, mg->pin); rb_sweep(lv, 1, mg->pin); }
-  residual(lv, mg->pin);
+  for (int s = 0; s < mg->nu_pre; ++s) { rb_sweep(lv, 0); rb_sweep(lv, 1); }
+  residual(lv);
   MGLevel& cv = mg->L[l + 1];
   restrict_(lv, cv);
   vcycle(mg, l + 1);
   prolong_add(lv, cv);
-  for (int s = 0; s < mg->nu_post; ++s) { rb_sweep(lv, 1, mg->pin); rb_sweep(lv, 0, mg->pin); }
+  for (int s = 0; s < mg->nu_post; ++s) { rb_sweep(lv, 1); rb_sweep(lv, 0); }
 }
 
 // z = M r_scaled: one V-cycle on A' x = r' (r' = h^2 r for the fine operator).
@@ -980,14 +986,18 @@ void orc_mg_vcycle(void* h, const double* rs, double* z) {
 int orc_mg_pcg(int nx, int ny, double h, const double* bx, const double* by, double* x, const double* b, double rtol,
                double atol, int max_iteration, int max_levels, int nu, int mode, double* out_res2, double* out_resinf) {
   const int N = nx * ny;
-  const double pin = bx[FX(1, 0, nx)] + by[FY(0, 1, nx)];
   std::vector<double> bxs(bx, bx + (size_t)(nx + 1) * ny), bys(by, by + (size_t)nx * (ny + 1));
-  void* mg = orc_mg_create(nx, ny, bxs.data(), bys.data(), pin, max_levels, 64, nu, nu, mode);
+  void* mg = orc_mg_create(nx, ny, bxs.data(), bys.data(), max_levels, 64, nu, nu, mode);
   if (!mg) return -1;
-  std::vector<double> r(N), z(N), p(N), q(N), rs(N);
+  std::vector<double> r(N), z(N), p(N), q(N), rs(N), bm(b, b + N);
+  // A0 is singular (constants); solve with the range component of b, i.e. b minus its mean.
+  double mean = 0;
+  for (int k = 0; k < N; ++k) mean += b[k];
+  mean /= N;
+  for (int k = 0; k < N; ++k) bm[k] -= mean;
   orc_apply_A0(nx, ny, h, bx, by, x, q.data());
-  for (int k = 0; k < N; ++k) r[k] = b[k] - q[k];
-  const double bn = std::sqrt(dot(N, b, b));
+  for (int k = 0; k < N; ++k) r[k] = bm[k] - q[k];
+  const double bn = std::sqrt(dot(N, bm.data(), bm.data()));
   auto rinf = [&]() { double m = 0; for (int k = 0; k < N; ++k) m = std::max(m, std::fabs(r[k])); return m; };
   auto conv = [&]() {
     const bool ok2 = rtol <= 0 || std::sqrt(dot(N, r.data(), r.data())) <= rtol * bn;
@@ -999,6 +1009,10 @@ int orc_mg_pcg(int nx, int ny, double h, const double* bx, const double* by, dou
   while (!conv() && it < max_iteration) {
     for (int k = 0; k < N; ++k) rs[k] = h * h * r[k];
     orc_mg_vcycle(mg, rs.data(), z.data());
+    double zm = 0;  // drop the preconditioner's constant (null-space) component
+    for (int k = 0; k < N; ++k) zm += z[k];
+    zm /= N;
+    for (int k = 0; k < N; ++k) z[k] -= zm;
     const double rz = dot(N, r.data(), z.data());
     if (it == 0) p = z;
     else {

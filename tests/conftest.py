@@ -1,0 +1,31 @@
+import importlib
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP library's C ABI)")
+
+
+@pytest.fixture(scope="session")
+def O():
+    import oracle
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def W():
+    return importlib.import_module("2d_engine_amd.workloads")
+
+
+@pytest.fixture(scope="session")
+def P2D():
+    """The product package; loading it fails loudly if libp2d.so is not built."""
+    return importlib.import_module("2d_engine_amd")

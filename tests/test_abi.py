@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every symbol include/p2d.h declares (no GPU needed)."""
+import ctypes
+import os
+import re
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "p2d.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(p2d_[a-z_0-9]+)\s*\(", src)))
+
+
+def test_header_symbols_exported():
+    lib = os.path.join(ROOT, "2d_engine_amd", "libp2d.so")
+    assert os.path.exists(lib), "build first: __graft_entry__.build()"
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (p2d_\w+)", out))
+    declared = _declared()
+    assert len(declared) >= 15
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    L = ctypes.CDLL(lib)
+    for s in declared:
+        getattr(L, s)
+
+
+def test_binding_matches_header(P2D):
+    from importlib import import_module
+    N = import_module("2d_engine_amd._native")
+    assert sorted(N.EXPORTS) == _declared()
+
+
+def test_bad_arguments_rejected_without_compute(P2D):
+    N = P2D._native
+    cfg = N.default_config(1, 8, 0.1)          # nx < 2
+    h = ctypes.c_void_p()
+    assert N.lib.p2d_create(ctypes.byref(cfg), ctypes.byref(h)) == N.P2D_BAD_ARG
+    cfg = N.default_config(8, 8, 0.125)
+    cfg.rtol = 0.0
+    cfg.div_tol = 0.0                           # no stopping rule
+    assert N.lib.p2d_create(ctypes.byref(cfg), ctypes.byref(h)) == N.P2D_BAD_ARG
+    cfg = N.default_config(8, 8, 0.125)
+    cfg.inner_rtol = 1.5
+    assert N.lib.p2d_create(ctypes.byref(cfg), ctypes.byref(h)) == N.P2D_BAD_ARG
+    assert N.status_string(N.P2D_NAN) == "NaN or breakdown"
+
+
+def test_status_codes_match_header():
+    src = open(os.path.join(ROOT, "include", "p2d.h")).read()
+    codes = dict(re.findall(r"(P2D_[A-Z_]+)\s*=\s*(\d+)", src))
+    from importlib import import_module
+    N = import_module("2d_engine_amd._native")
+    for k in ("P2D_OK", "P2D_NOT_CONVERGED", "P2D_NAN", "P2D_BAD_ARG", "P2D_HIP_ERROR", "P2D_NO_DEVICE"):
+        assert int(codes[k]) == getattr(N, k)

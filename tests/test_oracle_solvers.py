@@ -1,0 +1,127 @@
+"""Oracle solvers and the multigrid mirror, pinned by agreement and by properties.
+
+The converged pressure is fixed by the linear system up to a constant, so the
+restatements of CG (CG_METHOD.cpp:13-73), IC(0)-PCG (PCG_METHOD.cpp:176-264),
+Jacobi-PCG (PCG_METHOD.cpp:266-383) and the MG-PCG of this library must agree
+on the mean-zero pressure.  The V-cycle must be a symmetric positive-definite
+linear operator that reproduces constants, and must converge in a bounded
+number of iterations on the constant-coefficient problem.
+"""
+import numpy as np
+import pytest
+
+
+def _system(O, w):
+    rho, inv = O.density(w.phi, w.rho_minus, w.rho_plus)
+    div = O.divergence(w.u, w.v, w.h)
+    jc = w.sigma * O.curvature(w.phi, w.h)
+    return O.assemble_reference(w.phi, inv, div, w.h, jc=jc), jc
+
+
+def test_solvers_agree_on_mean_zero_pressure(O, W):
+    w = W.circle(64)
+    (A, b, x0, _), jc = _system(O, w)
+    x_ic0, it0, _ = O.pcg_ic0(A, b, w.n, w.n, tol=1e-13)
+    x_cg, it1, _ = O.cg(A, b, tol=1e-13)
+    x_jac, it2, _ = O.pcg_jacobi(A, b, tol=1e-13, threads=2)
+    bx, by = O.face_beta(w.phi, w.rho_minus, w.rho_plus)
+    rhs = O.rhs_consistent(w.phi, jc, bx, by, w.u, w.v, w.h)
+    x_mg, it3, _, _ = O.mg_pcg(bx, by, rhs, w.h, rtol=1e-13)
+    ref = x_ic0 - x_ic0.mean()
+    for x in (x_cg, x_jac, x_mg.ravel()):
+        xm = x - x.mean()
+        assert np.linalg.norm(xm - ref) <= 1e-10 * np.linalg.norm(ref)
+    assert it3 < it0 < it2 < it1      # MG < IC(0) < Jacobi < CG iterations
+    assert abs(x_ic0[0]) < 1e-6 * np.abs(x_ic0).max()   # the pin holds p(0,0) near 0
+
+
+def test_reference_absolute_stopping_rule(O, W):
+    """mode 0: CG stops at r.r < tol^2 (CG_METHOD.cpp:52); PCG at r.z < tol^2 (PCG_METHOD.cpp:253-254)."""
+    w = W.circle(32)
+    (A, b, _, _), _ = _system(O, w)
+    x, it, res = O.cg(A, b, tol=1e-6, relative=False)
+    assert res < 1e-6
+    r = b - A.matvec(x)
+    assert np.linalg.norm(r) < 1e-5
+    x, it, res = O.pcg_ic0(A, b, 32, 32, tol=1e-6, relative=False, max_iteration=3)
+    assert it == 4   # "if(num_iteration > max_iteration) break" runs max_iteration+1 iterations
+
+
+@pytest.fixture(scope="module")
+def mg_case(O, W):
+    w = W.make("drops64", 64)
+    bx, by = O.face_beta(w.phi, w.rho_minus, w.rho_plus)
+    return w, bx, by, O.MG(bx, by)
+
+
+def _dense_A_scaled(bx, by):
+    ny, nx = bx.shape[0], bx.shape[1] - 1
+    n = nx * ny
+    A = np.zeros((n, n))
+    for j in range(ny):
+        for i in range(nx):
+            c = i + j * nx
+            d = bx[j, i] + bx[j, i + 1] + by[j, i] + by[j + 1, i]
+            A[c, c] = d
+            if i > 0: A[c, c - 1] = -bx[j, i]
+            if i + 1 < nx: A[c, c + 1] = -bx[j, i + 1]
+            if j > 0: A[c, c - nx] = -by[j, i]
+            if j + 1 < ny: A[c, c + nx] = -by[j + 1, i]
+    return A
+
+
+def test_vcycle_symmetric_positive_linear(O, mg_case):
+    w, bx, by, mg = mg_case
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal((2, w.n, w.n))
+    x -= x.mean()
+    y -= y.mean()
+    Mx, My = mg.vcycle(x), mg.vcycle(y)
+    assert abs(np.vdot(Mx, y) - np.vdot(x, My)) <= 1e-12 * abs(np.vdot(Mx, y))
+    assert np.vdot(x, Mx) > 0 and np.vdot(y, My) > 0
+    assert np.linalg.norm(mg.vcycle(2 * x - 3 * y) - (2 * Mx - 3 * My)) <= 1e-12 * np.linalg.norm(Mx)
+
+
+def test_preconditioned_spectrum(O, W):
+    """On mean-zero vectors M is SPD and M A' has a bounded spread; constants are in M's null space."""
+    w = W.make("drops64", 32)
+    bx, by = O.face_beta(w.phi, w.rho_minus, w.rho_plus)
+    A = _dense_A_scaled(bx, by)
+    mg = O.MG(bx, by)
+    n = A.shape[0]
+    M = np.stack([mg.vcycle(np.eye(n)[k].reshape(w.n, w.n)).ravel() for k in range(n)], axis=1)
+    assert np.abs(M - M.T).max() <= 1e-12 * np.abs(M).max()
+    Q = np.eye(n) - 1.0 / n                       # projector onto mean-zero vectors
+    MQ = Q @ M @ Q
+    # the null-space mode is not amplified beyond the rest of the spectrum (a pinned
+    # coarse solve would scale it by ~N / pin)
+    one = np.ones(n)
+    assert np.linalg.norm(M @ one) / np.sqrt(n) <= np.linalg.norm(MQ, 2)
+    ev = np.linalg.eigvalsh(0.5 * (MQ + MQ.T))
+    assert ev[1] > 0                               # positive definite on the complement of constants
+    lam = np.sort(np.real(np.linalg.eigvals(MQ @ A)))[1:]   # drop the zero of the null space
+    assert lam[0] > 0.3 and lam[-1] / lam[0] < 40, (lam[0], lam[-1])
+
+
+def test_mg_pcg_iterations_constant_coefficient(O):
+    n, h = 128, 1.0 / 128
+    bx = np.ones((n, n + 1)); bx[:, 0] = bx[:, -1] = 0
+    by = np.ones((n + 1, n)); by[0] = by[-1] = 0
+    rng = np.random.default_rng(2)
+    b = rng.standard_normal((n, n)); b -= b.mean()
+    x, it, r2, _ = O.mg_pcg(bx, by, b, h, rtol=1e-10)
+    assert it <= 12, it
+    assert np.linalg.norm(b - O.apply_A0(bx, by, x, h)) <= 1e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("name,n,cap", [("circle", 128, 20), ("rt", 128, 20), ("drops64", 256, 40)])
+def test_mg_pcg_iterations_high_contrast(O, W, name, n, cap):
+    """1000:1 density: the operator-dependent interpolation keeps iteration counts bounded."""
+    w = W.make(name, n)
+    bx, by = O.face_beta(w.phi, w.rho_minus, w.rho_plus)
+    jc = w.sigma * O.curvature(w.phi, w.h)
+    b = O.rhs_consistent(w.phi, jc, bx, by, w.u, w.v, w.h)
+    x, it, _, _ = O.mg_pcg(bx, by, b, w.h, rtol=1e-12)
+    assert it <= cap, it
+    x2, it2, _, _ = O.mg_pcg(bx, by, b, w.h, rtol=1e-12, mode=0)   # plain averaging + bilinear
+    assert it2 > it
