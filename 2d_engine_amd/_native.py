@@ -20,10 +20,11 @@ FP64, MIXED = 0, 1
 PREC_F64, PREC_F32 = 0, 1
 
 # Kernels p2d_time_kernel can time (finest level).
-KERNEL_DIR_APPLY_F32 = 0
-KERNEL_DIR_APPLY_F64 = 1
-KERNEL_GS_F32 = 2
-KERNEL_CORRECT_RESID = 3
+KERNEL_DIR_F32 = 0
+KERNEL_DIR_F64 = 1
+KERNEL_PRE0_F32 = 2
+KERNEL_POST0_F32 = 3
+KERNEL_CORRECT_RESID = 4
 
 EXPORTS = [
     "p2d_config_default", "p2d_create", "p2d_destroy", "p2d_set_stream", "p2d_set_fluid",

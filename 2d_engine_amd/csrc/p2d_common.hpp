@@ -1,0 +1,251 @@
+// p2d device code for gfx950 (CDNA4), shared pieces: layout, solver state,
+// deterministic FP64 reductions, ghost-fluid and double-double helpers.
+//
+// Layout.  Every field the kernels touch is cell-pitched: nx x ny, i fastest,
+// cell (i,j) at i + j*nx.  Face coefficients too: bw(i,j) is beta on the WEST
+// face of cell (i,j) (the x-face between (i-1,j) and (i,j)), bs(i,j) on its
+// SOUTH face; the wall faces are implicit zeros (bw(0,j) = bs(i,0) = 0 are
+// stored, the east/north walls are not).  So a stencil row reads aligned,
+// equal-pitch arrays only.  The C ABI's face layout ((nx+1) x ny and
+// nx x (ny+1)) is converted at the boundary.
+//
+// Solver operators are h^2-scaled: (A' x)_c = sum_f beta_f (x_c - x_nb),
+// A' = h^2 A with A = -D B G of the reference's variable-coefficient matrix
+// (POISSON_SOLVER.cpp:749-872) without its pin: A' is singular on constants on
+// every level, the right-hand side is taken mean-free and the coarsest level is
+// solved with a pseudo-inverse.
+//
+// Reductions accumulate in FP64 and are deterministic: 64-lane wave shuffles,
+// a fixed-order block tree, then the last block to finish (atomic ticket) sums
+// the per-block partials in block order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace p2d {
+
+constexpr int BLOCK = 256;          // 4 wavefronts of 64
+constexpr int BAND = 16;            // rows per block of the band (row-marching) kernels
+constexpr int SMALL_THREADS = 1024; // single-workgroup coarse V-cycle
+constexpr int RED_K = 4;            // most quantities any kernel reduces
+
+__device__ __forceinline__ size_t ci(int i, int j, int nx) { return (size_t)i + (size_t)j * nx; }
+__device__ __forceinline__ size_t fxi(int i, int j, int nx) { return (size_t)i + (size_t)j * (nx + 1); }
+__device__ __forceinline__ size_t fyi(int i, int j, int nx) { return (size_t)i + (size_t)j * nx; }
+
+// Face coefficients of cell (i,j) in the cell-pitched layout.
+template <class T>
+struct Coef {
+  const T* bw;
+  const T* bs;
+  int nx, ny;
+  __device__ __forceinline__ T W(int i, int j) const { return bw[ci(i, j, nx)]; }
+  __device__ __forceinline__ T E(int i, int j) const { return i + 1 < nx ? bw[ci(i + 1, j, nx)] : (T)0; }
+  __device__ __forceinline__ T S(int i, int j) const { return bs[ci(i, j, nx)]; }
+  __device__ __forceinline__ T N(int i, int j) const { return j + 1 < ny ? bs[ci(i, j + 1, nx)] : (T)0; }
+};
+
+__device__ __forceinline__ int nblocks() { return (int)(gridDim.x * gridDim.y); }
+__device__ __forceinline__ int block_id() { return (int)(blockIdx.y * gridDim.x + blockIdx.x); }
+
+// Device-resident solver state.  Kernels of an inner iteration read `done` on
+// entry and return at once when it is set, so the host can enqueue iterations
+// ahead without waiting for the convergence test.
+struct Scal {
+  double rz, rz_old, pq, alpha, beta;
+  double r2, rinf, r0_2;
+  double rel2;    // stop when r2 <= rel2 * r0_2
+  double abs2;    // ... or when r2 <= abs2 and rinf <= ainf
+  double ainf;
+  double bmean;   // mean of the right-hand side (removed: A0 is singular on constants)
+  double zmean;   // mean of the preconditioned residual (removed: keeps iterates mean-free)
+  double rshift;  // mean of a replaced residual, removed by the next fused update (its constant
+                  // component is roundoff that no iterate can reduce: A0 p has zero sum)
+  double red[8];  // results of stand-alone reductions
+  int iter, done, max_iter, flag;  // flag: 1 = breakdown / NaN
+  int pending;                     // 1: a direction and alpha await the update x += a p, r -= a q
+  int rpar;                        // which of the two residual buffers holds the current residual
+  int err;                         // coefficient assembly error (zero level-set gradient)
+  int seg0;                        // iteration at which the current direction sequence started (beta = 0)
+};
+
+// --------------------------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Block reduction; bit k of maxmask selects max (else sum).  Result in thread 0.
+template <int K, int NT>
+__device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
+  __shared__ double sh[K][NT / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = ((maxmask >> k) & 1) ? wave_max(v[k]) : wave_sum(v[k]);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[k][w] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double a = sh[k][0];
+      for (int q = 1; q < NT / 64; ++q) a = ((maxmask >> k) & 1) ? fmax(a, sh[k][q]) : a + sh[k][q];
+      v[k] = a;
+    }
+  }
+}
+
+// Grid reduction over all blocks of a (BLOCK-thread) launch.  Returns true in the
+// last block; thread 0 of that block holds the totals in v.  partial must hold
+// K * nblocks() doubles; counter starts at 0 and is left at 0.
+template <int K>
+__device__ bool grid_reduce(double (&v)[K], unsigned maxmask, double* partial, unsigned* counter) {
+  block_reduce<K, BLOCK>(v, maxmask);
+  __shared__ int last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) partial[(size_t)k * nblocks() + block_id()] = v[k];
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    last = (t == (unsigned)nblocks() - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  double a[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = 0.0;
+  const volatile double* vp = partial;
+  const int G = nblocks();
+  for (int b = threadIdx.x; b < G; b += BLOCK) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double x = vp[(size_t)k * G + b];
+      a[k] = ((maxmask >> k) & 1) ? fmax(a[k], x) : a[k] + x;
+    }
+  }
+  block_reduce<K, BLOCK>(a, maxmask);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = a[k];
+    *counter = 0;
+  }
+  return true;
+}
+
+// --------------------------------------------------------------------------- ghost-fluid pieces
+__device__ __forceinline__ bool minus_side(double phi) { return phi <= 0.0; }
+
+// POISSON_SOLVER.cpp:743-746 with (L,R) = (left/bottom, right/top) cell:
+// beta = vR vL (|phiL|+|phiR|) / (vR |phiL| + vL |phiR|); 0/0 -> vR.
+__device__ __forceinline__ double gfm_beta(double vL, double vR, double phiL, double phiR) {
+#pragma clang fp contract(off)
+  const double aL = fabs(phiL), aR = fabs(phiR);
+  const double den = vR * aL + vL * aR;
+  if (den == 0.0) return vR;
+  return vR * vL * (aL + aR) / den;
+}
+
+// POISSON_SOLVER.cpp:879: a = (jc_c |phi_n| + jc_n |phi_c|)/(|phi_c| + |phi_n|).
+__device__ __forceinline__ double face_jump(double jcL, double jcR, double phiL, double phiR) {
+#pragma clang fp contract(off)
+  const double aL = fabs(phiL), aR = fabs(phiR);
+  return (jcR * aL + jcL * aR) / (aR + aL);
+}
+
+// Jump part of the face gradient, s a (s = +1 if the R cell is minus side),
+// 0 when both cells lie on one side or jc is absent.
+__device__ __forceinline__ double face_sa(const double* phi, const double* jc, size_t kL, size_t kR) {
+#pragma clang fp contract(off)
+  if (!jc) return 0.0;
+  const double pL = phi[kL], pR = phi[kR];
+  if (minus_side(pL) == minus_side(pR)) return 0.0;
+  const double a = face_jump(jc[kL], jc[kR], pL, pR);
+  return minus_side(pR) ? a : -a;
+}
+
+// Double-double pressure (hi + lo).  The refinement loop keeps p in two FP64
+// words so that face differences (hi_R - hi_L) + (lo_R - lo_L) stay accurate
+// when p carries large levels (pressure of light drops embedded in a heavy
+// fluid); a single FP64 word would round them at ~eps |p| and floor max|D u|.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+  s = a + b;
+  const double bp = s - a;
+  const double ap = s - bp;
+  e = (a - ap) + (b - bp);
+}
+__device__ __forceinline__ void dd_add(double hi, double lo, double e, double& rh, double& rl) {
+#pragma clang fp contract(off)
+  double s, t;
+  two_sum(hi, e, s, t);
+  t += lo;
+  rh = s + t;
+  rl = t - (rh - s);
+}
+__device__ __forceinline__ double dd_diff(const double* hi, const double* lo, size_t kR, size_t kL) {
+#pragma clang fp contract(off)
+  return (hi[kR] - hi[kL]) + (lo[kR] - lo[kL]);
+}
+
+// Level set with linear-extrapolation ghosts (FIELD_STRUCTURE_2D.h:585-639);
+// corners extrapolated from the y ghosts along x.
+__device__ __forceinline__ double phi_y(const double* phi, int nx, int ny, int i, int j) {
+#pragma clang fp contract(off)
+  if (j < 0) return 2.0 * phi[ci(i, 0, nx)] - phi[ci(i, 1, nx)];
+  if (j >= ny) return 2.0 * phi[ci(i, ny - 1, nx)] - phi[ci(i, ny - 2, nx)];
+  return phi[ci(i, j, nx)];
+}
+__device__ __forceinline__ double phi_at(const double* phi, int nx, int ny, int i, int j) {
+#pragma clang fp contract(off)
+  if (i < 0) return 2.0 * phi_y(phi, nx, ny, 0, j) - phi_y(phi, nx, ny, 1, j);
+  if (i >= nx) return 2.0 * phi_y(phi, nx, ny, nx - 1, j) - phi_y(phi, nx, ny, nx - 2, j);
+  return phi_y(phi, nx, ny, i, j);
+}
+
+// Curvature, LEVELSET_2D.h:929-984: central differences, one-sided fallback,
+// clamp to +-1/h.  Sets *bad when both gradients vanish (reference exit(0)).
+__device__ double curvature_at(const double* phi, int nx, int ny, double h, int i, int j, bool* bad) {
+#pragma clang fp contract(off)
+  const double one_over_2dx = 1.0 / (2.0 * h), one_over_dx2 = 1.0 / (h * h), one_over_dx = 1.0 / h;
+  const double c = phi_at(phi, nx, ny, i, j);
+  const double l = phi_at(phi, nx, ny, i - 1, j), r = phi_at(phi, nx, ny, i + 1, j);
+  const double b = phi_at(phi, nx, ny, i, j - 1), t = phi_at(phi, nx, ny, i, j + 1);
+  const double rt = phi_at(phi, nx, ny, i + 1, j + 1), rb = phi_at(phi, nx, ny, i + 1, j - 1);
+  const double lt = phi_at(phi, nx, ny, i - 1, j + 1), lb = phi_at(phi, nx, ny, i - 1, j - 1);
+  double px = (r - l) * one_over_2dx;
+  double py = (t - b) * one_over_2dx;
+  const double pxx = (r - 2 * c + l) * one_over_dx2;
+  const double pyy = (t - 2 * c + b) * one_over_dx2;
+  const double pxy = (rt - rb - lt + lb) * one_over_2dx * one_over_2dx;
+  double mag = sqrt(px * px + py * py);
+  double deno = mag * mag * mag;
+  double curv;
+  if (deno != 0) {
+    curv = -(px * px * pyy - 2.0 * px * py * pxy + py * py * pxx) * (1.0 / deno);
+  } else {
+    px = (r - c) * one_over_dx;
+    py = (t - c) * one_over_dx;
+    mag = sqrt(px * px + py * py);
+    deno = mag * mag * mag;
+    if (deno == 0) {
+      *bad = true;
+      return 0.0;
+    }
+    curv = -(px * px * pyy - 2.0 * px * py * pxy + py * py * pxx) * (1.0 / deno);
+  }
+  const double tol = 1.0 / h;
+  if (fabs(curv) <= tol) return curv;
+  return curv > tol ? tol : -tol;
+}
+
+}  // namespace p2d

@@ -100,7 +100,7 @@ def main():
     ms = statistics.median(t_m)
     value = 1000.0 / ms
 
-    # dominant kernel: FP32 fused direction + stencil (k_dir_apply<float>) at the finest level.
+    # FP32 direction update + stencil (k_dir<float>) at the finest level.
     # Algorithmic bytes per launch: read z, p_old, beta_x, beta_y; write p_new, q = 6 * 4 * N.
     reps = 200
     kms = ctx_m.time_kernel(0, reps)
@@ -108,7 +108,7 @@ def main():
     achieved = kbytes / (kms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_dir_apply<float> (FP32 direction update + 5-point GFM stencil + p.q)",
+                "kernel": "k_dir<float> (FP32 direction update + 5-point GFM stencil + p.q)",
                 "bytes_per_launch": kbytes, "avg_launch_ms": kms}
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "projections/s", "n_gpus": 1,

@@ -131,8 +131,10 @@ int p2d_mg_levels(p2d_ctx* ctx);
 
 /* Bench hook: average duration in ms of `reps` back-to-back launches of one kernel at
  * the finest level, timed with HIP events on the context's stream.
- * which: 0 = FP32 fused direction+stencil (the Krylov SpMV), 1 = FP64 of the same,
- *        2 = FP32 red-black smoothing half-sweep, 3 = FP64 true residual. */
+ * which: 0 = FP32 direction update + stencil + p.q (k_dir<float>), 1 = FP64 of the same,
+ *        2 = FP32 fused Krylov update + pre-smoothing + restriction (k_mg_pre<float,true>),
+ *        3 = FP32 prolongation + post-smoothing + r.z (k_mg_post<float,true>),
+ *        4 = true residual in double-double with the FP32 correction (k_correct_resid). */
 int p2d_time_kernel(p2d_ctx* ctx, int which, int reps, double* avg_ms);
 
 const char* p2d_status_string(int status);
