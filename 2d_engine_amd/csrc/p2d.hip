@@ -46,6 +46,15 @@ inline dim3 tgrid(int nx, int ny, int H) {
   const int TX = 64 - 2 * H;
   return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + TY - 1) / TY));
 }
+// Direction kernel: 64 x DTY cells per block.
+inline dim3 dgrid(int nx, int ny) {
+  return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + DTY - 1) / DTY));
+}
+// Refinement tile kernel: 64 x FTY cells per block.
+inline dim3 fgrid(int nx, int ny) {
+  return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + FTY - 1) / FTY));
+}
+constexpr int MAX_PASSES = 32;
 inline size_t count(dim3 g) { return (size_t)g.x * g.y * g.z; }
 
 template <class T>
@@ -74,9 +83,11 @@ struct p2d_ctx {
   double h = 0;
   hipStream_t own = nullptr, stream = nullptr;
   std::vector<void*> allocs;
+  size_t lds_limit = 65536;  // dynamic LDS available to the single-workgroup V-cycle
 
   // fluid state and coefficients (FP64 masters, cell-pitched)
   double *phi = nullptr, *jc = nullptr, *bw = nullptr, *bs = nullptr;
+  double *saw = nullptr, *sas = nullptr;  // ghost-fluid jump s a on the west / south face of each cell
   bool have_fluid = false, explicit_coeff = false, has_jc = true, coeff_ready = false;
   double rho_m = 1, rho_p = 1, sigma = 0;
 
@@ -102,10 +113,9 @@ struct p2d_ctx {
   float* binv32 = nullptr;
 
   double* partial = nullptr;
-  unsigned* counter = nullptr;
   Scal* s = nullptr;
   Scal* hs = nullptr;  // pinned host mirror
-  int last_iters[2] = {0, 0};
+  int last_seg[2][MAX_PASSES] = {};  // iterations of each pass of the last solve (FP64, mixed)
 };
 
 namespace {
@@ -166,13 +176,21 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
   const int L = num_levels(c->nx, c->ny, c->cfg.mg_max_levels);
   mg.L = L;
   mg.lv.resize(L);
+  // l0: the first level of at most SMALL_MAX_CELLS cells whose single-workgroup
+  // V-cycle fits the LDS limit (per level x, r, bw, bs, wx, wy; 4 arrays on the
+  // bottom; one residual array of level l0)
+  auto small_bytes = [&](int a) {
+    size_t e = (size_t)(c->nx >> a) * (c->ny >> a);
+    for (int l = a; l < L; ++l) e += (size_t)(c->nx >> l) * (c->ny >> l) * (l + 1 < L ? 6 : 4);
+    return e * sizeof(T);
+  };
   mg.l0 = L - 1;
   for (int l = 1; l < L; ++l)
-    if ((long)(c->nx >> l) * (c->ny >> l) <= SMALL_MAX_CELLS) {
+    if ((long)(c->nx >> l) * (c->ny >> l) <= SMALL_MAX_CELLS && small_bytes(l) <= c->lds_limit) {
       mg.l0 = l;
       break;
     }
-  mg.small_lds = 0;
+  mg.small_lds = small_bytes(mg.l0);
   for (int l = 0; l < L; ++l) {
     Level<T>& d = mg.lv[l];
     d.nx = c->nx >> l;
@@ -201,7 +219,6 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
     d.xa = xa;
     d.xb = xb;
     d.r = r;
-    if (l >= mg.l0) mg.small_lds += 4 * n * sizeof(T);
   }
   CHK(dalloc(c, &mg.d_lv, (size_t)L));
   HIPCHK(hipMemcpy(mg.d_lv, mg.lv.data(), sizeof(Level<T>) * L, hipMemcpyHostToDevice));
@@ -274,126 +291,171 @@ void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const int nu1 = c->cfg.mg_pre_sweeps, nu2 = c->cfg.mg_post_sweeps;
   const int Hpre = 2 * nu1 + 1, Hpost = 2 * nu2;
   const Level<T>* lv = mg.lv.data();
-  k_mg_pre<T, true><<<tgrid(lv[0].nx, lv[0].ny, Hpre), BLOCK, 0, c->stream>>>(
-      lv[0], lv[1], nullptr, lv[0].xa, nu1, v.r[0], v.r[1], v.x, v.p[0], v.p[1], v.q, c->partial, c->counter, c->s);
+  const dim3 gpre = tgrid(lv[0].nx, lv[0].ny, Hpre), gpost = tgrid(lv[0].nx, lv[0].ny, Hpost);
+  k_mg_pre<T, true><<<gpre, TBLOCK, 0, c->stream>>>(lv[0], lv[1], nullptr, lv[0].xa, nu1, v.r[0], v.r[1], v.x,
+                                                    v.p[0], v.p[1], v.q, c->partial, c->s);
+  k_fin_pre0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpre), (double)c->N, c->s);
   for (int l = 1; l < mg.l0; ++l)
-    k_mg_pre<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpre), BLOCK, 0, c->stream>>>(
+    k_mg_pre<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpre), TBLOCK, 0, c->stream>>>(
         lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-        nullptr, c->s);
+        c->s);
   k_mg_small<T><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(mg.d_lv, mg.l0, mg.L, nu1, nu2, mg.binv, c->s);
   for (int l = mg.l0 - 1; l >= 1; --l)
-    k_mg_post<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpost), BLOCK, 0, c->stream>>>(
-        lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa, lv[l].xb, nu2, nullptr, nullptr, c->s);
-  k_mg_post<T, true><<<tgrid(lv[0].nx, lv[0].ny, Hpost), BLOCK, 0, c->stream>>>(
-      lv[0], lv[1], nullptr, v.r[0], v.r[1], lv[0].xa, v.z, nu2, c->partial, c->counter, c->s);
+    k_mg_post<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpost), TBLOCK, 0, c->stream>>>(
+        lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa, lv[l].xb, nu2, nullptr, c->s);
+  k_mg_post<T, true><<<gpost, TBLOCK, 0, c->stream>>>(lv[0], lv[1], nullptr, v.r[0], v.r[1], lv[0].xa, v.z, nu2,
+                                                      c->partial, c->s);
+  k_fin_post0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpost), (double)c->N, c->s);
 }
 
 template <class T>
 void cg_iteration(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   vcycle<T>(c, mg, v);
   const Level<T>& l0 = mg.lv[0];
-  k_dir<T><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, v.p[0], v.p[1], v.q,
-                                                          c->partial, c->counter, c->s);
+  const dim3 g = dgrid(c->nx, c->ny);
+  k_dir<T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, v.p[0], v.p[1], v.q, c->partial, c->s);
+  k_fin_dir<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), c->s);
+}
+
+// Residual launch plus its finish (sum r^2, max |r|, sum r into s->red[0..2]).
+template <class T>
+void correct_resid(p2d_ctx* c, const double* b, const double* saw, const double* sas, const double* ph,
+                   const double* pl, const T* e, double* qh, double* ql, T* r) {
+  const dim3 g = fgrid(c->nx, c->ny);
+  k_correct_resid<T, T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, saw, sas, b, ph, pl, e, qh, ql, r,
+                                                    c->partial, c->s);
+  k_fin_store<3><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), 2u, 0, c->s);
 }
 
 struct SolveResult {
   int status = P2D_OK, iterations = 0, outer = 0;
-  double r2 = 0, rinf = 0;
+  double r2 = 0, rinf = 0, b2 = 0;
 };
 
-// Refinement step: p_new = p + x (double-double), r = b' - A0' p_new into the
-// residual buffer the next Krylov iteration reads; norms into hs->red.
-template <class T>
-int refine(p2d_ctx* c, const double* b, double* solh[2], double* soll[2], int* cur, KVec<T>& v) {
-  CHK(sync_scal(c));
-  T* rdst = c->hs->rpar ? v.r[1] : v.r[0];
-  k_correct_resid<T, T><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
-      c->nx, c->ny, c->bw, c->bs, b, solh[*cur], soll[*cur], v.x, solh[1 - *cur], soll[1 - *cur], rdst, c->partial,
-      c->counter, c->s);
-  *cur = 1 - *cur;
-  CHK(launched_ok());
-  return sync_scal(c);
-}
+// Right-hand side of one solve: b' (velocity part, mean in s->bmean) and the
+// per-face jump terms (NULL: none).
+struct Rhs {
+  const double* b;
+  const double* saw;
+  const double* sas;
+};
 
-// Solves A0' x = b' (b' h^2-scaled, its mean removed) for x = sol[*cur].
-// rtol: ||r||_2 <= rtol ||b||_2 ; atol: ||r'||_inf <= atol (scaled); <= 0 disables.
+// Solves A0' x = b' - jumps (h^2-scaled, mean removed) for x = sol[*cur], the
+// pressure in double-double.  rtol: ||r||_2 <= rtol ||r_0||_2; atol:
+// ||r||_inf <= atol (scaled); <= 0 disables.
+//
+// The host enqueues inner iterations in chunks sized from the previous solve's
+// pass lengths; the kernels skip work once s->done is set.  When a pass (inner
+// segment) has converged, the refinement (true residual in FP64, correction
+// added to the double-double pressure) and the continue kernel are enqueued
+// without waiting: the continue kernel applies the outer convergence test on
+// the device, so a pass costs one host synchronisation.
 template <class T>
-int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const double* b, double b2, double* solh[2],
-            double* soll[2], int* cur, double rtol, double atol, SolveResult* out) {
+int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double* solh[2], double* soll[2],
+            int* cur, double rtol, double atol, SolveResult* out) {
   const size_t N = c->N;
   const bool mixed = sizeof(T) == 4;
   k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, solh[0], 0.0);
   k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, soll[0], 0.0);
   *cur = 0;
   // the first residual goes to rbuf[0] (k_inner_init resets the parity to 0)
-  k_correct_resid<T, T><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, b, solh[0],
-                                                                       soll[0], nullptr, nullptr, nullptr, v.r[0],
-                                                                       c->partial, c->counter, c->s);
+  correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, solh[0], soll[0], (const T*)nullptr, nullptr, nullptr, v.r[0]);
   CHK(launched_ok());
   CHK(sync_scal(c));
+  const double m0 = c->hs->red[2] / (double)N;
+  const double b2 = std::max(0.0, c->hs->red[0] - (double)N * m0 * m0);
   double r2 = c->hs->red[0], rinf = c->hs->red[1];
-  const double big = 1e300;
   auto converged = [&](double a2, double ainf) {
     return (rtol <= 0 || a2 <= rtol * rtol * b2) && (atol <= 0 || ainf <= atol);
   };
   // Inner targets: FP64 runs to the targets (with a margin for the recursive
-  // residual) or 1e-14 relative; FP32 runs segments of inner_rtol relative
+  // residual) or 1e-14 relative; FP32 runs passes of inner_rtol relative
   // reduction between FP64 residual replacements.
+  const double big = 1e300;
   const double abs2 = rtol > 0 ? 0.25 * rtol * rtol * b2 : big;
   const double ainf = atol > 0 ? 0.5 * atol : big;
   const double rel = mixed ? c->cfg.inner_rtol : 1e-14;
+  const double out2 = rtol > 0 ? rtol * rtol * b2 : -1.0, outi = atol > 0 ? atol : -1.0;
   const bool verbose = std::getenv("P2D_VERBOSE") != nullptr;
   if (verbose)
     fprintf(stderr, "p2d solve: %s b2=%.3e rtol=%.1e atol=%.3e r0: r2=%.3e rinf=%.3e\n", mixed ? "mixed" : "fp64", b2,
             rtol, atol, r2, rinf);
   int status = P2D_OK, outer = 0, stalled = 0, iters = 0;
   double best_r2 = r2, best_rinf = rinf;
-  if (!converged(r2, rinf)) {
+  int* last = c->last_seg[slot];
+  auto predict = [&](int pass) { return std::max(2, (pass < MAX_PASSES && last[pass] > 0 ? last[pass] : 6) + 1); };
+  if (!converged(r2, rinf) && b2 > 0) {
     k_inner_init<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? -1.0 : abs2, mixed ? -1.0 : ainf,
                                          c->cfg.max_iterations);
     k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
     CHK(launched_ok());
-    int chunk = std::max(2, c->last_iters[slot] > 0 ? c->last_iters[slot] + 1 : 8);
+    int chunk = predict(0), seg_start = 0;
+    bool report = false;  // a refinement's norms wait in red (iterations do not touch red)
     while (true) {
       for (int k = 0; k < chunk; ++k) cg_iteration<T>(c, mg, v);
       CHK(launched_ok());
       CHK(sync_scal(c));
       iters = c->hs->iter;
+      if (verbose)
+        fprintf(stderr, "    sync: iter %d done %d recursive r2=%.3e (rel %.3e of segment start) rinf=%.3e\n",
+                c->hs->iter, c->hs->done, c->hs->r2, std::sqrt(c->hs->r2 / std::max(c->hs->r0_2, 1e-300)),
+                c->hs->rinf);
+      if (report) {
+        report = false;
+        r2 = c->hs->red[0];
+        rinf = c->hs->red[1];
+        if (verbose)
+          fprintf(stderr, "  pass %d: %d its -> true r2=%.3e (rel %.3e) rinf=%.3e\n", outer, seg_start, r2,
+                  std::sqrt(r2 / std::max(b2, 1e-300)), rinf);
+        if (!(r2 == r2)) {
+          status = P2D_NAN;
+          break;
+        }
+        // a pass that improves neither norm by 2x has hit the rounding floor of the
+        // FP64 residual (~eps max |beta grad p| per cell): stop after the next refinement
+        if (r2 > 0.25 * best_r2 && rinf > 0.5 * best_rinf) {
+          ++stalled;
+        } else {
+          stalled = 0;
+        }
+        best_r2 = std::min(best_r2, r2);
+        best_rinf = std::min(best_rinf, rinf);
+      }
       if (c->hs->flag) {
         status = P2D_NAN;
         break;
       }
+      if (c->hs->outer_done) break;
       if (!c->hs->done) {
-        chunk = std::max(2, std::min(8, chunk / 2));
+        chunk = 2;
         continue;
       }
-      // end of a segment: FP64 residual replacement
-      CHK(refine<T>(c, b, solh, soll, cur, v));
+      // end of a pass: refinement p += x (double-double), the true residual into the
+      // buffer the next pass reads
+      if (outer < MAX_PASSES) last[outer] = iters - seg_start;
+      T* rdst = c->hs->rpar ? v.r[1] : v.r[0];
+      correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, solh[*cur], soll[*cur], v.x, solh[1 - *cur], soll[1 - *cur], rdst);
+      *cur = 1 - *cur;
       ++outer;
-      r2 = c->hs->red[0];
-      rinf = c->hs->red[1];
-      if (verbose)
-        fprintf(stderr, "  pass %d: %d its total -> true r2=%.3e (rel %.3e) rinf=%.3e\n", outer, iters, r2,
-                std::sqrt(r2 / std::max(b2, 1e-300)), rinf);
-      if (!(r2 == r2)) {
-        status = P2D_NAN;
+      if (iters >= c->cfg.max_iterations || outer >= c->cfg.max_outer || stalled >= 1) {
+        CHK(launched_ok());
+        CHK(sync_scal(c));
+        r2 = c->hs->red[0];
+        rinf = c->hs->red[1];
+        if (verbose)
+          fprintf(stderr, "  pass %d: %d its -> true r2=%.3e (rel %.3e) rinf=%.3e (last)\n", outer, iters, r2,
+                  std::sqrt(r2 / std::max(b2, 1e-300)), rinf);
+        if (!(r2 == r2)) status = P2D_NAN;
         break;
       }
-      if (converged(r2, rinf)) break;
-      if (iters >= c->cfg.max_iterations || outer >= c->cfg.max_outer) break;
-      if (r2 > 0.25 * best_r2 && rinf > 0.5 * best_rinf) {  // FP64 floor reached
-        if (++stalled >= 2) break;
-      } else {
-        stalled = 0;
-      }
-      best_r2 = std::min(best_r2, r2);
-      best_rinf = std::min(best_rinf, rinf);
       k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
-      k_inner_continue<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? -1.0 : abs2, mixed ? -1.0 : ainf, mixed ? 1 : 0);
+      k_inner_continue<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? -1.0 : abs2, mixed ? -1.0 : ainf,
+                                               out2, outi);
       CHK(launched_ok());
-      chunk = std::max(2, std::min(8, chunk));
+      report = true;
+      seg_start = iters;
+      chunk = predict(outer);
     }
-    c->last_iters[slot] = iters;
   }
   if (status == P2D_OK && !converged(r2, rinf)) status = P2D_NOT_CONVERGED;
   out->status = status;
@@ -401,17 +463,18 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const double* b, double
   out->outer = outer;
   out->r2 = r2;
   out->rinf = rinf;
+  out->b2 = b2;
   return P2D_OK;
 }
 
-int solve(p2d_ctx* c, const double* b, double b2, double* solh[2], double* soll[2], int* cur, double rtol,
-          double atol, SolveResult* out) {
+int solve(p2d_ctx* c, const Rhs& rhs, double* solh[2], double* soll[2], int* cur, double rtol, double atol,
+          SolveResult* out) {
   if (c->cfg.precision == P2D_MIXED) {
     CHK(ensure_mg(c, P2D_PREC_F32));
-    return solve_t<float>(c, c->mg32, c->v32, 1, b, b2, solh, soll, cur, rtol, atol, out);
+    return solve_t<float>(c, c->mg32, c->v32, 1, rhs, solh, soll, cur, rtol, atol, out);
   }
   CHK(ensure_mg(c, P2D_PREC_F64));
-  return solve_t<double>(c, c->mg64, c->v64, 0, b, b2, solh, soll, cur, rtol, atol, out);
+  return solve_t<double>(c, c->mg64, c->v64, 0, rhs, solh, soll, cur, rtol, atol, out);
 }
 
 int stage_in(p2d_ctx* c, double* dst, const double* src, size_t n, int mem) {
@@ -439,6 +502,7 @@ __global__ void k_scal_arm(Scal* s, int iter, double beta, double alpha, int pen
     s->zmean = 0.0;
     s->rshift = 0.0;
     s->flag = 0;
+    s->outer_done = 0;
     s->rel2 = 0.0;
     s->abs2 = -1.0;
     s->ainf = -1.0;
@@ -515,6 +579,12 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
     return rc;
   };
   if (hipStreamCreate(&c->own) != hipSuccess) return fail(P2D_HIP_ERROR);
+  {
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+      c->lds_limit = std::min<size_t>((size_t)lds, 65536) - 1024;  // static LDS of the kernel
+  }
   c->stream = c->own;
   const size_t N = c->N;
   int rc = P2D_OK;
@@ -522,6 +592,8 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
   if ((rc = dalloc(c, &(p), (n))) != P2D_OK) return fail(rc)
   A(c->phi, N);
   A(c->jc, N);
+  A(c->saw, N);
+  A(c->sas, N);
   A(c->bw, N);
   A(c->bs, N);
   A(c->b, N);
@@ -555,9 +627,10 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
     size_t mb = std::max<size_t>(count(bgrid(c->nx + 1, c->ny + 1)), GRIDSTRIDE_BLOCKS);
     mb = std::max(mb, count(tgrid(c->nx, c->ny, 2 * MAX_NU + 1)));
     mb = std::max(mb, count(tgrid(c->nx, c->ny, 2 * 1)));
+    mb = std::max(mb, count(dgrid(c->nx, c->ny)));
+    mb = std::max(mb, count(fgrid(c->nx, c->ny)));
     A(c->partial, (size_t)RED_K * mb);
   }
-  A(c->counter, 4);
   A(c->s, 1);
 #undef A
   if ((rc = alloc_mg<double>(c, c->mg64, c->bw, c->bs)) != P2D_OK) return fail(rc);
@@ -602,7 +675,7 @@ int p2d_set_fluid(p2d_ctx* c, const double* phi, double rho_minus, double rho_pl
   c->sigma = sigma;
   c->have_fluid = true;
   c->explicit_coeff = false;
-  c->has_jc = true;
+  c->has_jc = sigma != 0.0;
   c->coeff_ready = false;
   invalidate(c);
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -647,20 +720,23 @@ int p2d_project(p2d_ctx* c, double* u, double* v, double dt, p2d_stats* st, int 
     c->coeff_ready = true;
     invalidate(c);
   }
-  const double* jc = c->has_jc ? c->jc : nullptr;
-  k_rhs<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, c->phi, jc, c->bw, c->bs, c->b,
-                                                      c->partial, c->counter, c->s);
+  const double* saw = c->has_jc ? c->saw : nullptr;
+  const double* sas = c->has_jc ? c->sas : nullptr;
+  if (c->has_jc)
+    k_jump<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->phi, c->jc, c->saw, c->sas);
+  k_rhs<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, c->b, c->partial);
+  k_fin_mean<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx, c->ny)), (double)c->N, c->s);
   CHK(launched_ok());
-  CHK(sync_scal(c));
-  const bool bad_geom = c->hs->err != 0;
-  const double b2 = c->hs->red[0];
   const double atol = c->cfg.div_tol > 0 ? 0.5 * c->cfg.div_tol * c->h * c->h / dt : 0.0;
   SolveResult sr;
-  CHK(solve(c, c->b, b2, c->ph, c->pl, &c->pcur, c->cfg.rtol, atol, &sr));
+  CHK(solve(c, Rhs{c->b, saw, sas}, c->ph, c->pl, &c->pcur, c->cfg.rtol, atol, &sr));
+  const bool bad_geom = c->hs->err != 0;
+  const double b2 = sr.b2;
   k_face_update<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->ph[c->pcur],
-                                                              c->pl[c->pcur], c->phi, jc, c->bw, c->bs, du, dv);
-  k_diagnostics<<<bgrid(c->nx + 1, c->ny + 1), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, du, dv, c->partial,
-                                                                      c->counter, c->s);
+                                                              c->pl[c->pcur], saw, sas, c->bw, c->bs, du, dv);
+  k_diagnostics<<<bgrid(c->nx + 1, c->ny + 1), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, du, dv, c->partial, c->s);
+  k_fin_store<3><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx + 1, c->ny + 1)), 7u, 0,
+                                                      c->s);
   CHK(launched_ok());
   if (mem == P2D_MEM_HOST) {
     CHK(stage_out(c, u, c->su, c->NU, mem));
@@ -688,7 +764,8 @@ int p2d_get_pressure(p2d_ctx* c, double* p, int mean_zero, int mem) {
   const size_t N = c->N;
   const double* sh = c->ph[c->pcur];
   const double* sl = c->pl[c->pcur];
-  k_psum<<<gs_blocks(N), BLOCK, 0, c->stream>>>(N, sh, sl, c->partial, c->counter, c->s);
+  k_psum<<<gs_blocks(N), BLOCK, 0, c->stream>>>(N, sh, sl, c->partial, c->s);
+  k_fin_store<1><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)gs_blocks(N), 0u, 3, c->s);
   double* dst = mem == P2D_MEM_HOST ? c->sp : p;
   k_shift<<<blocks_for(N), BLOCK, 0, c->stream>>>(N, sh, sl, dst, mean_zero, c->s);
   CHK(launched_ok());
@@ -720,16 +797,16 @@ int p2d_adjoint(p2d_ctx* c, const double* gu, const double* gv, double* du, doub
     g1 = c->sg1;
     g2 = c->sg2;
   }
-  k_adj_rhs<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, g1, g2, c->bw, c->bs, c->b, c->partial,
-                                                          c->counter, c->s);
+  k_adj_rhs<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, g1, g2, c->bw, c->bs, c->b, c->partial);
+  k_fin_mean<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx, c->ny)), (double)c->N, c->s);
   CHK(launched_ok());
-  CHK(sync_scal(c));
-  const double b2 = c->hs->red[0];
   SolveResult sr;
   int mcur = 0;
   const double rt = c->cfg.rtol > 0 ? c->cfg.rtol : 1e-10;
-  CHK(solve(c, c->b, b2, c->muh, c->mul, &mcur, rt, 0.0, &sr));
-  const double* jc = c->has_jc ? c->jc : nullptr;
+  CHK(solve(c, Rhs{c->b, nullptr, nullptr}, c->muh, c->mul, &mcur, rt, 0.0, &sr));
+  const double b2 = sr.b2;
+  const double* saw = c->has_jc ? c->saw : nullptr;
+  const double* sas = c->has_jc ? c->sas : nullptr;
   double *o1 = du, *o2 = dv, *o3 = dbx, *o4 = dby;
   if (mem == P2D_MEM_HOST) {
     o1 = du ? c->sa[0] : nullptr;
@@ -738,7 +815,7 @@ int p2d_adjoint(p2d_ctx* c, const double* gu, const double* gv, double* du, doub
     o4 = dby ? c->sa[3] : nullptr;
   }
   k_adj_faces<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, c->last_dt, g1, g2, c->muh[mcur],
-                                                            c->mul[mcur], c->ph[c->pcur], c->pl[c->pcur], c->phi, jc,
+                                                            c->mul[mcur], c->ph[c->pcur], c->pl[c->pcur], saw, sas,
                                                             o1, o2, o3, o4);
   CHK(launched_ok());
   if (mem == P2D_MEM_HOST) {
@@ -843,29 +920,27 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
   auto launch = [&]() {
     switch (which) {
       case 0:
-        k_dir<float><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, L32[0].bw, L32[0].bs, c->v32.z,
-                                                                   c->v32.p[0], c->v32.p[1], c->v32.q, c->partial,
-                                                                   c->counter, c->s);
+        k_dir<float><<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, L32[0].bw, L32[0].bs, c->v32.z,
+                                                                   c->v32.p[0], c->v32.p[1], c->v32.q, c->partial, c->s);
         break;
       case 1:
-        k_dir<double><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, c->v64.z,
-                                                                    c->v64.p[0], c->v64.p[1], c->v64.q, c->partial,
-                                                                    c->counter, c->s);
+        k_dir<double><<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, c->v64.z,
+                                                                    c->v64.p[0], c->v64.p[1], c->v64.q, c->partial, c->s);
         break;
       case 2:
-        k_mg_pre<float, true><<<tgrid(c->nx, c->ny, 2 * nu1 + 1), BLOCK, 0, c->stream>>>(
+        k_mg_pre<float, true><<<tgrid(c->nx, c->ny, 2 * nu1 + 1), TBLOCK, 0, c->stream>>>(
             L32[0], L32[1], nullptr, L32[0].xa, nu1, c->v32.r[0], c->v32.r[1], c->v32.x, c->v32.p[0], c->v32.p[1],
-            c->v32.q, c->partial, c->counter, c->s);
+            c->v32.q, c->partial, c->s);
         break;
       case 3:
-        k_mg_post<float, true><<<tgrid(c->nx, c->ny, 2 * nu2), BLOCK, 0, c->stream>>>(
-            L32[0], L32[1], nullptr, c->v32.r[0], c->v32.r[1], L32[0].xa, c->v32.z, nu2, c->partial, c->counter,
+        k_mg_post<float, true><<<tgrid(c->nx, c->ny, 2 * nu2), TBLOCK, 0, c->stream>>>(
+            L32[0], L32[1], nullptr, c->v32.r[0], c->v32.r[1], L32[0].xa, c->v32.z, nu2, c->partial,
             c->s);
         break;
       default:
-        k_correct_resid<float, float><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
-            c->nx, c->ny, c->bw, c->bs, c->b, c->ph[0], c->pl[0], c->v32.x, c->ph[1], c->pl[1], c->v32.r[0],
-            c->partial, c->counter, c->s);
+        k_correct_resid<float, float><<<fgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
+            c->nx, c->ny, c->bw, c->bs, nullptr, nullptr, c->b, c->ph[0], c->pl[0], c->v32.x, c->ph[1], c->pl[1],
+            c->v32.r[0], c->partial, c->s);
     }
   };
   k_scal_arm<<<1, 1, 0, c->stream>>>(c->s, 1, 0.25, 1e-3, 1);

@@ -16,8 +16,8 @@
 // solved with a pseudo-inverse.
 //
 // Reductions accumulate in FP64 and are deterministic: 64-lane wave shuffles,
-// a fixed-order block tree, then the last block to finish (atomic ticket) sums
-// the per-block partials in block order.
+// a fixed-order block tree, per-block partials, then a one-block finish kernel
+// sums the partials in block order.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -25,7 +25,7 @@
 namespace p2d {
 
 constexpr int BLOCK = 256;          // 4 wavefronts of 64
-constexpr int BAND = 16;            // rows per block of the band (row-marching) kernels
+constexpr int BAND = 4;             // rows per block of the band (row-marching) kernels
 constexpr int SMALL_THREADS = 1024; // single-workgroup coarse V-cycle
 constexpr int RED_K = 4;            // most quantities any kernel reduces
 
@@ -48,6 +48,20 @@ struct Coef {
 __device__ __forceinline__ int nblocks() { return (int)(gridDim.x * gridDim.y); }
 __device__ __forceinline__ int block_id() { return (int)(blockIdx.y * gridDim.x + blockIdx.x); }
 
+// XCD-aware tile order.  The dispatcher deals workgroups to the 8 XCDs
+// round-robin by linear id; tiles are renumbered so that every XCD owns one
+// contiguous run of row-major tiles, and the halo rows/columns that
+// neighbouring tiles share are fetched once into that XCD's L2.  Bijective for
+// any grid size.  Returns the (x, y) tile of this workgroup.
+constexpr int NXCD = 8;
+__device__ __forceinline__ void xcd_tile(int& tx, int& ty) {
+  const int G = nblocks(), w = block_id();
+  const int x = w % NXCD, k = w / NXCD, q = G / NXCD, rem = G % NXCD;
+  const int t = x * q + min(x, rem) + k;
+  tx = t % (int)gridDim.x;
+  ty = t / (int)gridDim.x;
+}
+
 // Device-resident solver state.  Kernels of an inner iteration read `done` on
 // entry and return at once when it is set, so the host can enqueue iterations
 // ahead without waiting for the convergence test.
@@ -67,6 +81,8 @@ struct Scal {
   int rpar;                        // which of the two residual buffers holds the current residual
   int err;                         // coefficient assembly error (zero level-set gradient)
   int seg0;                        // iteration at which the current direction sequence started (beta = 0)
+  int outer_done;                  // the outer targets hold after a refinement
+  int pad;
 };
 
 // --------------------------------------------------------------------------- reductions
@@ -104,42 +120,51 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
   }
 }
 
-// Grid reduction over all blocks of a (BLOCK-thread) launch.  Returns true in the
-// last block; thread 0 of that block holds the totals in v.  partial must hold
-// K * nblocks() doubles; counter starts at 0 and is left at 0.
-template <int K>
-__device__ bool grid_reduce(double (&v)[K], unsigned maxmask, double* partial, unsigned* counter) {
-  block_reduce<K, BLOCK>(v, maxmask);
-  __shared__ int last;
+// Grid reductions run in two launches.  The producing kernel stores one
+// partial per block (block_partial: partial[k * G + block], G blocks); a
+// one-block finish kernel launched next on the stream combines them in block
+// order (finish_reduce) and applies the solver logic.  There is no
+// cross-block synchronisation inside a kernel: the L2 caches of the XCDs are
+// not coherent with each other, so a device-scope fence per block would write
+// back L2 every time; the kernel boundary makes the partials visible once.
+constexpr int FINISH_THREADS = 1024;
+
+template <int K, int NT = BLOCK>
+__device__ __forceinline__ void block_partial(double (&v)[K], unsigned maxmask, double* partial) {
+  block_reduce<K, NT>(v, maxmask);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) partial[(size_t)k * nblocks() + block_id()] = v[k];
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    last = (t == (unsigned)nblocks() - 1);
   }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  double a[K];
+}
+
+// In a FINISH_THREADS-thread block: totals of G partials into thread 0's a.
+// Maxima are of non-negative quantities (start at 0).
+template <int K>
+__device__ __forceinline__ void finish_reduce(double (&a)[K], const double* __restrict__ partial, int G,
+                                              unsigned maxmask) {
 #pragma unroll
   for (int k = 0; k < K; ++k) a[k] = 0.0;
-  const volatile double* vp = partial;
-  const int G = nblocks();
-  for (int b = threadIdx.x; b < G; b += BLOCK) {
+  for (int b = threadIdx.x; b < G; b += FINISH_THREADS) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const double x = vp[(size_t)k * G + b];
+      const double x = partial[(size_t)k * G + b];
       a[k] = ((maxmask >> k) & 1) ? fmax(a[k], x) : a[k] + x;
     }
   }
-  block_reduce<K, BLOCK>(a, maxmask);
+  block_reduce<K, FINISH_THREADS>(a, maxmask);
+}
+
+// Totals into s->red[dst + k].
+template <int K>
+__global__ void __launch_bounds__(FINISH_THREADS) k_fin_store(const double* __restrict__ partial, int G,
+                                                              unsigned maxmask, int dst, Scal* s) {
+  double a[K];
+  finish_reduce<K>(a, partial, G, maxmask);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = a[k];
-    *counter = 0;
+    for (int k = 0; k < K; ++k) s->red[dst + k] = a[k];
   }
-  return true;
 }
 
 // --------------------------------------------------------------------------- ghost-fluid pieces
@@ -174,9 +199,9 @@ __device__ __forceinline__ double face_sa(const double* phi, const double* jc, s
 }
 
 // Double-double pressure (hi + lo).  The refinement loop keeps p in two FP64
-// words so that face differences (hi_R - hi_L) + (lo_R - lo_L) stay accurate
-// when p carries large levels (pressure of light drops embedded in a heavy
-// fluid); a single FP64 word would round them at ~eps |p| and floor max|D u|.
+// words so that face differences stay accurate when p carries large levels
+// (pressure of light drops embedded in a heavy fluid, jumps of sigma kappa);
+// a single FP64 word would round them at ~eps |p| and floor max|D u|.
 __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
 #pragma clang fp contract(off)
   s = a + b;
@@ -191,10 +216,6 @@ __device__ __forceinline__ void dd_add(double hi, double lo, double e, double& r
   t += lo;
   rh = s + t;
   rl = t - (rh - s);
-}
-__device__ __forceinline__ double dd_diff(const double* hi, const double* lo, size_t kR, size_t kL) {
-#pragma clang fp contract(off)
-  return (hi[kR] - hi[kL]) + (lo[kR] - lo[kL]);
 }
 
 // Level set with linear-extrapolation ghosts (FIELD_STRUCTURE_2D.h:585-639);

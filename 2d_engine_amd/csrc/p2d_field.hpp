@@ -1,10 +1,19 @@
 // p2d field kernels: coefficient assembly, right-hand side, the FP64
 // refinement step, the face update, diagnostics and the adjoint's face kernels.
 //
-// All of them are band kernels: block (bx, by) owns columns bx*BLOCK .. +BLOCK
-// and rows by*BAND .. +BAND, so the rows above and below a cell are read by the
-// same block (same XCD, same L2) and each element is fetched from HBM about once.
-// Launch geometry: dim3(ceil(nx/BLOCK), ceil(ny/BAND)) (rows ny+1 for face sweeps).
+// Most are band kernels: block (bx, by) owns columns bx*BLOCK .. +BLOCK and rows
+// by*BAND .. +BAND (launch dim3(ceil(nx/BLOCK), ceil(ny/BAND)), rows ny+1 for
+// face sweeps).  The refinement step, which runs several times per projection,
+// is a tile kernel: the double-double pressure of a 64 x DTY tile and its ring
+// is staged in LDS with every load issued at once.
+//
+// Interface faces.  At a face across the interface the pressure difference
+// p_R - p_L nearly cancels the ghost-fluid jump s a (both ~ sigma kappa while
+// their sum ~ h |u|).  The face flux G_f = beta_f (p_R - p_L + s_f a_f) is
+// therefore formed in one expression (fdiff) with the high words differenced
+// exactly, in the residual and in the face update alike; the right-hand side
+// keeps only the velocity part -h^2 D u/dt.  Rounding each term separately
+// would floor max |D u_proj| at ~eps sigma kappa dt / h^2.
 #pragma once
 #include "p2d_common.hpp"
 
@@ -14,6 +23,19 @@ namespace p2d {
   const int i = (int)(blockIdx.x * BLOCK + threadIdx.x); \
   const int j0 = (int)(blockIdx.y * BAND);               \
   const int j1 = min((int)(NROWS), j0 + BAND)
+
+constexpr int FTY = 16;  // rows per block of the refinement tile kernel
+
+// p_R - p_L + sa for double-double p = (h, l): the high words are differenced
+// exactly and the jump added exactly (two_sum), so only the (small) result is
+// rounded.
+__device__ __forceinline__ double fdiff(double hR, double lR, double hL, double lL, double sa) {
+#pragma clang fp contract(off)
+  double d, de, t, te;
+  two_sum(hR, -hL, d, de);
+  two_sum(d, sa, t, te);
+  return t + (te + (de + (lR - lL)));
+}
 
 // --------------------------------------------------------------------------- assembly
 // Per cell: jc = sigma * kappa (PROJECTION_2D.h:830-834), beta on its west and
@@ -49,6 +71,21 @@ __global__ void __launch_bounds__(BLOCK) k_coeff(int nx, int ny, double h, const
   }
 }
 
+// Ghost-fluid jump term s a on the west and south face of every cell
+// (POISSON_SOLVER.cpp:874-890 restated per face: a interpolated from jc by the
+// level set, s = +1 when the cell on the right/top is on the minus side).
+__global__ void __launch_bounds__(BLOCK) k_jump(int nx, int ny, const double* __restrict__ phi,
+                                                const double* __restrict__ jc, double* __restrict__ saw,
+                                                double* __restrict__ sas) {
+  P2D_BAND(ny);
+  if (i >= nx) return;
+  for (int j = j0; j < j1; ++j) {
+    const size_t k = ci(i, j, nx);
+    saw[k] = i > 0 ? face_sa(phi, jc, k - 1, k) : 0.0;
+    sas[k] = j > 0 ? face_sa(phi, jc, k - nx, k) : 0.0;
+  }
+}
+
 // C ABI face layout <-> cell-pitched layout.
 __global__ void __launch_bounds__(BLOCK) k_faces_to_cells(int nx, int ny, const double* __restrict__ fx,
                                                           const double* __restrict__ fy, double* __restrict__ bw,
@@ -72,92 +109,97 @@ __global__ void __launch_bounds__(BLOCK) k_cells_to_faces(int nx, int ny, const 
 }
 
 // --------------------------------------------------------------------------- right-hand side
-// Right-hand-side statistics: mean (removed by the residual kernel, A0 being
-// singular on constants) and the norms of b - mean into s->red[0..1].
-__device__ __forceinline__ void rhs_finalize(const double (&acc)[3], size_t N, Scal* s) {
-  const double m = acc[2] / (double)N;
-  s->bmean = m;
-  s->red[0] = fmax(0.0, acc[0] - (double)N * m * m);
-  s->red[1] = acc[1] + fabs(m);
+// Mean of the right-hand side (removed by the residual kernel, A0 being
+// singular on constants).
+__global__ void __launch_bounds__(FINISH_THREADS) k_fin_mean(const double* __restrict__ partial, int G, double N,
+                                                             Scal* s) {
+  double a[1];
+  finish_reduce<1>(a, partial, G, 0u);
+  if (threadIdx.x == 0) s->bmean = a[0] / N;
 }
 
-// b' = h^2 (-D u/dt + D(B J)), the reference's b = -div - F (POISSON_SOLVER.cpp:951)
-// with J_f = s_f a_f / h.  u: (nx+1) x ny x-faces, v: nx x (ny+1) y-faces.
+// Velocity part of b' = h^2 (-D u/dt + D(B J)), the reference's b = -div - F
+// (POISSON_SOLVER.cpp:951) with J_f = s_f a_f / h; the jump part D(B J) enters
+// the residual face by face (fdiff).  u: (nx+1) x ny x-faces, v: nx x (ny+1)
+// y-faces.  Block partials: sum b.
 __global__ void __launch_bounds__(BLOCK) k_rhs(int nx, int ny, double h, double dt, const double* __restrict__ u,
-                                               const double* __restrict__ v, const double* __restrict__ phi,
-                                               const double* __restrict__ jc, const double* __restrict__ bw,
-                                               const double* __restrict__ bs, double* __restrict__ b,
-                                               double* partial, unsigned* counter, Scal* s) {
+                                               const double* __restrict__ v, double* __restrict__ b,
+                                               double* partial) {
 #pragma clang fp contract(off)
   const double ih = 1.0 / h, idt = 1.0 / dt, h2 = h * h;
-  const Coef<double> C{bw, bs, nx, ny};
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[1] = {0.0};
   P2D_BAND(ny);
   for (int j = j0; j < j1 && i < nx; ++j) {
     const size_t k = ci(i, j, nx);
     const double d = (ih * (u[fxi(i + 1, j, nx)] - u[fxi(i, j, nx)]) + ih * (v[fyi(i, j + 1, nx)] - v[fyi(i, j, nx)])) * idt;
-    double Jl = 0, Jr = 0, Jb = 0, Jt = 0;
-    if (jc) {
-      if (i > 0) Jl = face_sa(phi, jc, k - 1, k) / h;
-      if (i + 1 < nx) Jr = face_sa(phi, jc, k, k + 1) / h;
-      if (j > 0) Jb = face_sa(phi, jc, k - nx, k) / h;
-      if (j + 1 < ny) Jt = face_sa(phi, jc, k, k + nx) / h;
-    }
-    const double f = (C.E(i, j) * Jr - C.W(i, j) * Jl + C.N(i, j) * Jt - C.S(i, j) * Jb) * ih;
-    const double bv = h2 * (-d + f);
+    const double bv = h2 * (-d);
     b[k] = bv;
-    acc[0] += bv * bv;
-    acc[1] = fmax(acc[1], fabs(bv));
-    acc[2] += bv;
+    acc[0] += bv;
   }
-  if (grid_reduce<3>(acc, 2u, partial, counter) && threadIdx.x == 0) rhs_finalize(acc, (size_t)nx * ny, s);
+  block_partial<1>(acc, 0u, partial);
 }
 
 // --------------------------------------------------------------------------- FP64 refinement step
 // On the double-double pressure: p_new = p_old + e (e may be NULL),
-// r = (b' - mean b') - A0' p_new with accurate face differences, stored as R;
-// ||r||_2^2, ||r||_inf into s->red[0..1].  The block marches down its band and
-// keeps the rows above/below in registers.
+// r = (b' - mean b') - sum_f +-G_f with G_f = beta_f fdiff(p, s a) (west and
+// south faces +, east and north -), stored as R; block partials sum r^2,
+// max |r|, sum r.  saw/sas may be NULL (no jump).  Grid dim3(ceil(nx/64),
+// ceil(ny/FTY)), BLOCK threads; p_new of the tile interior into (qh, ql).
 template <class TE, class R>
 __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const double* __restrict__ bw,
-                                                         const double* __restrict__ bs, const double* __restrict__ b,
+                                                         const double* __restrict__ bs,
+                                                         const double* __restrict__ saw,
+                                                         const double* __restrict__ sas, const double* __restrict__ b,
                                                          const double* __restrict__ ph, const double* __restrict__ pl,
                                                          const TE* __restrict__ e, double* __restrict__ qh,
                                                          double* __restrict__ ql, R* __restrict__ r,
-                                                         double* partial, unsigned* counter, Scal* s) {
+                                                         double* partial, const Scal* s) {
 #pragma clang fp contract(off)
-  double acc[3] = {0.0, 0.0, 0.0};
+  __shared__ double sh[FTY + 2][66], sl[FTY + 2][66];
   const double bm = s->bmean;
-  const Coef<double> C{bw, bs, nx, ny};
-  P2D_BAND(ny);
-  auto P = [&](size_t q, double& hh, double& ll) {
-    if (e) dd_add(ph[q], pl[q], (double)e[q], hh, ll);
-    else {
-      hh = ph[q];
-      ll = pl[q];
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * 64, j0 = by * FTY;
+  const int i = i0 + c;
+  auto stage = [&](int ii, int jj, int rr, int cc) {
+    double hh = 0.0, ll = 0.0;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+      const size_t q = ci(ii, jj, nx);
+      if (e) {
+        dd_add(ph[q], pl[q], (double)e[q], hh, ll);
+      } else {
+        hh = ph[q];
+        ll = pl[q];
+      }
     }
+    sh[rr][cc] = hh;
+    sl[rr][cc] = ll;
   };
-  if (i < nx && j0 < j1) {
-    double sh = 0, sl = 0, ch, cl, nh = 0, nl = 0;  // south, centre, north
-    if (j0 > 0) P(ci(i, j0 - 1, nx), sh, sl);
-    P(ci(i, j0, nx), ch, cl);
-    for (int j = j0; j < j1; ++j) {
+  constexpr int NW = BLOCK / 64, RR = (FTY + 2 + NW - 1) / NW;
+#pragma unroll
+  for (int m = 0; m < RR; ++m) {
+    const int rr = w + NW * m;
+    if (rr < FTY + 2) stage(i, j0 - 1 + rr, rr, c + 1);
+  }
+  if (threadIdx.x < 2 * (FTY + 2)) {
+    const int side = (int)threadIdx.x / (FTY + 2), rr = (int)threadIdx.x % (FTY + 2);
+    stage(side ? i0 + 64 : i0 - 1, j0 - 1 + rr, rr, side ? 65 : 0);
+  }
+  __syncthreads();
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int m = 0; m < FTY / NW; ++m) {
+    const int rr = w + NW * m, j = j0 + rr;
+    if (i < nx && j < ny) {
       const size_t k = ci(i, j, nx);
-      if (j + 1 < ny) P(k + nx, nh, nl);
-      double a = 0;
-      if (i > 0) {
-        double wh, wl;
-        P(k - 1, wh, wl);
-        a += C.W(i, j) * ((ch - wh) + (cl - wl));
-      }
-      if (i + 1 < nx) {
-        double eh, el;
-        P(k + 1, eh, el);
-        a += C.E(i, j) * ((ch - eh) + (cl - el));
-      }
-      if (j > 0) a += C.S(i, j) * ((ch - sh) + (cl - sl));
-      if (j + 1 < ny) a += C.N(i, j) * ((ch - nh) + (cl - nl));
-      const double rv = (b[k] - bm) - a;
+      const double ch = sh[rr + 1][c + 1], cl = sl[rr + 1][c + 1];
+      double g = 0.0;
+      if (i > 0) g += bw[k] * fdiff(ch, cl, sh[rr + 1][c], sl[rr + 1][c], saw ? saw[k] : 0.0);
+      if (i + 1 < nx) g -= bw[k + 1] * fdiff(sh[rr + 1][c + 2], sl[rr + 1][c + 2], ch, cl, saw ? saw[k + 1] : 0.0);
+      if (j > 0) g += bs[k] * fdiff(ch, cl, sh[rr][c + 1], sl[rr][c + 1], sas ? sas[k] : 0.0);
+      if (j + 1 < ny) g -= bs[k + nx] * fdiff(sh[rr + 2][c + 1], sl[rr + 2][c + 1], ch, cl, sas ? sas[k + nx] : 0.0);
+      const double rv = (b[k] - bm) - g;
       if (qh) {
         qh[k] = ch;
         ql[k] = cl;
@@ -166,24 +208,18 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
       acc[0] += rv * rv;
       acc[1] = fmax(acc[1], fabs(rv));
       acc[2] += rv;
-      sh = ch;
-      sl = cl;
-      ch = nh;
-      cl = nl;
     }
   }
-  if (grid_reduce<3>(acc, 2u, partial, counter) && threadIdx.x == 0) {
-    s->red[0] = acc[0];
-    s->red[1] = acc[1];
-    s->red[2] = acc[2];
-  }
+  block_partial<3>(acc, 2u, partial);
 }
 
 // --------------------------------------------------------------------------- face update / diagnostics
-// Consistent A14: u_f -= dt beta_f (p_R - p_L + s a)/h on interior faces (walls untouched).
+// Consistent A14: u_f -= dt G_f / h, G_f = beta_f fdiff(p, s a), on interior
+// faces (walls untouched): the same G_f as the residual, so D u_proj = -dt r / h^2
+// up to the rounding of u.
 __global__ void __launch_bounds__(BLOCK) k_face_update(int nx, int ny, double h, double dt,
                                                        const double* __restrict__ ph, const double* __restrict__ pl,
-                                                       const double* __restrict__ phi, const double* __restrict__ jc,
+                                                       const double* __restrict__ saw, const double* __restrict__ sas,
                                                        const double* __restrict__ bw, const double* __restrict__ bs,
                                                        double* __restrict__ u, double* __restrict__ v) {
 #pragma clang fp contract(off)
@@ -192,15 +228,14 @@ __global__ void __launch_bounds__(BLOCK) k_face_update(int nx, int ny, double h,
   if (i >= nx) return;
   for (int j = j0; j < j1; ++j) {
     const size_t k = ci(i, j, nx);
+    const double ch = ph[k], cl = pl[k];
     if (i > 0) {
-      double gap = dd_diff(ph, pl, k, k - 1);
-      gap += face_sa(phi, jc, k - 1, k);
-      u[fxi(i, j, nx)] -= dt * gap * bw[k] * ih;
+      const double G = bw[k] * fdiff(ch, cl, ph[k - 1], pl[k - 1], saw ? saw[k] : 0.0);
+      u[fxi(i, j, nx)] -= dt * G * ih;
     }
     if (j > 0) {
-      double gap = dd_diff(ph, pl, k, k - nx);
-      gap += face_sa(phi, jc, k - nx, k);
-      v[fyi(i, j, nx)] -= dt * gap * bs[k] * ih;
+      const double G = bs[k] * fdiff(ch, cl, ph[k - nx], pl[k - nx], sas ? sas[k] : 0.0);
+      v[fyi(i, j, nx)] -= dt * G * ih;
     }
   }
 }
@@ -208,8 +243,7 @@ __global__ void __launch_bounds__(BLOCK) k_face_update(int nx, int ny, double h,
 // max |D u| (cell divergence, not divided by dt), max |u|, max |v| into
 // s->red[0..2].  Grid covers (nx+1) columns and ny+1 rows.
 __global__ void __launch_bounds__(BLOCK) k_diagnostics(int nx, int ny, double h, const double* __restrict__ u,
-                                                       const double* __restrict__ v, double* partial,
-                                                       unsigned* counter, Scal* s) {
+                                                       const double* __restrict__ v, double* partial, Scal* s) {
 #pragma clang fp contract(off)
   const double ih = 1.0 / h;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -222,17 +256,15 @@ __global__ void __launch_bounds__(BLOCK) k_diagnostics(int nx, int ny, double h,
       acc[0] = fmax(acc[0], fabs(d));
     }
   }
-  if (grid_reduce<3>(acc, 7u, partial, counter) && threadIdx.x == 0) {
-    for (int q = 0; q < 3; ++q) s->red[q] = acc[q];
-  }
+  block_partial<3>(acc, 7u, partial);
 }
 
 // Sum of p = hi + lo into s->red[3] (for the mean); 1-D grid-stride launch.
 __global__ void __launch_bounds__(BLOCK) k_psum(size_t N, const double* __restrict__ ph, const double* __restrict__ pl,
-                                                double* partial, unsigned* counter, Scal* s) {
+                                                double* partial, Scal* s) {
   double acc[1] = {0.0};
   for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < N; k += (size_t)gridDim.x * BLOCK) acc[0] += ph[k] + pl[k];
-  if (grid_reduce<1>(acc, 0u, partial, counter) && threadIdx.x == 0) s->red[3] = acc[0];
+  block_partial<1>(acc, 0u, partial);
 }
 
 // out = p - mean (mean_zero) or p - p(0,0) (the reference's pin, POISSON_SOLVER.cpp:789-807).
@@ -246,15 +278,15 @@ __global__ void __launch_bounds__(BLOCK) k_shift(size_t N, const double* __restr
 }
 
 // --------------------------------------------------------------------------- adjoint
-// b' = -h^2 D(B g) with g taken on interior faces only; mean and norms as k_rhs.
+// b' = -h^2 D(B g) with g taken on interior faces only; block partials sum b.
 __global__ void __launch_bounds__(BLOCK) k_adj_rhs(int nx, int ny, double h, const double* __restrict__ gu,
                                                    const double* __restrict__ gv, const double* __restrict__ bw,
                                                    const double* __restrict__ bs, double* __restrict__ b,
-                                                   double* partial, unsigned* counter, Scal* s) {
+                                                   double* partial) {
 #pragma clang fp contract(off)
   const double ih = 1.0 / h, h2 = h * h;
   const Coef<double> C{bw, bs, nx, ny};
-  double acc[3] = {0.0, 0.0, 0.0};
+  double acc[1] = {0.0};
   P2D_BAND(ny);
   for (int j = j0; j < j1 && i < nx; ++j) {
     const size_t k = ci(i, j, nx);
@@ -264,11 +296,9 @@ __global__ void __launch_bounds__(BLOCK) k_adj_rhs(int nx, int ny, double h, con
     const double gt = j + 1 < ny ? C.N(i, j) * gv[fyi(i, j + 1, nx)] : 0.0;
     const double bv = h2 * (-((gr - gl) * ih + (gt - gb) * ih));
     b[k] = bv;
-    acc[0] += bv * bv;
-    acc[1] = fmax(acc[1], fabs(bv));
-    acc[2] += bv;
+    acc[0] += bv;
   }
-  if (grid_reduce<3>(acc, 2u, partial, counter) && threadIdx.x == 0) rhs_finalize(acc, (size_t)nx * ny, s);
+  block_partial<1>(acc, 0u, partial);
 }
 
 // du = g - G mu, dbeta = -dt du (G p + s a / h) on interior faces; walls 0.
@@ -276,7 +306,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_faces(int nx, int ny, double h, d
                                                      const double* __restrict__ gu, const double* __restrict__ gv,
                                                      const double* __restrict__ muh, const double* __restrict__ mul,
                                                      const double* __restrict__ ph, const double* __restrict__ pl,
-                                                     const double* __restrict__ phi, const double* __restrict__ jc,
+                                                     const double* __restrict__ saw, const double* __restrict__ sas,
                                                      double* du, double* dv, double* dbx, double* dby) {
 #pragma clang fp contract(off)
   const double ih = 1.0 / h;
@@ -288,8 +318,8 @@ __global__ void __launch_bounds__(BLOCK) k_adj_faces(int nx, int ny, double h, d
       const size_t f = fxi(i, j, nx);
       double t = 0, g = 0;
       if (i > 0) {
-        t = gu[f] - dd_diff(muh, mul, k, k - 1) * ih;
-        const double w = dd_diff(ph, pl, k, k - 1) * ih + face_sa(phi, jc, k - 1, k) / h;
+        t = gu[f] - fdiff(muh[k], mul[k], muh[k - 1], mul[k - 1], 0.0) * ih;
+        const double w = fdiff(ph[k], pl[k], ph[k - 1], pl[k - 1], saw ? saw[k] : 0.0) * ih;
         g = -dt * t * w;
       }
       if (du) du[f] = t;
@@ -303,8 +333,8 @@ __global__ void __launch_bounds__(BLOCK) k_adj_faces(int nx, int ny, double h, d
       const size_t f = fyi(i, j, nx);
       double t = 0, g = 0;
       if (j > 0) {
-        t = gv[f] - dd_diff(muh, mul, k, k - nx) * ih;
-        const double w = dd_diff(ph, pl, k, k - nx) * ih + face_sa(phi, jc, k - nx, k) / h;
+        t = gv[f] - fdiff(muh[k], mul[k], muh[k - nx], mul[k - nx], 0.0) * ih;
+        const double w = fdiff(ph[k], pl[k], ph[k - nx], pl[k - nx], sas ? sas[k] : 0.0) * ih;
         g = -dt * t * w;
       }
       if (dv) dv[f] = t;

@@ -12,7 +12,9 @@
 //   dir                p = (z - mean z) + beta p_old, q = A0' p, p.q -> alpha
 // The pre/post kernels are temporally blocked: a 64-column region (one
 // wavefront wide) of TY output rows plus a halo of H cells is staged once, the
-// sweeps run on LDS, and only the tile interior is written back.  Halo widths:
+// sweeps run on LDS, and only the tile interior is written back.  Every global
+// load of a tile is independent of the others, so a tile costs about one
+// memory round trip; coarse levels (small grids) are latency-bound.  Halo widths:
 // pre H = 2 nu + 1 (first red sweep from zero needs no neighbours; each later
 // half-sweep and the residual lose one exact ring; restriction reads one ring
 // outside the tile), post H = 2 nu.
@@ -21,11 +23,17 @@
 
 namespace p2d {
 
+constexpr int TBLOCK = 512;                // pre/post tile kernels: 8 wavefronts
+constexpr int TWAVES = TBLOCK / 64;
 constexpr int TY = 32;                     // output rows per pre/post tile
 constexpr int MAX_NU = 3;                  // sweeps per side supported
 constexpr int RYMAX = TY + 2 * (2 * MAX_NU + 1);  // region rows at most
-constexpr int MAXR = (RYMAX + 3) / 4;      // region rows per wavefront at most
-constexpr int SMALL_MAX_CELLS = 1024;      // top level of the single-workgroup V-cycle (32^2)
+constexpr int MAXR = (RYMAX + TWAVES - 1) / TWAVES;  // region rows per thread at most
+constexpr int NCXMAX = 32;                 // coarse columns of a tile at most (TX / 2 < 32)
+constexpr int DTY = 16;                    // rows per block of the direction kernel
+constexpr int SMALL_MAX_CELLS = 1024;      // top level of the single-workgroup V-cycle at most (32^2)
+constexpr int MAXL = 24;                   // multigrid levels at most
+constexpr int BOTTOM_LDS_MAX = 64;         // bottom sizes inverted in LDS
 
 template <class T>
 struct Level {
@@ -114,14 +122,16 @@ __global__ void __launch_bounds__(BLOCK) k_mg_coarsen(int fnx, int fny, const do
 
 // Pseudo-inverse of the singular bottom operator A (null space = constants):
 // A+ = (A + a 1 1^T)^-1 - 1 1^T / (a n^2) with a n = mean diagonal; the shifted
-// matrix is SPD, inverted by Gauss-Jordan without pivoting in one workgroup
-// (work is n x 2n in global memory).
+// matrix is SPD and is inverted by Gauss-Jordan without pivoting in one
+// workgroup: in place in LDS for n <= BOTTOM_LDS_MAX, else on an n x 2n
+// augmented matrix in global memory (work).
 __global__ void __launch_bounds__(SMALL_THREADS) k_mg_bottom_inverse(int nx, int ny, const double* __restrict__ bw,
                                                                      const double* __restrict__ bs, double* work,
                                                                      double* inv64, float* inv32) {
-  const int n = nx * ny, w = 2 * n, tid = threadIdx.x;
+  const int n = nx * ny, tid = threadIdx.x;
   const Coef<double> C{bw, bs, nx, ny};
   __shared__ double alpha_sh;
+  __shared__ double a[BOTTOM_LDS_MAX * BOTTOM_LDS_MAX];
   if (tid == 0) {
     double dsum = 0;
     for (int c = 0; c < n; ++c) {
@@ -132,6 +142,45 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_bottom_inverse(int nx, int
   }
   __syncthreads();
   const double alpha = alpha_sh;
+  const double corr = 1.0 / (alpha * n * (double)n);
+  if (n <= BOTTOM_LDS_MAX) {
+    for (int q = tid; q < n * n; q += SMALL_THREADS) a[q] = alpha;
+    __syncthreads();
+    for (int c = tid; c < n; c += SMALL_THREADS) {
+      const int i = c % nx, j = c / nx;
+      double* row = a + c * n;
+      row[c] += C.W(i, j) + C.E(i, j) + C.S(i, j) + C.N(i, j);
+      if (i > 0) row[c - 1] -= C.W(i, j);
+      if (i + 1 < nx) row[c + 1] -= C.E(i, j);
+      if (j > 0) row[c - nx] -= C.S(i, j);
+      if (j + 1 < ny) row[c + nx] -= C.N(i, j);
+    }
+    __syncthreads();
+    // in-place inversion: pivot row scaled, rank-1 update off row/column k,
+    // column k = -a_ik / a_kk, a_kk = 1 / a_kk
+    for (int k = 0; k < n; ++k) {
+      const double d = 1.0 / a[k * n + k];
+      for (int j = tid; j < n; j += SMALL_THREADS)
+        if (j != k) a[k * n + j] *= d;
+      __syncthreads();
+      for (int q = tid; q < n * n; q += SMALL_THREADS) {
+        const int r = q / n, c = q % n;
+        if (r != k && c != k) a[q] -= a[r * n + k] * a[k * n + c];
+      }
+      __syncthreads();
+      for (int r = tid; r < n; r += SMALL_THREADS)
+        if (r != k) a[r * n + k] *= -d;
+      if (tid == 0) a[k * n + k] = d;
+      __syncthreads();
+    }
+    for (int q = tid; q < n * n; q += SMALL_THREADS) {
+      const double v = a[q] - corr;
+      inv64[q] = v;
+      inv32[q] = (float)v;
+    }
+    return;
+  }
+  const int w = 2 * n;
   for (int q = tid; q < n * w; q += SMALL_THREADS) work[q] = (q % w) < n ? alpha : 0.0;
   __syncthreads();
   for (int c = tid; c < n; c += SMALL_THREADS) {
@@ -160,7 +209,6 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_bottom_inverse(int nx, int
       if (r != k) work[(size_t)r * w + k] = 0.0;
     __syncthreads();
   }
-  const double corr = 1.0 / (alpha * n * (double)n);
   for (int q = tid; q < n * n; q += SMALL_THREADS) {
     const double v = work[(size_t)(q / n) * w + n + (q % n)] - corr;
     inv64[q] = v;
@@ -189,6 +237,7 @@ __global__ void k_inner_init(Scal* s, double ncells, double rel2, double abs2, d
   s->max_iter = max_iter;
   s->iter = 0;
   s->seg0 = 0;
+  s->outer_done = 0;
   s->flag = 0;
   s->pending = 0;
   s->rpar = 0;
@@ -200,14 +249,27 @@ __global__ void k_inner_init(Scal* s, double ncells, double rel2, double abs2, d
     s->flag = 1;
   }
 }
-// Continue after a residual replacement: new segment start, keep the iteration
-// count; keep the direction and rz_old (CG with residual replacement) or, with
-// restart, begin a new direction sequence (beta = 0).  FP32 segments restart:
-// their recursive residual drifts from the FP64 one, and the stale rz_old would
-// give a beta that lets the old direction dominate.
-__global__ void k_inner_continue(Scal* s, double ncells, double rel2, double abs2, double ainf, int restart) {
+// Continue after a residual replacement (red = norms of the true residual):
+// stop when the outer targets hold (out2 on sum r^2, outi on max |r|; < 0
+// disables), else start a new segment that keeps the iteration count and
+// begins a new direction sequence (beta = 0).  The recursive residual of the
+// finished segment may have drifted far from the true one (FP32, or FP64 with
+// large pressure levels); with the old direction and rz_old kept, beta would
+// be rz_new / rz_old >> 1 and the stale direction would dominate.
+__global__ void k_inner_continue(Scal* s, double ncells, double rel2, double abs2, double ainf, double out2,
+                                 double outi) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (restart) s->seg0 = s->iter;
+  if (!(s->red[0] == s->red[0])) {
+    s->flag = 1;
+    s->done = 1;
+    return;
+  }
+  if ((out2 < 0.0 || s->red[0] <= out2) && (outi < 0.0 || s->red[1] <= outi)) {
+    s->outer_done = 1;
+    s->done = 1;
+    return;
+  }
+  s->seg0 = s->iter;
   segment_start(s, ncells);
   s->rel2 = rel2;
   s->abs2 = abs2;
@@ -217,9 +279,9 @@ __global__ void k_inner_continue(Scal* s, double ncells, double rel2, double abs
   s->done = (met || s->iter >= s->max_iter) ? 1 : 0;
 }
 
-// Shared staging of a pre/post tile: every thread owns column c of the 64-wide
-// region and rows w, w+4, ... ; it keeps r, the four face coefficients and
-// 1/diag of its cells in registers.
+// Shared staging of a pre/post tile: thread (c, w) owns column c of the
+// 64-wide region and rows w, w + TWAVES, ...; it keeps r, the four face
+// coefficients and 1/diag of its cells in registers.
 template <class T>
 struct TileCells {
   T rr[MAXR], bW[MAXR], bE[MAXR], bS[MAXR], bN[MAXR], inv[MAXR];
@@ -231,7 +293,7 @@ __device__ __forceinline__ void tile_sweep(TileCells<T>& tc, T (*xs)[66], int c,
                                            int color, bool zero) {
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
-    const int r = w + 4 * m;
+    const int r = w + TWAVES * m;
     if (r >= RY || !tc.val[m] || ((gi + gy0 + r) & 1) != color) continue;
     const T off = zero ? (T)0
                        : tc.bW[m] * xs[r + 1][c] + tc.bE[m] * xs[r + 1][c + 2] + tc.bS[m] * xs[r][c + 1] +
@@ -241,24 +303,32 @@ __device__ __forceinline__ void tile_sweep(TileCells<T>& tc, T (*xs)[66], int c,
 }
 
 // --------------------------------------------------------------------------- pre-smoothing tile kernel
-// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TX = 64 - 2H, H = 2 nu + 1.
+// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, TX = 64 - 2H, H = 2 nu + 1.
 // FUSE (finest level): r_in = rbuf[s->rpar], the pending update r -= a q,
-// x += a p is applied first (tile interiors write r to rbuf[1 - rpar] and x),
-// ||r|| is reduced and the convergence decision taken.
+// x += a p and the residual shift are applied first (tile interiors write r to
+// rbuf[1 - rpar] and x), ||r|| is reduced and the convergence decision taken.
+// Restriction r_H = P^T res: every fine cell splits its residual into the four
+// weight products; the x-direction sums run across lanes (a region row is one
+// wavefront), the y-direction sums through LDS.  No global load depends on
+// another, so a tile costs about one memory round trip.
 template <class T, bool FUSE>
-__global__ void __launch_bounds__(BLOCK) k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu,
-                                                  T* rA, T* rB, T* xsol, const T* pA, const T* pB,
-                                                  const T* __restrict__ qv, double* partial, unsigned* counter,
-                                                  Scal* s) {
+__global__ void __launch_bounds__(TBLOCK) k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu,
+                                                   T* rA, T* rB, T* xsol, const T* pA, const T* pB,
+                                                   const T* __restrict__ qv, double* partial,
+                                                   Scal* s) {
   if (s->done) return;
   __shared__ T xs[RYMAX + 2][66];
-  __shared__ T rs[RYMAX][64];
   const int H = 2 * nu + 1, TX = 64 - 2 * H, RY = TY + 2 * H;
-  const int tx0 = (int)blockIdx.x * TX, ty0 = (int)blockIdx.y * TY;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int tx0 = bx * TX, ty0 = by * TY;
   const int gx0 = tx0 - H, gy0 = ty0 - H;
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int gi = gx0 + c;
   const int nx = F.nx, ny = F.ny;
+  const bool col_in = gi >= 0 && gi < nx;
+  const bool col_int = c >= H && c < H + TX;      // tile interior column
+  const bool col_fp = c >= H - 1 && c <= H + TX;  // restriction footprint column
   const T* rin = r_lvl;
   T* rout = nullptr;
   const T* pdir = nullptr;
@@ -273,28 +343,36 @@ __global__ void __launch_bounds__(BLOCK) k_mg_pre(Level<T> F, Level<T> Cc, const
     shift = s->rshift;
     pdir = ((s->iter + 1) & 1) ? pB : pA;
   }
-  double acc[2] = {0.0, 0.0};
+  double acc[3] = {0.0, 0.0, 0.0};
   TileCells<T> tc;
+  T wxr[MAXR], wyr[MAXR];
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
-    const int r = w + 4 * m, gj = gy0 + r;
-    const bool v = r < RY && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
+    const int r = w + TWAVES * m, gj = gy0 + r;
+    const bool v = r < RY && col_in && gj >= 0 && gj < ny;
     tc.val[m] = v;
-    T rv = 0, bw = 0, be = 0, bsv = 0, bn = 0;
+    T rv = 0, bw = 0, be = 0, bsv = 0, bn = 0, wx = 0, wy = 0;
     if (v) {
       const size_t k = ci(gi, gj, nx);
       rv = rin[k];
-      if (FUSE) rv = (T)(((double)rv - (pend ? alpha * (double)qv[k] : 0.0)) - shift);
-      if (FUSE && c >= H && c < H + TX && r >= H && r < H + TY) {
-        rout[k] = rv;
-        if (pend) xsol[k] = (T)((double)xsol[k] + alpha * (double)pdir[k]);
-        acc[0] += (double)rv * (double)rv;
-        acc[1] = fmax(acc[1], fabs((double)rv));
+      if (FUSE) {
+        rv = (T)(((double)rv - (pend ? alpha * (double)qv[k] : 0.0)) - shift);
+        if (col_int && r >= H && r < H + TY) {
+          rout[k] = rv;
+          if (pend) xsol[k] = (T)((double)xsol[k] + alpha * (double)pdir[k]);
+          acc[0] += (double)rv * (double)rv;
+          acc[1] = fmax(acc[1], fabs((double)rv));
+          acc[2] += (double)rv;
+        }
       }
       bw = F.bw[k];
       be = gi + 1 < nx ? F.bw[k + 1] : (T)0;
       bsv = F.bs[k];
       bn = gj + 1 < ny ? F.bs[k + nx] : (T)0;
+      if (col_fp && r >= H - 1 && r <= H + TY) {
+        wx = F.wx[k];
+        wy = F.wy[k];
+      }
     }
     tc.rr[m] = rv;
     tc.bW[m] = bw;
@@ -302,13 +380,15 @@ __global__ void __launch_bounds__(BLOCK) k_mg_pre(Level<T> F, Level<T> Cc, const
     tc.bS[m] = bsv;
     tc.bN[m] = bn;
     tc.inv[m] = v ? (T)1 / (bw + be + bsv + bn) : (T)0;
+    wxr[m] = wx;
+    wyr[m] = wy;
     if (r < RY) xs[r + 1][c + 1] = 0;
   }
-  for (int t = threadIdx.x; t < 66; t += BLOCK) {
+  for (int t = threadIdx.x; t < 66; t += TBLOCK) {
     xs[0][t] = 0;
     xs[RY + 1][t] = 0;
   }
-  for (int t = threadIdx.x; t < RY; t += BLOCK) {
+  for (int t = threadIdx.x; t < RY; t += TBLOCK) {
     xs[t + 1][0] = 0;
     xs[t + 1][65] = 0;
   }
@@ -319,83 +399,121 @@ __global__ void __launch_bounds__(BLOCK) k_mg_pre(Level<T> F, Level<T> Cc, const
       __syncthreads();
     }
   // residual (reference order: r - (diag x - offsum)) and the interior write of x
+  T res[MAXR];
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
-    const int r = w + 4 * m;
-    if (r >= RY) continue;
-    T res = 0;
-    if (tc.val[m]) {
+    const int r = w + TWAVES * m;
+    res[m] = 0;
+    if (r < RY && tc.val[m]) {
       const T x = xs[r + 1][c + 1];
       const T off =
           tc.bW[m] * xs[r + 1][c] + tc.bE[m] * xs[r + 1][c + 2] + tc.bS[m] * xs[r][c + 1] + tc.bN[m] * xs[r + 2][c + 1];
-      res = tc.rr[m] - ((tc.bW[m] + tc.bE[m] + tc.bS[m] + tc.bN[m]) * x - off);
-      if (c >= H && c < H + TX && r >= H && r < H + TY) x_lvl[ci(gi, gy0 + r, nx)] = x;
+      res[m] = tc.rr[m] - ((tc.bW[m] + tc.bE[m] + tc.bS[m] + tc.bN[m]) * x - off);
+      if (col_int && r >= H && r < H + TY) x_lvl[ci(gi, gy0 + r, nx)] = x;
     }
-    rs[r][c] = res;
+  }
+  __syncthreads();  // xs is reused below for the row sums us[2][RYMAX][NCXMAX]
+  T* us = &xs[0][0];
+  const int NCX = TX / 2, NCY = TY / 2, cx0 = tx0 / 2, cy0 = ty0 / 2;
+  const bool owner = col_int && ((c - H) & 1) == 0;  // fine column 2I of coarse column I
+  const int ic = (c - H) >> 1;
+#pragma unroll
+  for (int m = 0; m < MAXR; ++m) {
+    const int r = w + TWAVES * m;
+    if (r >= RY) continue;  // uniform across the wavefront
+    // A: part of res for the parent row (weight wy), B: for the neighbouring row (1 - wy);
+    // p*: part for the parent column (wx), q*: for the neighbouring column (1 - wx).
+    const T A = res[m] * wyr[m], B = res[m] * ((T)1 - wyr[m]);
+    const T pa = wxr[m] * A, qa = ((T)1 - wxr[m]) * A, pb = wxr[m] * B, qb = ((T)1 - wxr[m]) * B;
+    // coarse column I gathers q(2I-1) + p(2I) + p(2I+1) + q(2I+2) at the lane of fine column 2I
+    const T sa = (pa + __shfl_down(pa, 1, 64)) + (__shfl_up(qa, 1, 64) + __shfl_down(qa, 2, 64));
+    const T sb = (pb + __shfl_down(pb, 1, 64)) + (__shfl_up(qb, 1, 64) + __shfl_down(qb, 2, 64));
+    if (owner && r >= H - 1 && r <= H + TY) {
+      us[r * NCXMAX + ic] = sa;
+      us[(RYMAX + r) * NCXMAX + ic] = sb;
+    }
   }
   __syncthreads();
-  // restriction r_H = P^T res over the coarse cells of this tile
-  const int NCX = TX / 2, NCY = TY / 2, cx0 = tx0 / 2, cy0 = ty0 / 2;
-  for (int t = threadIdx.x; t < NCX * NCY; t += BLOCK) {
-    const int I = cx0 + t % NCX, J = cy0 + t / NCX;
+  // coarse row J gathers B(2J-1) + A(2J) + A(2J+1) + B(2J+2)
+  for (int t = threadIdx.x; t < NCX * NCY; t += TBLOCK) {
+    const int a = t % NCX, b = t / NCX, I = cx0 + a, J = cy0 + b;
     if (I >= Cc.nx || J >= Cc.ny) continue;
-    T sum = 0;
-    for (int j = 2 * J - 1; j <= 2 * J + 2; ++j) {
-      if (j < 0 || j >= ny) continue;
-      for (int i = 2 * I - 1; i <= 2 * I + 2; ++i) {
-        if (i < 0 || i >= nx) continue;
-        const size_t kk = ci(i, j, nx);
-        const T wy = w1<T>(j, J, Cc.ny, F.wy[kk]);
-        const T wx = w1<T>(i, I, Cc.nx, F.wx[kk]);
-        if (wx == (T)0 || wy == (T)0) continue;
-        sum += wx * wy * rs[j - gy0][i - gx0];
-      }
-    }
-    Cc.r[ci(I, J, Cc.nx)] = sum;
+    const int jr = 2 * b + H;  // region row of fine row 2J
+    Cc.r[ci(I, J, Cc.nx)] = (us[(RYMAX + jr - 1) * NCXMAX + a] + us[jr * NCXMAX + a]) +
+                            (us[(jr + 1) * NCXMAX + a] + us[(RYMAX + jr + 2) * NCXMAX + a]);
   }
-  if (FUSE) {
-    if (grid_reduce<2>(acc, 2u, partial, counter) && threadIdx.x == 0) {
-      s->rpar = 1 - s->rpar;
-      s->rshift = 0.0;
-      if (pend) {
-        s->r2 = acc[0];
-        s->rinf = acc[1];
-        s->rz_old = s->rz;
-        s->iter += 1;
-        const bool met = acc[0] <= s->rel2 * s->r0_2 || (acc[0] <= s->abs2 && acc[1] <= s->ainf);
-        if (!(acc[0] == acc[0])) s->flag = 1;
-        if (met || s->iter >= s->max_iter || s->flag) s->done = 1;
-        s->pending = 0;
-      }
+  if (FUSE) block_partial<3, TBLOCK>(acc, 2u, partial);
+}
+
+// Finish of the fused finest pre kernel: residual buffers swap, the shift is
+// spent, and a pending update counts as an iteration with its convergence test
+// on the mean-free norm sum r^2 - (sum r)^2 / N (the recursive residual's
+// constant component is roundoff that no iteration reduces).
+__global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __restrict__ partial, int G,
+                                                             double ncells, Scal* s) {
+  if (s->done) return;
+  double a[3];
+  finish_reduce<3>(a, partial, G, 2u);
+  if (threadIdx.x == 0) {
+    s->rpar = 1 - s->rpar;
+    s->rshift = 0.0;
+    if (s->pending) {
+      a[0] = fmax(a[0] - a[2] * a[2] / ncells, 0.0);
+      s->r2 = a[0];
+      s->rinf = a[1];
+      s->rz_old = s->rz;
+      s->iter += 1;
+      const bool met = a[0] <= s->rel2 * s->r0_2 || (a[0] <= s->abs2 && a[1] <= s->ainf);
+      if (!(a[0] == a[0])) s->flag = 1;
+      if (met || s->iter >= s->max_iter || s->flag) s->done = 1;
+      s->pending = 0;
     }
   }
 }
 
 // --------------------------------------------------------------------------- post-smoothing tile kernel
-// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TX = 64 - 2H, H = 2 nu.
-// x = x_pre + P x_coarse, then nu (black, red) sweeps; the tile interior is
-// written to x_out.  LEVEL0: r = rbuf[s->rpar], x_out = z, and r.z, sum z,
-// sum r are reduced: zmean, rz = r.(z - zmean), beta.
+// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, TX = 64 - 2H, H = 2 nu.
+// x = x_pre + P x_coarse (the coarse values around the tile are staged in LDS),
+// then nu (black, red) sweeps; the tile interior is written to x_out.
+// LEVEL0: r = rbuf[s->rpar], x_out = z, and r.z, sum z, sum r are reduced:
+// zmean, rz = r.(z - zmean), beta.
+constexpr int PCW = 35;                           // staged coarse columns
+constexpr int PCH = (TY + 4 * MAX_NU) / 2 + 3;    // staged coarse rows
 template <class T, bool LEVEL0>
-__global__ void __launch_bounds__(BLOCK) k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB,
-                                                   const T* x_in, T* x_out, int nu, double* partial,
-                                                   unsigned* counter, Scal* s) {
+__global__ void __launch_bounds__(TBLOCK) k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA,
+                                                    const T* rB, const T* x_in, T* x_out, int nu, double* partial, Scal* s) {
   if (s->done) return;
   __shared__ T xs[RYMAX + 2][66];
+  __shared__ T xcs[PCH][PCW];
   const int H = 2 * nu, TX = 64 - 2 * H, RY = TY + 2 * H;
-  const int tx0 = (int)blockIdx.x * TX, ty0 = (int)blockIdx.y * TY;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int tx0 = bx * TX, ty0 = by * TY;
   const int gx0 = tx0 - H, gy0 = ty0 - H;
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int gi = gx0 + c;
-  const int nx = F.nx, ny = F.ny;
+  const int nx = F.nx, ny = F.ny, cnx = Cc.nx, cny = Cc.ny;
+  const int cxa = (gx0 >> 1) - 1, cya = (gy0 >> 1) - 1;  // first staged coarse column / row (floor)
   const T* rin = LEVEL0 ? (s->rpar ? rB : rA) : r_lvl;
+  constexpr int NCS = (PCH * PCW + TBLOCK - 1) / TBLOCK;
+  T cv[NCS];
+#pragma unroll
+  for (int q = 0; q < NCS; ++q) {
+    const int t = (int)threadIdx.x + q * TBLOCK;
+    cv[q] = 0;
+    if (t < PCH * PCW) {
+      const int I = min(max(cxa + t % PCW, 0), cnx - 1), J = min(max(cya + t / PCW, 0), cny - 1);
+      cv[q] = Cc.xb[ci(I, J, cnx)];
+    }
+  }
   TileCells<T> tc;
+  T x0r[MAXR], wxr[MAXR], wyr[MAXR];
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
-    const int r = w + 4 * m, gj = gy0 + r;
+    const int r = w + TWAVES * m, gj = gy0 + r;
     const bool v = r < RY && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
     tc.val[m] = v;
-    T rv = 0, bw = 0, be = 0, bsv = 0, bn = 0, x0 = 0;
+    T rv = 0, bw = 0, be = 0, bsv = 0, bn = 0, x0 = 0, wx = 0, wy = 0;
     if (v) {
       const size_t k = ci(gi, gj, nx);
       rv = rin[k];
@@ -403,16 +521,9 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(Level<T> F, Level<T> Cc, cons
       be = gi + 1 < nx ? F.bw[k + 1] : (T)0;
       bsv = F.bs[k];
       bn = gj + 1 < ny ? F.bs[k + nx] : (T)0;
-      // prolongation of the coarse correction
-      const int I = gi >> 1, J = gj >> 1;
-      int I2 = I + ((gi & 1) ? 1 : -1), J2 = J + ((gj & 1) ? 1 : -1);
-      I2 = I2 < 0 ? 0 : (I2 > Cc.nx - 1 ? Cc.nx - 1 : I2);
-      J2 = J2 < 0 ? 0 : (J2 > Cc.ny - 1 ? Cc.ny - 1 : J2);
-      const T wx = F.wx[k], wy = F.wy[k];
-      const T* xc = Cc.xb;
-      const T e = wx * wy * xc[ci(I, J, Cc.nx)] + ((T)1 - wx) * wy * xc[ci(I2, J, Cc.nx)] +
-                  wx * ((T)1 - wy) * xc[ci(I, J2, Cc.nx)] + ((T)1 - wx) * ((T)1 - wy) * xc[ci(I2, J2, Cc.nx)];
-      x0 = x_in[k] + e;
+      wx = F.wx[k];
+      wy = F.wy[k];
+      x0 = x_in[k];
     }
     tc.rr[m] = rv;
     tc.bW[m] = bw;
@@ -420,15 +531,42 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(Level<T> F, Level<T> Cc, cons
     tc.bS[m] = bsv;
     tc.bN[m] = bn;
     tc.inv[m] = v ? (T)1 / (bw + be + bsv + bn) : (T)0;
-    if (r < RY) xs[r + 1][c + 1] = x0;
+    x0r[m] = x0;
+    wxr[m] = wx;
+    wyr[m] = wy;
   }
-  for (int t = threadIdx.x; t < 66; t += BLOCK) {
+#pragma unroll
+  for (int q = 0; q < NCS; ++q) {
+    const int t = (int)threadIdx.x + q * TBLOCK;
+    if (t < PCH * PCW) xcs[t / PCW][t % PCW] = cv[q];
+  }
+  for (int t = threadIdx.x; t < 66; t += TBLOCK) {
     xs[0][t] = 0;
     xs[RY + 1][t] = 0;
   }
-  for (int t = threadIdx.x; t < RY; t += BLOCK) {
+  for (int t = threadIdx.x; t < RY; t += TBLOCK) {
     xs[t + 1][0] = 0;
     xs[t + 1][65] = 0;
+  }
+  __syncthreads();
+  // prolongation of the coarse correction
+#pragma unroll
+  for (int m = 0; m < MAXR; ++m) {
+    const int r = w + TWAVES * m, gj = gy0 + r;
+    if (r >= RY) continue;
+    T x0 = 0;
+    if (tc.val[m]) {
+      const int I = gi >> 1, J = gj >> 1;
+      int I2 = I + ((gi & 1) ? 1 : -1), J2 = J + ((gj & 1) ? 1 : -1);
+      I2 = I2 < 0 ? 0 : (I2 > cnx - 1 ? cnx - 1 : I2);
+      J2 = J2 < 0 ? 0 : (J2 > cny - 1 ? cny - 1 : J2);
+      const int a = I - cxa, a2 = I2 - cxa, b = J - cya, b2 = J2 - cya;
+      const T wx = wxr[m], wy = wyr[m];
+      const T e = wx * wy * xcs[b][a] + ((T)1 - wx) * wy * xcs[b][a2] + wx * ((T)1 - wy) * xcs[b2][a] +
+                  ((T)1 - wx) * ((T)1 - wy) * xcs[b2][a2];
+      x0 = x0r[m] + e;
+    }
+    xs[r + 1][c + 1] = x0;
   }
   __syncthreads();
   for (int sw = 0; sw < nu; ++sw)
@@ -439,7 +577,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(Level<T> F, Level<T> Cc, cons
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
-    const int r = w + 4 * m;
+    const int r = w + TWAVES * m;
     if (r >= RY || !tc.val[m] || c < H || c >= H + TX || r < H || r >= H + TY) continue;
     const T z = xs[r + 1][c + 1];
     x_out[ci(gi, gy0 + r, nx)] = z;
@@ -449,140 +587,183 @@ __global__ void __launch_bounds__(BLOCK) k_mg_post(Level<T> F, Level<T> Cc, cons
       acc[2] += (double)tc.rr[m];
     }
   }
-  if (LEVEL0) {
-    if (grid_reduce<3>(acc, 0u, partial, counter) && threadIdx.x == 0) {
-      const double N = (double)nx * (double)ny;
-      const double zm = acc[1] / N;
-      const double rz = acc[0] - zm * acc[2];
-      s->zmean = zm;
-      s->rz = rz;
-      s->beta = s->iter == s->seg0 ? 0.0 : rz / s->rz_old;
-    }
+  if (LEVEL0) block_partial<3, TBLOCK>(acc, 0u, partial);
+}
+
+// Finish of the finest post kernel: zmean, rz = r.(z - zmean), beta.
+__global__ void __launch_bounds__(FINISH_THREADS) k_fin_post0(const double* __restrict__ partial, int G, double N,
+                                                              Scal* s) {
+  if (s->done) return;
+  double a[3];
+  finish_reduce<3>(a, partial, G, 0u);
+  if (threadIdx.x == 0) {
+    const double zm = a[1] / N;
+    const double rz = a[0] - zm * a[2];
+    s->zmean = zm;
+    s->rz = rz;
+    s->beta = s->iter == s->seg0 ? 0.0 : rz / s->rz_old;
   }
 }
 
 // --------------------------------------------------------------------------- single-workgroup coarse V-cycle
-// Levels l0..L-1 (l0 >= 1; at most SMALL_MAX_CELLS cells on l0) in one
-// workgroup: r of level l0 and the coefficients of every level are staged in
-// LDS, the weights are recomputed from them, the bottom uses the dense
-// pseudo-inverse binv (global), and the result is written to Ls[l0].xb.
+// Levels l0..L-1 (l0 >= 1) in one workgroup.  LDS per level: x, r, bw, bs and,
+// above the bottom, the interpolation weights wx, wy (computed here from the
+// coefficients); one array of n(l0) for residuals.  The bottom applies the
+// dense pseudo-inverse binv (global); the result goes to Ls[l0].xb.
 template <class T>
 __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(const Level<T>* __restrict__ Ls, int l0, int L, int nu_pre,
                                                             int nu_post, const T* __restrict__ binv, const Scal* s) {
   if (s->done) return;
   extern __shared__ unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
+  __shared__ int dnx[MAXL], dny[MAXL], doff[MAXL];
+  __shared__ int tmp_off;
   const int tid = threadIdx.x;
-  // LDS layout per level: x, r, bw, bs (n_l each)
-  T* X[16];
-  T* R[16];
-  T* BW[16];
-  T* BS[16];
-  {
-    size_t off = 0;
+  if (tid == 0) {
+    int off = 0;
     for (int l = l0; l < L; ++l) {
-      const size_t n = (size_t)Ls[l].nx * Ls[l].ny;
-      X[l] = smem + off;
-      R[l] = smem + off + n;
-      BW[l] = smem + off + 2 * n;
-      BS[l] = smem + off + 3 * n;
-      off += 4 * n;
+      dnx[l] = Ls[l].nx;
+      dny[l] = Ls[l].ny;
+      doff[l] = off;
+      off += dnx[l] * dny[l] * (l + 1 < L ? 6 : 4);
     }
+    tmp_off = off;
   }
+  __syncthreads();
+  // array a of level l: 0 = x, 1 = r, 2 = bw, 3 = bs, 4 = wx, 5 = wy
+  auto arr = [&](int l, int a) { return smem + doff[l] + a * dnx[l] * dny[l]; };
+  T* tmp = smem + tmp_off;
   for (int l = l0; l < L; ++l) {
-    const int n = Ls[l].nx * Ls[l].ny;
+    const int n = dnx[l] * dny[l];
+    const T* gbw = Ls[l].bw;
+    const T* gbs = Ls[l].bs;
+    const T* gr = Ls[l].r;
+    T* BW = arr(l, 2);
+    T* BS = arr(l, 3);
+    T* R = arr(l, 1);
     for (int t = tid; t < n; t += SMALL_THREADS) {
-      BW[l][t] = Ls[l].bw[t];
-      BS[l][t] = Ls[l].bs[t];
-      if (l == l0) R[l][t] = Ls[l].r[t];
+      BW[t] = gbw[t];
+      BS[t] = gbs[t];
+      if (l == l0) R[t] = gr[t];
     }
   }
   __syncthreads();
-  auto diag = [&](int l, int i, int j) {
-    const int nx = Ls[l].nx, ny = Ls[l].ny;
-    const T e = i + 1 < nx ? BW[l][i + 1 + j * nx] : (T)0;
-    const T nn = j + 1 < ny ? BS[l][i + (j + 1) * nx] : (T)0;
-    return BW[l][i + j * nx] + e + BS[l][i + j * nx] + nn;
-  };
-  auto offs = [&](int l, int i, int j) {
-    const int nx = Ls[l].nx, ny = Ls[l].ny, k = i + j * nx;
-    const T* x = X[l];
-    T o = 0;
-    if (i > 0) o += BW[l][k] * x[k - 1];
-    if (i + 1 < nx) o += BW[l][k + 1] * x[k + 1];
-    if (j > 0) o += BS[l][k] * x[k - nx];
-    if (j + 1 < ny) o += BS[l][k + nx] * x[k + nx];
-    return o;
-  };
+  for (int l = l0; l + 1 < L; ++l) {
+    const int nx = dnx[l], ny = dny[l], n = nx * ny;
+    const T* BW = arr(l, 2);
+    const T* BS = arr(l, 3);
+    T* WX = arr(l, 4);
+    T* WY = arr(l, 5);
+    for (int t = tid; t < n; t += SMALL_THREADS) {
+      const int i = t % nx, j = t / nx;
+      auto fx = [&](int k) { return k < nx ? BW[k + j * nx] : (T)0; };
+      auto fy = [&](int k) { return k < ny ? BS[i + k * nx] : (T)0; };
+      T wx = od_weight<T>(fx, i, nx), wy = od_weight<T>(fy, j, ny);
+      const int Q = (i >> 1) + ((i & 1) ? 1 : -1), Qj = (j >> 1) + ((j & 1) ? 1 : -1);
+      if (Q < 0 || Q > (nx >> 1) - 1) wx = (T)1;
+      if (Qj < 0 || Qj > (ny >> 1) - 1) wy = (T)1;
+      WX[t] = wx;
+      WY[t] = wy;
+    }
+  }
+  __syncthreads();
   auto sweep = [&](int l, int color, bool zero) {
-    const int nx = Ls[l].nx, ny = Ls[l].ny, half = (nx + 1) >> 1;
+    const int nx = dnx[l], ny = dny[l], half = (nx + 1) >> 1;
+    T* X = arr(l, 0);
+    const T* R = arr(l, 1);
+    const T* BW = arr(l, 2);
+    const T* BS = arr(l, 3);
     for (int t = tid; t < half * ny; t += SMALL_THREADS) {
       const int j = t / half, i = 2 * (t % half) + ((j + color) & 1);
       if (i >= nx) continue;
       const int k = i + j * nx;
-      const T o = zero ? (T)0 : offs(l, i, j);
-      X[l][k] = (R[l][k] + o) / diag(l, i, j);
+      const T e = i + 1 < nx ? BW[k + 1] : (T)0, nn = j + 1 < ny ? BS[k + nx] : (T)0;
+      const T d = BW[k] + e + BS[k] + nn;
+      T o = 0;
+      if (!zero) {
+        if (i > 0) o += BW[k] * X[k - 1];
+        if (i + 1 < nx) o += e * X[k + 1];
+        if (j > 0) o += BS[k] * X[k - nx];
+        if (j + 1 < ny) o += nn * X[k + nx];
+      }
+      X[k] = (R[k] + o) / d;
     }
     __syncthreads();
   };
-  auto wts = [&](int l, int i, int j, T& wx, T& wy) {
-    const int nx = Ls[l].nx, ny = Ls[l].ny;
-    auto fx = [&](int k) { return k < nx ? BW[l][k + j * nx] : (T)0; };
-    auto fy = [&](int k) { return k < ny ? BS[l][i + k * nx] : (T)0; };
-    wx = od_weight<T>(fx, i, nx);
-    wy = od_weight<T>(fy, j, ny);
-    const int Q = (i >> 1) + ((i & 1) ? 1 : -1), Qj = (j >> 1) + ((j & 1) ? 1 : -1);
-    if (Q < 0 || Q > (nx >> 1) - 1) wx = (T)1;
-    if (Qj < 0 || Qj > (ny >> 1) - 1) wy = (T)1;
-  };
-  for (int l = l0; l < L - 1; ++l) {
+  for (int l = l0; l + 1 < L; ++l) {
     for (int sw = 0; sw < nu_pre; ++sw) {
       sweep(l, 0, sw == 0);
       sweep(l, 1, false);
     }
-    const int nx = Ls[l].nx, ny = Ls[l].ny, cnx = Ls[l + 1].nx, cny = Ls[l + 1].ny;
-    for (int t = tid; t < cnx * cny; t += SMALL_THREADS) {
-      const int I = t % cnx, J = t / cnx;
-      T sum = 0;
-      for (int j = 2 * J - 1; j <= 2 * J + 2; ++j) {
-        if (j < 0 || j >= ny) continue;
-        for (int i = 2 * I - 1; i <= 2 * I + 2; ++i) {
-          if (i < 0 || i >= nx) continue;
-          T wx, wy;
-          wts(l, i, j, wx, wy);
-          const T a = w1<T>(j, J, cny, wy), b = w1<T>(i, I, cnx, wx);
-          if (a == (T)0 || b == (T)0) continue;
-          const int k = i + j * nx;
-          const T res = R[l][k] - (diag(l, i, j) * X[l][k] - offs(l, i, j));
-          sum += b * a * res;
-        }
+    const int nx = dnx[l], ny = dny[l], n = nx * ny, cnx = dnx[l + 1], cny = dny[l + 1];
+    {
+      const T* X = arr(l, 0);
+      const T* R = arr(l, 1);
+      const T* BW = arr(l, 2);
+      const T* BS = arr(l, 3);
+      for (int k = tid; k < n; k += SMALL_THREADS) {
+        const int i = k % nx, j = k / nx;
+        const T e = i + 1 < nx ? BW[k + 1] : (T)0, nn = j + 1 < ny ? BS[k + nx] : (T)0;
+        T o = 0;
+        if (i > 0) o += BW[k] * X[k - 1];
+        if (i + 1 < nx) o += e * X[k + 1];
+        if (j > 0) o += BS[k] * X[k - nx];
+        if (j + 1 < ny) o += nn * X[k + nx];
+        tmp[k] = R[k] - ((BW[k] + e + BS[k] + nn) * X[k] - o);
       }
-      R[l + 1][t] = sum;
+    }
+    __syncthreads();
+    {
+      const T* WX = arr(l, 4);
+      const T* WY = arr(l, 5);
+      T* Rc = arr(l + 1, 1);
+      for (int t = tid; t < cnx * cny; t += SMALL_THREADS) {
+        const int I = t % cnx, J = t / cnx;
+        T sum = 0;
+        for (int j = 2 * J - 1; j <= 2 * J + 2; ++j) {
+          if (j < 0 || j >= ny) continue;
+          for (int i = 2 * I - 1; i <= 2 * I + 2; ++i) {
+            if (i < 0 || i >= nx) continue;
+            const int k = i + j * nx;
+            const T a = w1<T>(j, J, cny, WY[k]), b = w1<T>(i, I, cnx, WX[k]);
+            if (a == (T)0 || b == (T)0) continue;
+            sum += b * a * tmp[k];
+          }
+        }
+        Rc[t] = sum;
+      }
     }
     __syncthreads();
   }
   {
-    const int n = Ls[L - 1].nx * Ls[L - 1].ny;
-    for (int a = tid; a < n; a += SMALL_THREADS) {
+    // bottom: x = binv r, one wavefront per row
+    const int l = L - 1, n = dnx[l] * dny[l];
+    const T* R = arr(l, 1);
+    T* X = arr(l, 0);
+    const int lane = tid & 63;
+    for (int a = tid >> 6; a < n; a += SMALL_THREADS / 64) {
       double acc = 0;
-      for (int q = 0; q < n; ++q) acc += (double)binv[(size_t)a * n + q] * (double)R[L - 1][q];
-      X[L - 1][a] = (T)acc;
+      for (int q = lane; q < n; q += 64) acc += (double)binv[(size_t)a * n + q] * (double)R[q];
+      acc = wave_sum(acc);
+      if (lane == 0) X[a] = (T)acc;
     }
     __syncthreads();
   }
   for (int l = L - 2; l >= l0; --l) {
-    const int nx = Ls[l].nx, ny = Ls[l].ny, cnx = Ls[l + 1].nx, cny = Ls[l + 1].ny;
+    const int nx = dnx[l], ny = dny[l], cnx = dnx[l + 1], cny = dny[l + 1];
+    T* X = arr(l, 0);
+    const T* WX = arr(l, 4);
+    const T* WY = arr(l, 5);
+    const T* xc = arr(l + 1, 0);
     for (int t = tid; t < nx * ny; t += SMALL_THREADS) {
       const int i = t % nx, j = t / nx;
-      T wx, wy;
-      wts(l, i, j, wx, wy);
+      const T wx = WX[t], wy = WY[t];
       const int I = i >> 1, J = j >> 1;
       int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
       I2 = I2 < 0 ? 0 : (I2 > cnx - 1 ? cnx - 1 : I2);
       J2 = J2 < 0 ? 0 : (J2 > cny - 1 ? cny - 1 : J2);
-      const T* xc = X[l + 1];
-      X[l][t] += wx * wy * xc[I + J * cnx] + ((T)1 - wx) * wy * xc[I2 + J * cnx] + wx * ((T)1 - wy) * xc[I + J2 * cnx] +
-                 ((T)1 - wx) * ((T)1 - wy) * xc[I2 + J2 * cnx];
+      X[t] += wx * wy * xc[I + J * cnx] + ((T)1 - wx) * wy * xc[I2 + J * cnx] + wx * ((T)1 - wy) * xc[I + J2 * cnx] +
+              ((T)1 - wx) * ((T)1 - wy) * xc[I2 + J2 * cnx];
     }
     __syncthreads();
     for (int sw = 0; sw < nu_post; ++sw) {
@@ -590,58 +771,83 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(const Level<T>* __re
       sweep(l, 0, false);
     }
   }
-  const int n0 = Ls[l0].nx * Ls[l0].ny;
-  for (int t = tid; t < n0; t += SMALL_THREADS) Ls[l0].xb[t] = X[l0][t];
+  const int n0 = dnx[l0] * dny[l0];
+  const T* X0 = arr(l0, 0);
+  T* out = Ls[l0].xb;
+  for (int t = tid; t < n0; t += SMALL_THREADS) out[t] = X0[t];
 }
 
 // --------------------------------------------------------------------------- Krylov direction kernel
 // p = (z - mean z) + beta p_old (FP64 arithmetic, stored as T; p_old/p_new =
 // pbuf[iter & 1] / pbuf[(iter+1) & 1]), q = A0' p, alpha = rz / p.q; marks the
-// update pending.  Band kernel marching down rows with the rows above/below in
-// registers.  Grid dim3(ceil(nx/BLOCK), ceil(ny/BAND)).
+// update pending.  Tile kernel: p of a 64 x DTY tile plus a one-cell ring is
+// formed in LDS (all loads issued at once), then the stencil reads LDS.
+// Grid dim3(ceil(nx/64), ceil(ny/DTY)), BLOCK threads.
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restrict__ bw, const T* __restrict__ bs,
                                                const T* __restrict__ z, T* pA, T* pB, T* __restrict__ q,
-                                               double* partial, unsigned* counter, Scal* s) {
+                                               double* partial, Scal* s) {
   if (s->done) return;
+  __shared__ T ps[DTY + 2][66];
   const double beta = s->beta, zm = s->zmean;
   const bool first = s->iter == s->seg0;
   const T* pold = (s->iter & 1) ? pB : pA;
   T* pnew = (s->iter & 1) ? pA : pB;
-  const Coef<T> C{bw, bs, nx, ny};
-  const int i = (int)(blockIdx.x * BLOCK + threadIdx.x);
-  const int j0 = (int)(blockIdx.y * BAND), j1 = min(ny, j0 + BAND);
-  auto P = [&](size_t k) {
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * 64, j0 = by * DTY;
+  const int i = i0 + c;
+  auto P = [&](int ii, int jj) -> T {
+    if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) return (T)0;
+    const size_t k = ci(ii, jj, nx);
     return first ? (T)((double)z[k] - zm) : (T)(((double)z[k] - zm) + beta * (double)pold[k]);
   };
+  constexpr int NW = BLOCK / 64, RR = (DTY + 2 + NW - 1) / NW;
+#pragma unroll
+  for (int m = 0; m < RR; ++m) {
+    const int r = w + NW * m;
+    if (r < DTY + 2) ps[r][c + 1] = P(i, j0 - 1 + r);
+  }
+  if (threadIdx.x < 2 * (DTY + 2)) {
+    const int side = (int)threadIdx.x / (DTY + 2), r = (int)threadIdx.x % (DTY + 2);
+    ps[r][side ? 65 : 0] = P(side ? i0 + 64 : i0 - 1, j0 - 1 + r);
+  }
+  __syncthreads();
   double acc[1] = {0.0};
-  if (i < nx && j0 < j1) {
-    T ps = j0 > 0 ? P(ci(i, j0 - 1, nx)) : (T)0;
-    T pc = P(ci(i, j0, nx));
-    T pn = 0;
-    for (int j = j0; j < j1; ++j) {
+#pragma unroll
+  for (int m = 0; m < DTY / NW; ++m) {
+    const int r = w + NW * m, j = j0 + r;
+    if (i < nx && j < ny) {
       const size_t k = ci(i, j, nx);
-      if (j + 1 < ny) pn = P(k + nx);
+      const T pc = ps[r + 1][c + 1];
       T a = 0;
-      if (i > 0) a += C.W(i, j) * (pc - P(k - 1));
-      if (i + 1 < nx) a += C.E(i, j) * (pc - P(k + 1));
-      if (j > 0) a += C.S(i, j) * (pc - ps);
-      if (j + 1 < ny) a += C.N(i, j) * (pc - pn);
+      if (i > 0) a += bw[k] * (pc - ps[r + 1][c]);
+      if (i + 1 < nx) a += bw[k + 1] * (pc - ps[r + 1][c + 2]);
+      if (j > 0) a += bs[k] * (pc - ps[r][c + 1]);
+      if (j + 1 < ny) a += bs[k + nx] * (pc - ps[r + 2][c + 1]);
       pnew[k] = pc;
       q[k] = a;
       acc[0] += (double)pc * (double)a;
-      ps = pc;
-      pc = pn;
     }
   }
-  if (grid_reduce<1>(acc, 0u, partial, counter) && threadIdx.x == 0) {
-    s->pq = acc[0];
-    if (!(acc[0] > 0.0)) {
+  block_partial<1>(acc, 0u, partial);
+}
+
+// Finish of the direction kernel: alpha = rz / p.q and the update pending
+// (breakdown when p.q is not positive).
+__global__ void __launch_bounds__(FINISH_THREADS) k_fin_dir(const double* __restrict__ partial, int G, Scal* s) {
+  if (s->done) return;
+  double a[1];
+  finish_reduce<1>(a, partial, G, 0u);
+  if (threadIdx.x == 0) {
+    s->pq = a[0];
+    if (!(a[0] > 0.0)) {
       s->done = 1;
       s->flag = 1;
       s->alpha = 0.0;
     } else {
-      s->alpha = s->rz / acc[0];
+      s->alpha = s->rz / a[0];
       s->pending = 1;
     }
   }
