@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -56,6 +57,12 @@ inline dim3 fgrid(int nx, int ny) {
 }
 constexpr int MAX_PASSES = 32;
 inline size_t count(dim3 g) { return (size_t)g.x * g.y * g.z; }
+inline double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+inline double ms_since(std::chrono::steady_clock::time_point a) {
+  return ms_between(a, std::chrono::steady_clock::now());
+}
 
 template <class T>
 struct MG {
@@ -116,6 +123,7 @@ struct p2d_ctx {
   Scal* s = nullptr;
   Scal* hs = nullptr;  // pinned host mirror
   int last_seg[2][MAX_PASSES] = {};  // iterations of each pass of the last solve (FP64, mixed)
+  int last_passes[2] = {0, 0};       // refinement passes of the last solve
 };
 
 namespace {
@@ -225,56 +233,40 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
   return P2D_OK;
 }
 
-// Coarse FP64 coefficients of every level and the bottom pseudo-inverse.
-int mg_setup_common(p2d_ctx* c) {
-  MG<double>& m = c->mg64;
-  for (int l = 0; l + 1 < m.L; ++l) {
-    const Level<double>& f = m.lv[l];
-    const Level<double>& g = m.lv[l + 1];
-    k_mg_coarsen<<<dim3(blocks_for((size_t)g.nx), (unsigned)g.ny), BLOCK, 0, c->stream>>>(
-        f.nx, f.ny, f.bw, f.bs, const_cast<double*>(g.bw), const_cast<double*>(g.bs));
-  }
-  const Level<double>& B = m.lv[m.L - 1];
-  k_mg_bottom_inverse<<<1, SMALL_THREADS, 0, c->stream>>>(B.nx, B.ny, B.bw, B.bs, c->bottom_work, c->binv64,
-                                                          c->binv32);
-  c->mg_common = true;
-  return launched_ok();
-}
-
+// Multigrid setup for precision prec: per fine level one k_mg_setup sweep that
+// writes what is missing (the coarse FP64 coefficients shared by both
+// hierarchies, the FP32 coefficient copies, the interpolation weights), then
+// the bottom pseudo-inverse.
 template <class T>
-int mg_weights(p2d_ctx* c, MG<T>& mg) {
-  for (int l = 0; l < mg.l0; ++l) {
-    const Level<T>& d = mg.lv[l];
-    const Level<double>& m = c->mg64.lv[l];
-    k_mg_weights<T><<<dim3(blocks_for((size_t)d.nx), (unsigned)d.ny), BLOCK, 0, c->stream>>>(
-        d.nx, d.ny, m.bw, m.bs, const_cast<T*>(d.wx), const_cast<T*>(d.wy));
+int mg_setup(p2d_ctx* c, MG<T>& mg, bool cast32) {
+  MG<double>& m = c->mg64;
+  const bool common = c->mg_common;
+  for (int l = 0; l < m.L; ++l) {
+    const Level<double>& f = m.lv[l];
+    T* wx = l < mg.l0 ? const_cast<T*>(mg.lv[l].wx) : nullptr;
+    T* wy = l < mg.l0 ? const_cast<T*>(mg.lv[l].wy) : nullptr;
+    float* f32bw = cast32 ? const_cast<float*>(c->mg32.lv[l].bw) : nullptr;
+    float* f32bs = cast32 ? const_cast<float*>(c->mg32.lv[l].bs) : nullptr;
+    double* cbw = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bw) : nullptr;
+    double* cbs = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bs) : nullptr;
+    if (!wx && !f32bw && !cbw) continue;
+    k_mg_setup<T><<<dim3(blocks_for((size_t)f.nx), (unsigned)f.ny), BLOCK, 0, c->stream>>>(
+        f.nx, f.ny, f.bw, f.bs, wx, wy, f32bw, f32bs, cbw, cbs);
   }
-  return launched_ok();
-}
-
-int mg_cast32(p2d_ctx* c) {
-  for (int l = 0; l < c->mg32.L; ++l) {
-    const Level<float>& d = c->mg32.lv[l];
-    const Level<double>& m = c->mg64.lv[l];
-    const size_t n = (size_t)d.nx * d.ny;
-    k_cast<double, float><<<blocks_for(n), BLOCK, 0, c->stream>>>(n, m.bw, const_cast<float*>(d.bw));
-    k_cast<double, float><<<blocks_for(n), BLOCK, 0, c->stream>>>(n, m.bs, const_cast<float*>(d.bs));
+  if (!common) {
+    const Level<double>& B = m.lv[m.L - 1];
+    k_mg_bottom_inverse<<<1, SMALL_THREADS, 0, c->stream>>>(B.nx, B.ny, B.bw, B.bs, c->bottom_work, c->binv64,
+                                                            c->binv32);
+    c->mg_common = true;
   }
+  mg.valid = true;
   return launched_ok();
 }
 
 int ensure_mg(p2d_ctx* c, int prec) {
   if (!c->coeff_ready) return P2D_BAD_ARG;
-  if (!c->mg_common) CHK(mg_setup_common(c));
-  if (prec == P2D_PREC_F64 && !c->mg64.valid) {
-    CHK(mg_weights(c, c->mg64));
-    c->mg64.valid = true;
-  }
-  if (prec == P2D_PREC_F32 && !c->mg32.valid) {
-    CHK(mg_cast32(c));
-    CHK(mg_weights(c, c->mg32));
-    c->mg32.valid = true;
-  }
+  if (prec == P2D_PREC_F64 && !c->mg64.valid) CHK(mg_setup<double>(c, c->mg64, false));
+  if (prec == P2D_PREC_F32 && !c->mg32.valid) CHK(mg_setup<float>(c, c->mg32, true));
   return P2D_OK;
 }
 
@@ -333,7 +325,8 @@ struct SolveResult {
 };
 
 // Right-hand side of one solve: b' (velocity part, mean in s->bmean) and the
-// per-face jump terms (NULL: none).
+// per-face jump terms (NULL: none).  The initial residual r0 (p = 0) is already
+// in the solve precision's rbuf[0] and its norms in s->red (k_rhs, k_adj_rhs).
 struct Rhs {
   const double* b;
   const double* saw;
@@ -355,108 +348,114 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
             int* cur, double rtol, double atol, SolveResult* out) {
   const size_t N = c->N;
   const bool mixed = sizeof(T) == 4;
-  k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, solh[0], 0.0);
-  k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, soll[0], 0.0);
   *cur = 0;
-  // the first residual goes to rbuf[0] (k_inner_init resets the parity to 0)
-  correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, solh[0], soll[0], (const T*)nullptr, nullptr, nullptr, v.r[0]);
-  CHK(launched_ok());
-  CHK(sync_scal(c));
-  const double m0 = c->hs->red[2] / (double)N;
-  const double b2 = std::max(0.0, c->hs->red[0] - (double)N * m0 * m0);
-  double r2 = c->hs->red[0], rinf = c->hs->red[1];
-  auto converged = [&](double a2, double ainf) {
-    return (rtol <= 0 || a2 <= rtol * rtol * b2) && (atol <= 0 || ainf <= atol);
-  };
-  // Inner targets: FP64 runs to the targets (with a margin for the recursive
-  // residual) or 1e-14 relative; FP32 runs passes of inner_rtol relative
-  // reduction between FP64 residual replacements.
-  const double big = 1e300;
-  const double abs2 = rtol > 0 ? 0.25 * rtol * rtol * b2 : big;
-  const double ainf = atol > 0 ? 0.5 * atol : big;
+  // Inner targets: FP64 runs to the outer targets (with a margin for the
+  // recursive residual) or 1e-14 relative; FP32 runs passes of inner_rtol
+  // relative reduction between FP64 residual replacements.
   const double rel = mixed ? c->cfg.inner_rtol : 1e-14;
-  const double out2 = rtol > 0 ? rtol * rtol * b2 : -1.0, outi = atol > 0 ? atol : -1.0;
   const bool verbose = std::getenv("P2D_VERBOSE") != nullptr;
-  if (verbose)
-    fprintf(stderr, "p2d solve: %s b2=%.3e rtol=%.1e atol=%.3e r0: r2=%.3e rinf=%.3e\n", mixed ? "mixed" : "fp64", b2,
-            rtol, atol, r2, rinf);
-  int status = P2D_OK, outer = 0, stalled = 0, iters = 0;
-  double best_r2 = r2, best_rinf = rinf;
+  const auto t_solve = std::chrono::steady_clock::now();
+  k_inner_init<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? 0 : 1, rtol, atol, c->cfg.max_iterations);
+  k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
+  CHK(launched_ok());
+  int status = P2D_OK, outer = 0, stalled = 0, iters = 0, seg_start = 0;
+  double r2 = 0, rinf = 0, b2 = 0, best_r2 = 1e300, best_rinf = 1e300;
   int* last = c->last_seg[slot];
+  const int last_passes = c->last_passes[slot];
   auto predict = [&](int pass) { return std::max(2, (pass < MAX_PASSES && last[pass] > 0 ? last[pass] : 6) + 1); };
-  if (!converged(r2, rinf) && b2 > 0) {
-    k_inner_init<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? -1.0 : abs2, mixed ? -1.0 : ainf,
-                                         c->cfg.max_iterations);
-    k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
+  int chunk = predict(0);
+  bool report = false;  // a refinement's norms wait in red (the iterations do not touch red)
+  bool first = true;
+  while (true) {
+    const auto t_launch0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < chunk; ++k) cg_iteration<T>(c, mg, v);
     CHK(launched_ok());
-    int chunk = predict(0), seg_start = 0;
-    bool report = false;  // a refinement's norms wait in red (iterations do not touch red)
-    while (true) {
-      for (int k = 0; k < chunk; ++k) cg_iteration<T>(c, mg, v);
-      CHK(launched_ok());
-      CHK(sync_scal(c));
-      iters = c->hs->iter;
+    const auto t_launch1 = std::chrono::steady_clock::now();
+    CHK(sync_scal(c));
+    iters = c->hs->iter;
+    if (first) {
+      first = false;
+      b2 = c->hs->b2;
+      best_r2 = r2 = c->hs->r0_2;
+      best_rinf = rinf = c->hs->red[1];
       if (verbose)
-        fprintf(stderr, "    sync: iter %d done %d recursive r2=%.3e (rel %.3e of segment start) rinf=%.3e\n",
-                c->hs->iter, c->hs->done, c->hs->r2, std::sqrt(c->hs->r2 / std::max(c->hs->r0_2, 1e-300)),
-                c->hs->rinf);
-      if (report) {
-        report = false;
-        r2 = c->hs->red[0];
-        rinf = c->hs->red[1];
-        if (verbose)
-          fprintf(stderr, "  pass %d: %d its -> true r2=%.3e (rel %.3e) rinf=%.3e\n", outer, seg_start, r2,
-                  std::sqrt(r2 / std::max(b2, 1e-300)), rinf);
-        if (!(r2 == r2)) {
-          status = P2D_NAN;
-          break;
-        }
-        // a pass that improves neither norm by 2x has hit the rounding floor of the
-        // FP64 residual (~eps max |beta grad p| per cell): stop after the next refinement
-        if (r2 > 0.25 * best_r2 && rinf > 0.5 * best_rinf) {
-          ++stalled;
-        } else {
-          stalled = 0;
-        }
-        best_r2 = std::min(best_r2, r2);
-        best_rinf = std::min(best_rinf, rinf);
-      }
-      if (c->hs->flag) {
+        fprintf(stderr, "p2d solve: %s b2=%.3e rtol=%.1e atol=%.3e r0 rinf=%.3e\n", mixed ? "mixed" : "fp64", b2, rtol,
+                atol, rinf);
+    }
+    if (verbose)
+      fprintf(stderr, "    sync at %.3f ms (launch %.3f ms, wait %.3f ms): iter %d done %d recursive r2=%.3e rinf=%.3e\n",
+              ms_since(t_solve), ms_between(t_launch0, t_launch1), ms_since(t_launch1), c->hs->iter, c->hs->done,
+              c->hs->r2, c->hs->rinf);
+    if (report) {
+      report = false;
+      r2 = c->hs->red[0];
+      rinf = c->hs->red[1];
+      if (verbose)
+        fprintf(stderr, "  pass %d: %d its -> true r2=%.3e (rel %.3e) rinf=%.3e\n", outer, seg_start, r2,
+                std::sqrt(r2 / std::max(b2, 1e-300)), rinf);
+      if (!(r2 == r2)) {
         status = P2D_NAN;
         break;
       }
-      if (c->hs->outer_done) break;
-      if (!c->hs->done) {
-        chunk = 2;
-        continue;
+      // a pass that improves neither norm by 2x has hit the rounding floor of the
+      // FP64 residual (~eps max |beta grad p| per cell): stop after the next refinement
+      if (r2 > 0.25 * best_r2 && rinf > 0.5 * best_rinf) {
+        ++stalled;
+      } else {
+        stalled = 0;
       }
-      // end of a pass: refinement p += x (double-double), the true residual into the
-      // buffer the next pass reads
-      if (outer < MAX_PASSES) last[outer] = iters - seg_start;
-      T* rdst = c->hs->rpar ? v.r[1] : v.r[0];
-      correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, solh[*cur], soll[*cur], v.x, solh[1 - *cur], soll[1 - *cur], rdst);
-      *cur = 1 - *cur;
-      ++outer;
-      if (iters >= c->cfg.max_iterations || outer >= c->cfg.max_outer || stalled >= 1) {
-        CHK(launched_ok());
-        CHK(sync_scal(c));
-        r2 = c->hs->red[0];
-        rinf = c->hs->red[1];
-        if (verbose)
-          fprintf(stderr, "  pass %d: %d its -> true r2=%.3e (rel %.3e) rinf=%.3e (last)\n", outer, iters, r2,
-                  std::sqrt(r2 / std::max(b2, 1e-300)), rinf);
-        if (!(r2 == r2)) status = P2D_NAN;
-        break;
-      }
-      k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
-      k_inner_continue<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? -1.0 : abs2, mixed ? -1.0 : ainf,
-                                               out2, outi);
-      CHK(launched_ok());
-      report = true;
-      seg_start = iters;
-      chunk = predict(outer);
+      best_r2 = std::min(best_r2, r2);
+      best_rinf = std::min(best_rinf, rinf);
     }
+    if (c->hs->flag) {
+      status = P2D_NAN;
+      break;
+    }
+    if (c->hs->outer_done) break;
+    if (!c->hs->done) {
+      chunk = chunk == 0 ? predict(outer) : 2;  // after a look at the outer test: the predicted pass length
+      continue;
+    }
+    // end of a pass: refinement p += x (double-double), the true residual into the
+    // buffer the next pass reads
+    if (outer < MAX_PASSES) last[outer] = iters - seg_start;
+    T* rdst = c->hs->rpar ? v.r[1] : v.r[0];
+    const double* oh = outer > 0 ? solh[*cur] : nullptr;  // the first pass starts from p = 0
+    const double* ol = outer > 0 ? soll[*cur] : nullptr;
+    correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, oh, ol, v.x, solh[1 - *cur], soll[1 - *cur], rdst);
+    *cur = 1 - *cur;
+    ++outer;
+    if (iters >= c->cfg.max_iterations || outer >= c->cfg.max_outer || stalled >= 1) {
+      CHK(launched_ok());
+      CHK(sync_scal(c));
+      r2 = c->hs->red[0];
+      rinf = c->hs->red[1];
+      if (verbose)
+        fprintf(stderr, "  pass %d: %d its -> true r2=%.3e (rel %.3e) rinf=%.3e (last)\n", outer, iters, r2,
+                std::sqrt(r2 / std::max(b2, 1e-300)), rinf);
+      if (!(r2 == r2)) status = P2D_NAN;
+      break;
+    }
+    k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
+    k_inner_continue<<<1, 1, 0, c->stream>>>(c->s, (double)N);
+    CHK(launched_ok());
+    report = true;
+    seg_start = iters;
+    // when the previous solve ended after this many passes, look at the outer test
+    // before enqueueing more iterations
+    chunk = outer >= last_passes ? 0 : predict(outer);
   }
+  c->last_passes[slot] = outer;
+  if (outer == 0) {  // no refinement ran: p = 0
+    k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, solh[*cur], 0.0);
+    k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, soll[*cur], 0.0);
+    CHK(launched_ok());
+    r2 = c->hs->red[0];
+    rinf = c->hs->red[1];
+  }
+  auto converged = [&](double a2, double ai) {
+    return (rtol <= 0 || a2 <= rtol * rtol * b2) && (atol <= 0 || ai <= atol);
+  };
   if (status == P2D_OK && !converged(r2, rinf)) status = P2D_NOT_CONVERGED;
   out->status = status;
   out->iterations = iters;
@@ -529,13 +528,13 @@ void p2d_config_default(p2d_config* cfg, int nx, int ny, double h) {
   cfg->h = h;
   cfg->precision = P2D_FP64;
   cfg->mg_max_levels = 0;
-  cfg->mg_pre_sweeps = 2;
-  cfg->mg_post_sweeps = 2;
+  cfg->mg_pre_sweeps = 3;
+  cfg->mg_post_sweeps = 3;
   cfg->rtol = 1e-10;
   cfg->div_tol = 1e-10;
   cfg->max_iterations = 1000;
   cfg->max_outer = 12;
-  cfg->inner_rtol = 1e-5;
+  cfg->inner_rtol = 1e-4;
 }
 
 const char* p2d_status_string(int st) {
@@ -724,8 +723,13 @@ int p2d_project(p2d_ctx* c, double* u, double* v, double dt, p2d_stats* st, int 
   const double* sas = c->has_jc ? c->sas : nullptr;
   if (c->has_jc)
     k_jump<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->phi, c->jc, c->saw, c->sas);
-  k_rhs<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, c->b, c->partial);
-  k_fin_mean<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx, c->ny)), (double)c->N, c->s);
+  if (c->cfg.precision == P2D_MIXED)
+    k_rhs<float><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, c->bw, c->bs, saw, sas,
+                                                               c->b, c->v32.r[0], c->partial);
+  else
+    k_rhs<double><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, c->bw, c->bs, saw, sas,
+                                                                c->b, c->v64.r[0], c->partial);
+  k_fin_rhs<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx, c->ny)), (double)c->N, c->s);
   CHK(launched_ok());
   const double atol = c->cfg.div_tol > 0 ? 0.5 * c->cfg.div_tol * c->h * c->h / dt : 0.0;
   SolveResult sr;
@@ -797,8 +801,13 @@ int p2d_adjoint(p2d_ctx* c, const double* gu, const double* gv, double* du, doub
     g1 = c->sg1;
     g2 = c->sg2;
   }
-  k_adj_rhs<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, g1, g2, c->bw, c->bs, c->b, c->partial);
-  k_fin_mean<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx, c->ny)), (double)c->N, c->s);
+  if (c->cfg.precision == P2D_MIXED)
+    k_adj_rhs<float><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, g1, g2, c->bw, c->bs, c->b,
+                                                                   c->v32.r[0], c->partial);
+  else
+    k_adj_rhs<double><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, g1, g2, c->bw, c->bs, c->b,
+                                                                    c->v64.r[0], c->partial);
+  k_fin_rhs<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx, c->ny)), (double)c->N, c->s);
   CHK(launched_ok());
   SolveResult sr;
   int mcur = 0;

@@ -73,6 +73,9 @@ struct Scal {
   double ainf;
   double bmean;   // mean of the right-hand side (removed: A0 is singular on constants)
   double zmean;   // mean of the preconditioned residual (removed: keeps iterates mean-free)
+  double b2;      // mean-free ||r0||^2 of the solve (the relative target's reference)
+  double out2;    // outer targets: sum r^2 <= out2, max |r| <= outi (< 0: disabled)
+  double outi;
   double rshift;  // mean of a replaced residual, removed by the next fused update (its constant
                   // component is roundoff that no iterate can reduce: A0 p has zero sum)
   double red[8];  // results of stand-alone reductions

@@ -109,38 +109,61 @@ __global__ void __launch_bounds__(BLOCK) k_cells_to_faces(int nx, int ny, const 
 }
 
 // --------------------------------------------------------------------------- right-hand side
-// Mean of the right-hand side (removed by the residual kernel, A0 being
-// singular on constants).
-__global__ void __launch_bounds__(FINISH_THREADS) k_fin_mean(const double* __restrict__ partial, int G, double N,
+// Right-hand-side statistics (block partials sum r0^2, max |r0|, sum r0, sum b):
+// the initial residual's norms into s->red[0..2] and the mean of b' (removed by
+// the residual kernel, A0 being singular on constants) into s->bmean.
+__global__ void __launch_bounds__(FINISH_THREADS) k_fin_rhs(const double* __restrict__ partial, int G, double N,
                                                              Scal* s) {
-  double a[1];
-  finish_reduce<1>(a, partial, G, 0u);
-  if (threadIdx.x == 0) s->bmean = a[0] / N;
+  double a[4];
+  finish_reduce<4>(a, partial, G, 2u);
+  if (threadIdx.x == 0) {
+    s->red[0] = a[0];
+    s->red[1] = a[1];
+    s->red[2] = a[2];
+    s->bmean = a[3] / N;
+  }
 }
 
 // Velocity part of b' = h^2 (-D u/dt + D(B J)), the reference's b = -div - F
-// (POISSON_SOLVER.cpp:951) with J_f = s_f a_f / h; the jump part D(B J) enters
-// the residual face by face (fdiff).  u: (nx+1) x ny x-faces, v: nx x (ny+1)
-// y-faces.  Block partials: sum b.
+// (POISSON_SOLVER.cpp:951) with J_f = s_f a_f / h: b = -h^2 D u/dt; the jump
+// part enters the residual face by face (fdiff).  Also the initial residual
+// (p = 0) r0 = b - sum_f +-beta_f s_f a_f, stored as R for the first solve
+// pass (its mean is removed there).  u: (nx+1) x ny x-faces, v: nx x (ny+1)
+// y-faces; saw/sas may be NULL (no jump).
+template <class R>
 __global__ void __launch_bounds__(BLOCK) k_rhs(int nx, int ny, double h, double dt, const double* __restrict__ u,
-                                               const double* __restrict__ v, double* __restrict__ b,
-                                               double* partial) {
+                                               const double* __restrict__ v, const double* __restrict__ bw,
+                                               const double* __restrict__ bs, const double* __restrict__ saw,
+                                               const double* __restrict__ sas, double* __restrict__ b,
+                                               R* __restrict__ r0, double* partial) {
 #pragma clang fp contract(off)
   const double ih = 1.0 / h, idt = 1.0 / dt, h2 = h * h;
-  double acc[1] = {0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   P2D_BAND(ny);
   for (int j = j0; j < j1 && i < nx; ++j) {
     const size_t k = ci(i, j, nx);
     const double d = (ih * (u[fxi(i + 1, j, nx)] - u[fxi(i, j, nx)]) + ih * (v[fyi(i, j + 1, nx)] - v[fyi(i, j, nx)])) * idt;
     const double bv = h2 * (-d);
+    double g = 0.0;
+    if (saw) {
+      if (i > 0) g += bw[k] * saw[k];
+      if (i + 1 < nx) g -= bw[k + 1] * saw[k + 1];
+      if (j > 0) g += bs[k] * sas[k];
+      if (j + 1 < ny) g -= bs[k + nx] * sas[k + nx];
+    }
+    const double rv = bv - g;
     b[k] = bv;
-    acc[0] += bv;
+    r0[k] = (R)rv;
+    acc[0] += rv * rv;
+    acc[1] = fmax(acc[1], fabs(rv));
+    acc[2] += rv;
+    acc[3] += bv;
   }
-  block_partial<1>(acc, 0u, partial);
+  block_partial<4>(acc, 2u, partial);
 }
 
 // --------------------------------------------------------------------------- FP64 refinement step
-// On the double-double pressure: p_new = p_old + e (e may be NULL),
+// On the double-double pressure: p_new = p_old + e (e may be NULL; ph/pl NULL: p_old = 0),
 // r = (b' - mean b') - sum_f +-G_f with G_f = beta_f fdiff(p, s a) (west and
 // south faces +, east and north -), stored as R; block partials sum r^2,
 // max |r|, sum r.  saw/sas may be NULL (no jump).  Grid dim3(ceil(nx/64),
@@ -166,7 +189,9 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
     double hh = 0.0, ll = 0.0;
     if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
       const size_t q = ci(ii, jj, nx);
-      if (e) {
+      if (!ph) {
+        hh = e ? (double)e[q] : 0.0;
+      } else if (e) {
         dd_add(ph[q], pl[q], (double)e[q], hh, ll);
       } else {
         hh = ph[q];
@@ -278,15 +303,17 @@ __global__ void __launch_bounds__(BLOCK) k_shift(size_t N, const double* __restr
 }
 
 // --------------------------------------------------------------------------- adjoint
-// b' = -h^2 D(B g) with g taken on interior faces only; block partials sum b.
+// b' = -h^2 D(B g) with g taken on interior faces only; r0 = b' (no jump) as R;
+// block partials as k_rhs.
+template <class R>
 __global__ void __launch_bounds__(BLOCK) k_adj_rhs(int nx, int ny, double h, const double* __restrict__ gu,
                                                    const double* __restrict__ gv, const double* __restrict__ bw,
                                                    const double* __restrict__ bs, double* __restrict__ b,
-                                                   double* partial) {
+                                                   R* __restrict__ r0, double* partial) {
 #pragma clang fp contract(off)
   const double ih = 1.0 / h, h2 = h * h;
   const Coef<double> C{bw, bs, nx, ny};
-  double acc[1] = {0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   P2D_BAND(ny);
   for (int j = j0; j < j1 && i < nx; ++j) {
     const size_t k = ci(i, j, nx);
@@ -296,9 +323,13 @@ __global__ void __launch_bounds__(BLOCK) k_adj_rhs(int nx, int ny, double h, con
     const double gt = j + 1 < ny ? C.N(i, j) * gv[fyi(i, j + 1, nx)] : 0.0;
     const double bv = h2 * (-((gr - gl) * ih + (gt - gb) * ih));
     b[k] = bv;
-    acc[0] += bv;
+    r0[k] = (R)bv;
+    acc[0] += bv * bv;
+    acc[1] = fmax(acc[1], fabs(bv));
+    acc[2] += bv;
+    acc[3] += bv;
   }
-  block_partial<1>(acc, 0u, partial);
+  block_partial<4>(acc, 2u, partial);
 }
 
 // du = g - G mu, dbeta = -dt du (G p + s a / h) on interior faces; walls 0.

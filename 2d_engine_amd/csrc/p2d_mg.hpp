@@ -81,24 +81,6 @@ __device__ __forceinline__ T od_weight(F face, int i, int nf) {
 }
 
 // --------------------------------------------------------------------------- multigrid setup (FP64 masters)
-// Parent weights of every cell of a fine level, computed in FP64, stored as T.
-template <class T>
-__global__ void __launch_bounds__(BLOCK) k_mg_weights(int nx, int ny, const double* __restrict__ bw,
-                                                      const double* __restrict__ bs, T* __restrict__ wx,
-                                                      T* __restrict__ wy) {
-  const int i = (int)(blockIdx.x * BLOCK + threadIdx.x), j = (int)blockIdx.y;
-  if (i >= nx) return;
-  auto fx = [&](int k) { return k < nx ? bw[ci(k, j, nx)] : 0.0; };
-  auto fy = [&](int k) { return k < ny ? bs[ci(i, k, nx)] : 0.0; };
-  double a = od_weight<double>(fx, i, nx);
-  double b = od_weight<double>(fy, j, ny);
-  const int Q = (i >> 1) + ((i & 1) ? 1 : -1), Qj = (j >> 1) + ((j & 1) ? 1 : -1);
-  if (Q < 0 || Q > (nx >> 1) - 1) a = 1.0;
-  if (Qj < 0 || Qj > (ny >> 1) - 1) b = 1.0;
-  wx[ci(i, j, nx)] = (T)a;
-  wy[ci(i, j, nx)] = (T)b;
-}
-
 // Coarse face = average over the two fine rows of the series conductance
 // 2 / (0.5/b_{k-1} + 1/b_k + 0.5/b_{k+1}) between the two coarse centres.
 template <class F>
@@ -107,17 +89,41 @@ __device__ __forceinline__ double coarse_face(F face, int k, int nf) {
   const double r = 0.5 / face(k - 1) + 1.0 / face(k) + 0.5 / face(k + 1);
   return 2.0 / r;
 }
-// Grid: dim3(ceil(cnx/BLOCK), cny).
-__global__ void __launch_bounds__(BLOCK) k_mg_coarsen(int fnx, int fny, const double* __restrict__ fbw,
-                                                      const double* __restrict__ fbs, double* __restrict__ cbw,
-                                                      double* __restrict__ cbs) {
-  const int cnx = fnx >> 1;  // the grid covers the fny / 2 coarse rows
-  const int I = (int)(blockIdx.x * BLOCK + threadIdx.x), J = (int)blockIdx.y;
-  if (I >= cnx) return;
-  auto row = [&](int j) { return [=](int k) { return k < fnx ? fbw[ci(k, j, fnx)] : 0.0; }; };
-  auto col = [&](int i) { return [=](int k) { return k < fny ? fbs[ci(i, k, fnx)] : 0.0; }; };
-  cbw[ci(I, J, cnx)] = 0.5 * (coarse_face(row(2 * J), 2 * I, fnx) + coarse_face(row(2 * J + 1), 2 * I, fnx));
-  cbs[ci(I, J, cnx)] = 0.5 * (coarse_face(col(2 * I), 2 * J, fny) + coarse_face(col(2 * I + 1), 2 * J, fny));
+
+// One setup sweep over a fine level (FP64 coefficients bw, bs), every output
+// optional (NULL): the parent weights of each cell, computed in FP64 and
+// stored as T; the FP32 copy of the coefficients; and, from the thread of the
+// lower-left child of each coarse cell, the coarse FP64 coefficients.
+// Grid dim3(ceil(nx/BLOCK), ny).
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_mg_setup(int nx, int ny, const double* __restrict__ bw,
+                                                    const double* __restrict__ bs, T* __restrict__ wx,
+                                                    T* __restrict__ wy, float* __restrict__ f32bw,
+                                                    float* __restrict__ f32bs, double* __restrict__ cbw,
+                                                    double* __restrict__ cbs) {
+  const int i = (int)(blockIdx.x * BLOCK + threadIdx.x), j = (int)blockIdx.y;
+  if (i >= nx) return;
+  const size_t k = ci(i, j, nx);
+  auto row = [&](int jj) { return [=](int q) { return q < nx ? bw[ci(q, jj, nx)] : 0.0; }; };
+  auto col = [&](int ii) { return [=](int q) { return q < ny ? bs[ci(ii, q, nx)] : 0.0; }; };
+  if (wx) {
+    double a = od_weight<double>(row(j), i, nx);
+    double b = od_weight<double>(col(i), j, ny);
+    const int Q = (i >> 1) + ((i & 1) ? 1 : -1), Qj = (j >> 1) + ((j & 1) ? 1 : -1);
+    if (Q < 0 || Q > (nx >> 1) - 1) a = 1.0;
+    if (Qj < 0 || Qj > (ny >> 1) - 1) b = 1.0;
+    wx[k] = (T)a;
+    wy[k] = (T)b;
+  }
+  if (f32bw) {
+    f32bw[k] = (float)bw[k];
+    f32bs[k] = (float)bs[k];
+  }
+  if (cbw && (i & 1) == 0 && (j & 1) == 0) {
+    const int I = i >> 1, J = j >> 1, cnx = nx >> 1;
+    cbw[ci(I, J, cnx)] = 0.5 * (coarse_face(row(2 * J), 2 * I, nx) + coarse_face(row(2 * J + 1), 2 * I, nx));
+    cbs[ci(I, J, cnx)] = 0.5 * (coarse_face(col(2 * I), 2 * J, ny) + coarse_face(col(2 * I + 1), 2 * J, ny));
+  }
 }
 
 // Pseudo-inverse of the singular bottom operator A (null space = constants):
@@ -228,12 +234,24 @@ __device__ __forceinline__ void segment_start(Scal* s, double ncells) {
   s->r2 = s->r0_2;
   s->rinf = s->red[1];
 }
-__global__ void k_inner_init(Scal* s, double ncells, double rel2, double abs2, double ainf, int max_iter) {
+// Start of a solve from the initial residual's norms in red: the reference
+// norm b2 (mean-free), the outer targets (rtol on ||r||_2 relative to b2, atol
+// on max |r|; <= 0 disables) and the first segment's inner targets: relative
+// rel2 and, when inner_abs, absolute ones with a margin for the recursive
+// residual (0.25 out2, 0.5 outi).  Sets outer_done when r0 already meets the
+// outer targets.
+__global__ void k_inner_init(Scal* s, double ncells, double rel2, int inner_abs, double rtol, double atol,
+                             int max_iter) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double m = s->red[2] / ncells;
+  const double b2 = fmax(s->red[0] - ncells * m * m, 0.0);
+  s->b2 = b2;
+  s->out2 = rtol > 0.0 ? rtol * rtol * b2 : -1.0;
+  s->outi = atol > 0.0 ? atol : -1.0;
   segment_start(s, ncells);
   s->rel2 = rel2;
-  s->abs2 = abs2;
-  s->ainf = ainf;
+  s->abs2 = inner_abs && s->out2 >= 0.0 ? 0.25 * s->out2 : (inner_abs ? 1e300 : -1.0);
+  s->ainf = inner_abs && s->outi >= 0.0 ? 0.5 * s->outi : (inner_abs ? 1e300 : -1.0);
   s->max_iter = max_iter;
   s->iter = 0;
   s->seg0 = 0;
@@ -242,40 +260,42 @@ __global__ void k_inner_init(Scal* s, double ncells, double rel2, double abs2, d
   s->pending = 0;
   s->rpar = 0;
   s->rz = s->rz_old = s->pq = s->alpha = s->beta = s->zmean = 0.0;
-  const bool met = (s->r2 <= abs2 && s->rinf <= ainf) || s->r0_2 == 0.0;
-  s->done = (met || max_iter <= 0) ? 1 : 0;
-  if (!(s->r2 == s->r2)) {
+  if (!(s->red[0] == s->red[0])) {
     s->done = 1;
     s->flag = 1;
+    return;
   }
+  if (((s->out2 < 0.0 || s->red[0] <= s->out2) && (s->outi < 0.0 || s->red[1] <= s->outi)) || b2 == 0.0) {
+    s->outer_done = 1;
+    s->done = 1;
+    return;
+  }
+  const bool met = (s->r2 <= s->abs2 && s->rinf <= s->ainf) || s->r0_2 == 0.0;
+  s->done = (met || max_iter <= 0) ? 1 : 0;
 }
 // Continue after a residual replacement (red = norms of the true residual):
-// stop when the outer targets hold (out2 on sum r^2, outi on max |r|; < 0
-// disables), else start a new segment that keeps the iteration count and
+// stop when the outer targets hold, else start a new segment (same inner
+// targets as the first) that keeps the iteration count and
 // begins a new direction sequence (beta = 0).  The recursive residual of the
 // finished segment may have drifted far from the true one (FP32, or FP64 with
 // large pressure levels); with the old direction and rz_old kept, beta would
 // be rz_new / rz_old >> 1 and the stale direction would dominate.
-__global__ void k_inner_continue(Scal* s, double ncells, double rel2, double abs2, double ainf, double out2,
-                                 double outi) {
+__global__ void k_inner_continue(Scal* s, double ncells) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (!(s->red[0] == s->red[0])) {
     s->flag = 1;
     s->done = 1;
     return;
   }
-  if ((out2 < 0.0 || s->red[0] <= out2) && (outi < 0.0 || s->red[1] <= outi)) {
+  if ((s->out2 < 0.0 || s->red[0] <= s->out2) && (s->outi < 0.0 || s->red[1] <= s->outi)) {
     s->outer_done = 1;
     s->done = 1;
     return;
   }
   s->seg0 = s->iter;
   segment_start(s, ncells);
-  s->rel2 = rel2;
-  s->abs2 = abs2;
-  s->ainf = ainf;
   s->pending = 0;
-  const bool met = (s->r2 <= abs2 && s->rinf <= ainf) || s->r0_2 == 0.0;
+  const bool met = (s->r2 <= s->abs2 && s->rinf <= s->ainf) || s->r0_2 == 0.0;
   s->done = (met || s->iter >= s->max_iter) ? 1 : 0;
 }
 
@@ -288,17 +308,27 @@ struct TileCells {
   bool val[MAXR];
 };
 
+// One red-black half-sweep over the region held by this thread.  Cells of one
+// colour read only cells of the other colour, so all neighbour reads of the
+// half-sweep are gathered first (one LDS round trip for all rows), then the
+// updated values are written.
 template <class T>
 __device__ __forceinline__ void tile_sweep(TileCells<T>& tc, T (*xs)[66], int c, int w, int RY, int gi, int gy0,
                                            int color, bool zero) {
+  T off[MAXR];
+#pragma unroll
+  for (int m = 0; m < MAXR; ++m) {
+    const int r = w + TWAVES * m;
+    off[m] = 0;
+    if (zero || r >= RY) continue;
+    off[m] = tc.bW[m] * xs[r + 1][c] + tc.bE[m] * xs[r + 1][c + 2] + tc.bS[m] * xs[r][c + 1] +
+             tc.bN[m] * xs[r + 2][c + 1];
+  }
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
     const int r = w + TWAVES * m;
     if (r >= RY || !tc.val[m] || ((gi + gy0 + r) & 1) != color) continue;
-    const T off = zero ? (T)0
-                       : tc.bW[m] * xs[r + 1][c] + tc.bE[m] * xs[r + 1][c + 2] + tc.bS[m] * xs[r][c + 1] +
-                             tc.bN[m] * xs[r + 2][c + 1];
-    xs[r + 1][c + 1] = (tc.rr[m] + off) * tc.inv[m];
+    xs[r + 1][c + 1] = (tc.rr[m] + off[m]) * tc.inv[m];
   }
 }
 

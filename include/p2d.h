@@ -55,13 +55,13 @@ typedef struct p2d_config {
   double h;            /* cell size */
   int precision;       /* P2D_FP64: FP64 MG-PCG; P2D_MIXED: FP32 MG-PCG inside FP64 refinement */
   int mg_max_levels;   /* 0: coarsen until <= 64 cells or an odd size */
-  int mg_pre_sweeps;   /* red-black Gauss-Seidel sweeps before restriction (default 2) */
-  int mg_post_sweeps;  /* ... after prolongation (default 2; equal counts keep M symmetric) */
+  int mg_pre_sweeps;   /* red-black Gauss-Seidel sweeps before restriction, 1..3 (default 3) */
+  int mg_post_sweeps;  /* ... after prolongation, 1..3 (default 3; equal counts keep M symmetric) */
   double rtol;         /* stop when ||r||_2 <= rtol ||b||_2 (0 disables); PROJECTION_2D's `tolerance` */
   double div_tol;      /* stop when the predicted max|D u_proj| <= div_tol (0 disables) */
   int max_iterations;  /* cap on inner Krylov iterations per solve (PROJECTION_2D's `max_iteration`) */
   int max_outer;       /* cap on FP64 refinement passes */
-  double inner_rtol;   /* mixed: relative residual reduction of each FP32 inner solve */
+  double inner_rtol;   /* mixed: relative residual reduction of each FP32 inner solve (default 1e-4) */
 } p2d_config;
 
 typedef struct p2d_stats {

@@ -49,14 +49,14 @@ def test_stencil_matches_oracle(P2D, O, W, prec):
         assert np.abs(y - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_vcycle_matches_oracle(P2D, O, W, prec):
+@pytest.mark.parametrize("prec,nu", [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3)])
+def test_vcycle_matches_oracle(P2D, O, W, prec, nu):
     w = W.make("drops64", 256)
-    print(f"case vcycle prec={prec}", flush=True)
+    print(f"case vcycle prec={prec} nu={nu}", flush=True)
     bx, by, jc = _coeffs(O, w)
-    ctx = P2D.Context(w.n, w.n, w.h)
+    ctx = P2D.Context(w.n, w.n, w.h, mg_pre_sweeps=nu, mg_post_sweeps=nu)
     ctx.set_coefficients(bx, by, w.phi, jc)
-    mg = O.MG(bx, by)
+    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(6).standard_normal((w.n, w.n))
     r -= r.mean()
