@@ -5,6 +5,10 @@ package is its Python binding, a host-side mirror of the reference's
 PROJECTION_2D interface, and the seeded workload generators.  Import it with
 importlib.import_module("2d_engine_amd") (the name starts with a digit).
 """
+try:  # torch's HIP runtime, when present, must be the first one loaded (see _native)
+    import torch  # noqa: F401
+except ImportError:
+    pass
 from . import workloads  # noqa: F401
 from ._native import (FP64, MIXED, PREC_F32, PREC_F64, Context, P2DError, lib,  # noqa: F401
                       status_string)

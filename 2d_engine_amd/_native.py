@@ -5,6 +5,11 @@ module raises at import, and when no GPU is visible every compute call raises
 P2DError (status P2D_NO_DEVICE).  Arrays may be numpy (host memory) or torch
 tensors on the GPU (device memory, used in place); shapes follow the C ABI:
 cells (ny, nx), x-faces (ny, nx+1), y-faces (ny+1, nx), row-major.
+
+PyTorch-ROCm ships its own HIP runtime (libamdhip64); the first one loaded in
+a process serves both it and libp2d.so.  In a process that uses torch tensors
+with this binding, import torch before this module (the package does so when
+torch is installed) so that both share torch's runtime, its streams included.
 """
 import ctypes
 import os

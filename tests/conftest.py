@@ -4,6 +4,14 @@ import sys
 
 import pytest
 
+# PyTorch-ROCm carries its own HIP runtime (libamdhip64); in one process the
+# first one loaded serves both it and libp2d.so, so torch goes first whenever a
+# test may use it next to the library.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (ROOT, os.path.join(ROOT, "oracle")):
     if p not in sys.path:
