@@ -30,11 +30,15 @@ KERNEL_DIR_F64 = 1
 KERNEL_PRE0_F32 = 2
 KERNEL_POST0_F32 = 3
 KERNEL_CORRECT_RESID = 4
+KERNEL_CORRECT_RESID_F64 = 5
+KERNEL_PRE0_F64 = 6
+KERNEL_POST0_F64 = 7
 
 EXPORTS = [
     "p2d_config_default", "p2d_create", "p2d_destroy", "p2d_set_stream", "p2d_set_fluid",
     "p2d_set_coefficients", "p2d_project", "p2d_get_pressure", "p2d_get_coefficients", "p2d_adjoint",
-    "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_status_string",
+    "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_kernel_timing",
+    "p2d_kernel_time", "p2d_status_string",
 ]
 
 
@@ -87,6 +91,8 @@ def _load():
         "p2d_vcycle": (i, [vp, vp, vp, i, i]),
         "p2d_mg_levels": (i, [vp]),
         "p2d_time_kernel": (i, [vp, i, i, ctypes.POINTER(d)]),
+        "p2d_kernel_timing": (i, [vp, i, i]),
+        "p2d_kernel_time": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(i)]),
         "p2d_status_string": (ctypes.c_char_p, [i]),
     }
     for name, (res, args) in sig.items():
@@ -266,6 +272,16 @@ class Context:
     @property
     def mg_levels(self):
         return lib.p2d_mg_levels(self._h)
+
+    def kernel_timing(self, which, enable=True):
+        """Time every launch of kernel `which` inside the solves (HIP events on the context stream)."""
+        _check(lib.p2d_kernel_timing(self._h, which, 1 if enable else 0), "p2d_kernel_timing")
+
+    def kernel_time(self, which):
+        """(total ms, launches) of the launches timed since kernel_timing(which)."""
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _check(lib.p2d_kernel_time(self._h, which, ctypes.byref(ms), ctypes.byref(n)), "p2d_kernel_time")
+        return ms.value, n.value
 
     def time_kernel(self, which, reps=50):
         ms = ctypes.c_double()

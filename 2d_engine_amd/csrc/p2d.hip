@@ -56,6 +56,11 @@ inline dim3 fgrid(int nx, int ny) {
   return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + FTY - 1) / FTY));
 }
 constexpr int MAX_PASSES = 32;
+// Kernels p2d_time_kernel / p2d_kernel_timing name (finest level).
+enum {
+  KT_DIR_F32 = 0, KT_DIR_F64 = 1, KT_PRE0_F32 = 2, KT_POST0_F32 = 3, KT_RESID_F32 = 4,
+  KT_RESID_F64 = 5, KT_PRE0_F64 = 6, KT_POST0_F64 = 7, KT_COUNT = 8
+};
 inline size_t count(dim3 g) { return (size_t)g.x * g.y * g.z; }
 inline double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
   return std::chrono::duration<double, std::milli>(b - a).count();
@@ -91,6 +96,15 @@ struct p2d_ctx {
   hipStream_t own = nullptr, stream = nullptr;
   std::vector<void*> allocs;
   size_t lds_limit = 65536;  // dynamic LDS available to the single-workgroup V-cycle
+  // bench hook: HIP-event timing of every launch of selected finest-level kernels
+  struct KTimer {
+    std::vector<hipEvent_t> ev;  // start/end pairs recorded since the last harvest
+    size_t used = 0;
+    double total_ms = 0;
+    long launches = 0;
+  };
+  unsigned timing_mask = 0;
+  KTimer ktimer[KT_COUNT];
 
   // fluid state and coefficients (FP64 masters, cell-pitched)
   double *phi = nullptr, *jc = nullptr, *bw = nullptr, *bs = nullptr;
@@ -171,9 +185,44 @@ int num_levels(int nx, int ny, int max_levels) {
   return n;
 }
 
+// Kernel timing: an event pair around each launch of a selected kernel on the
+// context stream; durations are harvested at the host synchronisations.
+void kt_begin(p2d_ctx* c, int which) {
+  if (!((c->timing_mask >> which) & 1)) return;
+  auto& t = c->ktimer[which];
+  while (t.ev.size() < t.used + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) {
+      c->timing_mask &= ~(1u << which);
+      return;
+    }
+    t.ev.push_back(e);
+  }
+  (void)hipEventRecord(t.ev[t.used], c->stream);
+}
+void kt_end(p2d_ctx* c, int which) {
+  if (!((c->timing_mask >> which) & 1)) return;
+  auto& t = c->ktimer[which];
+  (void)hipEventRecord(t.ev[t.used + 1], c->stream);
+  t.used += 2;
+}
+void kt_harvest(p2d_ctx* c) {
+  for (auto& t : c->ktimer) {
+    for (size_t k = 0; k + 1 < t.used; k += 2) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]) == hipSuccess) {
+        t.total_ms += ms;
+        ++t.launches;
+      }
+    }
+    t.used = 0;
+  }
+}
+
 int sync_scal(p2d_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->hs, c->s, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  kt_harvest(c);
   return P2D_OK;
 }
 
@@ -284,8 +333,11 @@ void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const int Hpre = 2 * nu1 + 1, Hpost = 2 * nu2;
   const Level<T>* lv = mg.lv.data();
   const dim3 gpre = tgrid(lv[0].nx, lv[0].ny, Hpre), gpost = tgrid(lv[0].nx, lv[0].ny, Hpost);
+  const int kpre = sizeof(T) == 4 ? KT_PRE0_F32 : KT_PRE0_F64, kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
+  kt_begin(c, kpre);
   k_mg_pre<T, true><<<gpre, TBLOCK, 0, c->stream>>>(lv[0], lv[1], nullptr, lv[0].xa, nu1, v.r[0], v.r[1], v.x,
                                                     v.p[0], v.p[1], v.q, c->partial, c->s);
+  kt_end(c, kpre);
   k_fin_pre0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpre), (double)c->N, c->s);
   for (int l = 1; l < mg.l0; ++l)
     k_mg_pre<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpre), TBLOCK, 0, c->stream>>>(
@@ -295,8 +347,10 @@ void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   for (int l = mg.l0 - 1; l >= 1; --l)
     k_mg_post<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpost), TBLOCK, 0, c->stream>>>(
         lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa, lv[l].xb, nu2, nullptr, c->s);
+  kt_begin(c, kpost);
   k_mg_post<T, true><<<gpost, TBLOCK, 0, c->stream>>>(lv[0], lv[1], nullptr, v.r[0], v.r[1], lv[0].xa, v.z, nu2,
                                                       c->partial, c->s);
+  kt_end(c, kpost);
   k_fin_post0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpost), (double)c->N, c->s);
 }
 
@@ -305,7 +359,10 @@ void cg_iteration(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   vcycle<T>(c, mg, v);
   const Level<T>& l0 = mg.lv[0];
   const dim3 g = dgrid(c->nx, c->ny);
+  const int kd = sizeof(T) == 4 ? KT_DIR_F32 : KT_DIR_F64;
+  kt_begin(c, kd);
   k_dir<T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, v.p[0], v.p[1], v.q, c->partial, c->s);
+  kt_end(c, kd);
   k_fin_dir<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), c->s);
 }
 
@@ -314,8 +371,11 @@ template <class T>
 void correct_resid(p2d_ctx* c, const double* b, const double* saw, const double* sas, const double* ph,
                    const double* pl, const T* e, double* qh, double* ql, T* r) {
   const dim3 g = fgrid(c->nx, c->ny);
+  const int kr = sizeof(T) == 4 ? KT_RESID_F32 : KT_RESID_F64;
+  kt_begin(c, kr);
   k_correct_resid<T, T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, saw, sas, b, ph, pl, e, qh, ql, r,
                                                     c->partial, c->s);
+  kt_end(c, kr);
   k_fin_store<3><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), 2u, 0, c->s);
 }
 
@@ -653,6 +713,8 @@ int p2d_destroy(p2d_ctx* c) {
   if (!c) return P2D_BAD_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void* p : c->allocs) (void)hipFree(p);
+  for (auto& t : c->ktimer)
+    for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
   if (c->hs) (void)hipHostFree(c->hs);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
@@ -903,6 +965,27 @@ int p2d_vcycle(p2d_ctx* c, const void* r, void* z, int prec, int mem) {
 }
 
 int p2d_mg_levels(p2d_ctx* c) { return c ? c->mg64.L : -1; }
+
+int p2d_kernel_timing(p2d_ctx* c, int which, int enable) {
+  if (!c || which < 0 || which >= KT_COUNT) return P2D_BAD_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  kt_harvest(c);
+  auto& t = c->ktimer[which];
+  t.total_ms = 0;
+  t.launches = 0;
+  if (enable) c->timing_mask |= 1u << which;
+  else c->timing_mask &= ~(1u << which);
+  return P2D_OK;
+}
+
+int p2d_kernel_time(p2d_ctx* c, int which, double* total_ms, int* launches) {
+  if (!c || which < 0 || which >= KT_COUNT || !total_ms || !launches) return P2D_BAD_ARG;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  kt_harvest(c);
+  *total_ms = c->ktimer[which].total_ms;
+  *launches = (int)c->ktimer[which].launches;
+  return P2D_OK;
+}
 
 int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
   if (!c || !avg_ms || reps < 1 || which < 0 || which > 4 || !c->coeff_ready) return P2D_BAD_ARG;

@@ -3,18 +3,22 @@
 One step = one full projection (PROJECTION_2D::Solve) of the Rayleigh-Taylor
 workload (phi = y - 0.5 - 0.01 cos(8 pi x), rho 1000 above / 1 below, sigma 0,
 u,v ~ U[-1,1] seed 2, dt = 1) from device-resident phi, u, v to device-resident
-projected u, v and p: coefficient assembly, right-hand side, the solve to
-max|D u_proj| <= 1e-10 (and ||r||_2/||b||_2 <= 1e-10), the face update and the
-FP64 divergence check.  Each step starts from the same unprojected velocity
-(a device copy outside the timed interval) with a zero initial pressure.
+projected u, v and p: coefficient assembly, right-hand side, multigrid setup,
+the solve to max|D u_proj| <= 1e-10 (and ||r||_2/||b||_2 <= 1e-10), the face
+update and the FP64 divergence check.  Each step starts from the same
+unprojected velocity (a device copy outside the timed interval) with a zero
+initial pressure.
 
 value = projections/s of the mixed-precision path (FP32 MG-PCG inside FP64
-refinement); the FP64 path is timed the same way and reported beside it.
+refinement); the FP64 path is timed the same way and reported beside it, as
+are the adjoint's cost, per-kernel HBM rates at the finest level and, from the
+CPU leg, the error of the mixed result against the FP64 CPU oracle.
 """
 import argparse
 import importlib
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -24,6 +28,30 @@ sys.path.insert(0, ROOT)
 
 METRIC = "projections/s at 2048² ρ 1000:1, div≤1e-10: mixed vs FP64; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+
+# Finest-level kernels of one FP32 inner iteration (and the FP64 refinement
+# step), with their algorithmic HBM bytes per cell: every array the kernel
+# must read or write once (halo re-reads and cache effects excluded).
+# `which` is the p2d_time_kernel index: 50 back-to-back launches of the kernel
+# on the solver's buffers (the workload's coefficients, patterned vectors),
+# bracketed by one pair of HIP events on the context's stream, after the timed
+# region.  (An event pair around every launch inside
+# the solves perturbs them: the events' cache maintenance slows the timed kernel
+# by ~40% and the projection by ~7%.)
+KERNELS = [
+    (2, "k_mg_pre<float,true>", 45,
+     "Krylov update r -= a q, x += a p, 3 red-black sweeps from 0, residual, restriction: "
+     "reads r q beta_x beta_y wx wy x p (32 B), writes r x x_pre (12 B) and coarse r (1 B)"),
+    (3, "k_mg_post<float,true>", 29,
+     "prolongation, 3 black-red sweeps, r.z: reads r x_pre beta_x beta_y wx wy (24 B) and coarse x "
+     "(1 B), writes z (4 B)"),
+    (0, "k_dir<float>", 24,
+     "p = z - mean z + beta p_old, q = A p, p.q: reads z p_old beta_x beta_y (16 B), writes p q (8 B)"),
+    (4, "k_correct_resid<float,float>", 64,
+     "FP64 refinement: reads b p_hi p_lo (24 B) x (4 B) beta_x beta_y (16 B), writes p_hi p_lo (16 B) r (4 B)"),
+]
+DOMINANT = "k_mg_pre<float,true>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_mg_pre_rt2048_mixed_pmc.md")
 
 
 def log(*a):
@@ -55,7 +83,38 @@ def time_path(P2D, torch, w, precision, steps, warmup, dev):
         stats = st
         log(f"  {'mixed' if precision else 'fp64 '} step {k}: {e0.elapsed_time(e1):.3f} ms iters={st['iterations']} "
             f"outer={st['outer']} max_div={st['max_div']:.2e} rel_res={st['rel_residual']:.2e}")
-    return ctx, times, stats
+    return ctx, times, stats, (u, v)
+
+
+def time_adjoint(ctx, torch, w, uv, steps, dev):
+    """The adjoint of the last projection (loss 0.5 ||u_proj - u_t||^2, u_t seed 3), timed like the forward."""
+    W = importlib.import_module("2d_engine_amd.workloads")
+    ut, vt = W.target_velocity(w.n)
+    gu = (uv[0] - torch.from_numpy(ut).to(dev)).contiguous()
+    gv = (uv[1] - torch.from_numpy(vt).to(dev)).contiguous()
+    stream = torch.cuda.current_stream(dev)
+    times, st = [], None
+    for k in range(steps + 1):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        *_, st = ctx.adjoint(gu, gv)
+        e1.record(stream)
+        e1.synchronize()
+        if k:
+            times.append(e0.elapsed_time(e1))
+    return statistics.median(times), st
+
+
+def read_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
 
 
 def cpu_baseline(w):
@@ -63,11 +122,12 @@ def cpu_baseline(w):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     t = time.perf_counter()
-    _, u, v, st = O.project(w.phi, w.rho_minus, w.rho_plus, w.sigma, w.u, w.v, w.h, solver=O.SOLVER_MG, tol=1e-10)
+    p, u, v, st = O.project(w.phi, w.rho_minus, w.rho_plus, w.sigma, w.u, w.v, w.h, solver=O.SOLVER_MG, tol=1e-12)
     dt = time.perf_counter() - t
-    return {"value": 1.0 / dt, "unit": "projections/s", "cores": 1, "kind": "port",
+    base = {"value": 1.0 / dt, "unit": "projections/s", "cores": 1, "kind": "port",
             "sample": f"one full projection of the same {w.n}^2 RT workload by the oracle "
-                      f"(serial FP64 CSR-free MG-PCG, {st['iterations']} iterations, rel 1e-10), {dt:.1f} s"}
+                      f"(serial FP64 MG-PCG to relative residual 1e-12, {st['iterations']} iterations), {dt:.1f} s"}
+    return base, (p, u, v)
 
 
 def main():
@@ -78,11 +138,14 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--pmc-calibration", action="store_true",
+                    help="end with three 256 MiB device copies (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design (y-strips), not built", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     import torch
     P2D = importlib.import_module("2d_engine_amd")
     W = P2D.workloads
@@ -91,25 +154,34 @@ def main():
     w = W.rt(args.n)
     N = w.n * w.n
 
-    ctx_m, t_m, st_m = time_path(P2D, torch, w, P2D.MIXED, args.steps, args.warmup, dev)
+    ctx_m, t_m, st_m, uv_m = time_path(P2D, torch, w, P2D.MIXED, args.steps, args.warmup, dev)
+    p_m = ctx_m.pressure(mean_zero=True)
     fp64 = None
     if not args.no_fp64:
-        ctx_d, t_d, st_d = time_path(P2D, torch, w, P2D.FP64, args.steps, args.warmup, dev)
+        ctx_d, t_d, st_d, _ = time_path(P2D, torch, w, P2D.FP64, args.steps, args.warmup, dev)
         fp64 = {"projections_per_s": 1000.0 / statistics.median(t_d), "ms_median": statistics.median(t_d),
                 "iterations": st_d["iterations"], "outer": st_d["outer"], "max_div": st_d["max_div"]}
     ms = statistics.median(t_m)
     value = 1000.0 / ms
+    adj_ms, st_a = time_adjoint(ctx_m, torch, w, uv_m, max(3, args.steps // 4), dev)
+    adjoint = {"ms_median": adj_ms, "iterations": st_a["iterations"],
+               "forward_plus_adjoint_over_forward": (ms + adj_ms) / ms}
 
-    # FP32 direction update + stencil (k_dir<float>) at the finest level.
-    # Algorithmic bytes per launch: read z, p_old, beta_x, beta_y; write p_new, q = 6 * 4 * N.
-    reps = 200
-    kms = ctx_m.time_kernel(0, reps)
-    kbytes = 6 * 4 * N
-    achieved = kbytes / (kms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_dir<float> (FP32 direction update + 5-point GFM stencil + p.q)",
-                "bytes_per_launch": kbytes, "avg_launch_ms": kms}
+    # per-kernel HBM rates at the finest level: algorithmic bytes per launch over the
+    # average launch duration (after the adjoint, which overwrites no forward state read here)
+    kernels = {}
+    for which, name, bpc, what in KERNELS:
+        kms = ctx_m.time_kernel(which, 50)
+        kbytes = bpc * N
+        kernels[name] = {"avg_launch_ms": round(kms, 5), "bytes_per_launch": kbytes,
+                         "achieved_GBs": round(kbytes / (kms * 1e-3) / 1e9, 1), "bytes_per_cell": bpc,
+                         "accounting": what}
+    dom = kernels[DOMINANT]
+    traffic = read_traffic()
+    roofline = {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": DOMINANT + " (the largest share of the solve's kernel time)",
+                "bytes_per_launch": dom["bytes_per_launch"], "avg_launch_ms": dom["avg_launch_ms"]}
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "projections/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
@@ -121,12 +193,28 @@ def main():
                   "rel_residual": st_m["rel_residual"]},
         "fp64": fp64,
         "speedup_vs_fp64": round(value / fp64["projections_per_s"], 3) if fp64 else None,
+        "adjoint": adjoint,
         "roofline": roofline,
+        "kernels": kernels,
     }
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, serial)")
-        line["cpu_baseline"] = cpu_baseline(w)
+        base, (p_o, u_o, v_o) = cpu_baseline(w)
+        line["cpu_baseline"] = base
+        um, vm = uv_m[0].cpu().numpy(), uv_m[1].cpu().numpy()
+        p_o = p_o - p_o.mean()
+        du = max(np.abs(um - u_o).max(), np.abs(vm - v_o).max())
+        line["mixed"]["error_vs_fp64_oracle"] = {
+            "pressure_rel_l2": float(np.linalg.norm(p_m - p_o) / np.linalg.norm(p_o)),
+            "velocity_max_abs": float(du),
+            "velocity_rel_max": float(du / max(np.abs(u_o).max(), np.abs(v_o).max()))}
     print(json.dumps(line), flush=True)
+    if args.pmc_calibration:
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

@@ -137,6 +137,15 @@ int p2d_mg_levels(p2d_ctx* ctx);
  *        4 = true residual in double-double with the FP32 correction (k_correct_resid). */
 int p2d_time_kernel(p2d_ctx* ctx, int which, int reps, double* avg_ms);
 
+/* Bench hook: from now on (enable = 1) time every launch of kernel `which` inside the
+ * solves with a pair of HIP events on the context's stream, or stop (0); both reset the
+ * totals.  which: p2d_time_kernel's 0..4, and 5 = FP64 true residual, 6 = FP64 fused
+ * pre-smoothing (k_mg_pre<double,true>), 7 = FP64 post-smoothing (k_mg_post<double,true>). */
+int p2d_kernel_timing(p2d_ctx* ctx, int which, int enable);
+
+/* Totals of the timed launches of kernel `which` since the last reset. */
+int p2d_kernel_time(p2d_ctx* ctx, int which, double* total_ms, int* launches);
+
 const char* p2d_status_string(int status);
 
 #ifdef __cplusplus
