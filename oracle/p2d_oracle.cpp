@@ -17,6 +17,22 @@
 //
 // Every function cites the reference lines it follows.  Where this file
 // deviates from the reference on purpose, the comment says "DEVIATION".
+//
+// How each part is pinned (tests/test_oracle_*.py; the reference itself does
+// not build here: Windows.h and Boost):
+//   face beta, jump terms, assembly  closed forms, matrix invariants, the
+//                                    Liu-Fedkiw-Kang known answer (test 3) under
+//                                    refinement, balanced-force drop
+//   curvature                        circle and RT closed forms at O(h^2), clamp
+//   CG / IC(0)-PCG / Jacobi-PCG /    agreement on the mean-zero pressure (the
+//   MG-PCG                           system fixes it), absolute stopping rule
+//   multigrid V-cycle                symmetry, positivity, linearity, spectrum
+//   consistent update, adjoint       D u_proj = -dt r identity, central differences,
+//                                    idempotence and self-adjointness
+// Parity unpinned (only a build of the reference could pin them): IC(0)-PCG
+// iterates and iteration counts (orc_pcg_ic0), the reference-literal smeared
+// density update (orc_update_reference_literal) and the curvature of the four
+// corner cells (orc_curvature).
 
 #include <algorithm>
 #include <cmath>

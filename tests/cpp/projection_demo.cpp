@@ -1,0 +1,67 @@
+// C++ host driver of the projection drop-in (p2d::PROJECTION_2D over the C ABI),
+// used by tests/test_gpu_cpp_host.py.
+//
+//   projection_demo <dir> <n> <precision 0|1> <rho_minus> <rho_plus> <sigma>
+//
+// reads <dir>/phi.bin (n*n), u.bin ((n+1)*n), v.bin (n*(n+1)) as raw FP64,
+// runs Solve(dt = 1) and writes <dir>/p_out.bin, u_out.bin, v_out.bin and one
+// line of stats to stdout.
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "p2d_projection.hpp"
+
+static std::vector<double> read_all(const std::string& path, size_t n) {
+  std::vector<double> a(n);
+  FILE* f = std::fopen(path.c_str(), "rb");
+  if (!f || std::fread(a.data(), sizeof(double), n, f) != n) {
+    std::fprintf(stderr, "cannot read %s\n", path.c_str());
+    std::exit(2);
+  }
+  std::fclose(f);
+  return a;
+}
+
+static void write_all(const std::string& path, const std::vector<double>& a) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f || std::fwrite(a.data(), sizeof(double), a.size(), f) != a.size()) {
+    std::fprintf(stderr, "cannot write %s\n", path.c_str());
+    std::exit(2);
+  }
+  std::fclose(f);
+}
+
+int main(int argc, char** argv) {
+  if (argc != 7) {
+    std::fprintf(stderr, "usage: %s dir n precision rho_minus rho_plus sigma\n", argv[0]);
+    return 2;
+  }
+  const std::string dir = argv[1];
+  const int n = std::atoi(argv[2]);
+  const size_t N = (size_t)n * n, NU = (size_t)(n + 1) * n;
+  std::vector<double> phi = read_all(dir + "/phi.bin", N);
+  std::vector<double> u = read_all(dir + "/u.bin", NU);
+  std::vector<double> v = read_all(dir + "/v.bin", NU);
+  try {
+    p2d::PROJECTION_2D projection(n, n, 1.0 / n);
+    projection.precision = std::atoi(argv[3]);
+    projection.air_bubble_rising = true;  // minus side (phi <= 0) holds air_density
+    projection.air_density = std::atof(argv[4]);
+    projection.water_density = std::atof(argv[5]);
+    projection.surface_tension = std::atof(argv[6]);
+    projection.tolerance = 1e-12;
+    projection.Solve(1.0, phi.data(), u.data(), v.data());
+    write_all(dir + "/p_out.bin", projection.pressure_field);
+    write_all(dir + "/u_out.bin", u);
+    write_all(dir + "/v_out.bin", v);
+    const p2d_stats& s = projection.last_stats;
+    std::printf("status %d iterations %d outer %d max_div %.3e max_u %.6f max_v %.6f\n", s.status, s.iterations,
+                s.outer, s.max_div, projection.max_velocity_x, projection.max_velocity_y);
+    return projection.converged() ? 0 : 1;
+  } catch (const p2d::Error& e) {
+    std::fprintf(stderr, "p2d error: %s\n", e.what());
+    return 3;
+  }
+}

@@ -55,3 +55,14 @@ def test_status_codes_match_header():
     N = import_module("2d_engine_amd._native")
     for k in ("P2D_OK", "P2D_NOT_CONVERGED", "P2D_NAN", "P2D_BAD_ARG", "P2D_HIP_ERROR", "P2D_NO_DEVICE"):
         assert int(codes[k]) == getattr(N, k)
+
+
+def test_cpp_header_compiles(tmp_path):
+    """include/p2d_projection.hpp (the C++ host side) and p2d.h compile as C++17 without HIP headers."""
+    import subprocess
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "p2d_projection.hpp"\nint main() { p2d::PROJECTION_2D p(64, 64, 1.0 / 64); return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

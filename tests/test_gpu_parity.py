@@ -94,3 +94,19 @@ def test_projection_matches_oracle(P2D, O, W, name, n, prec):
     assert du <= 1e-8 and dv <= 1e-8
     # walls untouched
     assert np.all(u[:, 0] == 0) and np.all(u[:, -1] == 0) and np.all(v[0] == 0) and np.all(v[-1] == 0)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_projection_matches_golden(P2D, W, prec):
+    """The config-0 case against the committed oracle fixture (no oracle run on the GPU box)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "circle128.npz"))
+    w = W.circle(128)
+    print(f"case golden circle 128 prec={prec}", flush=True)
+    ctx = P2D.Context(128, 128, w.h, precision=prec, rtol=1e-12, div_tol=1e-10)
+    ctx.set_fluid(w.phi, w.rho_minus, w.rho_plus, w.sigma)
+    u, v, st = ctx.project(w.u, w.v, w.dt)
+    p = ctx.pressure(mean_zero=True)
+    assert np.linalg.norm(p - g["p"]) <= 1e-9 * np.linalg.norm(g["p"])
+    scale = max(np.abs(g["u"]).max(), np.abs(g["v"]).max())
+    assert max(np.abs(u - g["u"]).max(), np.abs(v - g["v"]).max()) <= 1e-8 * scale
