@@ -42,9 +42,9 @@ inline unsigned gs_blocks(size_t n) { return (unsigned)std::min<size_t>(blocks_f
 inline dim3 bgrid(int ncols, int nrows) {
   return dim3(blocks_for((size_t)ncols), (unsigned)((nrows + BAND - 1) / BAND));
 }
-// Pre/post tiles: 64 - 2H columns x TY rows of output per block.
+// Pre/post tiles: tile_tx(H) columns x TY rows of output per block.
 inline dim3 tgrid(int nx, int ny, int H) {
-  const int TX = 64 - 2 * H;
+  const int TX = tile_tx(H);
   return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + TY - 1) / TY));
 }
 // Direction kernel: 64 x DTY cells per block.
@@ -684,8 +684,7 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
   {
     // partial sums: the largest block count of any reducing launch
     size_t mb = std::max<size_t>(count(bgrid(c->nx + 1, c->ny + 1)), GRIDSTRIDE_BLOCKS);
-    mb = std::max(mb, count(tgrid(c->nx, c->ny, 2 * MAX_NU + 1)));
-    mb = std::max(mb, count(tgrid(c->nx, c->ny, 2 * 1)));
+    for (int H = 2; H <= 2 * MAX_NU + 1; ++H) mb = std::max(mb, count(tgrid(c->nx, c->ny, H)));
     mb = std::max(mb, count(dgrid(c->nx, c->ny)));
     mb = std::max(mb, count(fgrid(c->nx, c->ny)));
     A(c->partial, (size_t)RED_K * mb);

@@ -10,9 +10,10 @@
 //   post(l0-1..1)      prolongation + nu black-red sweeps
 //   post(level 0)      the same, writes z and reduces r.z, sum z, sum r
 //   dir                p = (z - mean z) + beta p_old, q = A0' p, p.q -> alpha
-// The pre/post kernels are temporally blocked: a 64-column region (one
-// wavefront wide) of TY output rows plus a halo of H cells is staged once, the
-// sweeps run on LDS, and only the tile interior is written back.  Every global
+// The pre/post kernels are temporally blocked: a 128-column region (one
+// wavefront wide, two columns per lane) of TY output rows plus a halo of H
+// cells is staged once, the sweeps run on LDS, and only the tile interior is
+// written back.  Every global
 // load of a tile is independent of the others, so a tile costs about one
 // memory round trip; coarse levels (small grids) are latency-bound.  Halo widths:
 // pre H = 2 nu + 1 (first red sweep from zero needs no neighbours; each later
@@ -23,13 +24,22 @@
 
 namespace p2d {
 
-constexpr int TBLOCK = 512;                // pre/post tile kernels: 8 wavefronts
+constexpr int TBLOCK = 1024;               // pre/post tile kernels: 16 wavefronts
 constexpr int TWAVES = TBLOCK / 64;
 constexpr int TY = 32;                     // output rows per pre/post tile
 constexpr int MAX_NU = 3;                  // sweeps per side supported
 constexpr int RYMAX = TY + 2 * (2 * MAX_NU + 1);  // region rows at most
 constexpr int MAXR = (RYMAX + TWAVES - 1) / TWAVES;  // region rows per thread at most
-constexpr int NCXMAX = 32;                 // coarse columns of a tile at most (TX / 2 < 32)
+constexpr int NCXMAX = 64;                 // coarse columns of a tile at most (TX / 2 < 64)
+
+// Tiles are 128 columns wide; the x halo HX is the smallest even width >= H so
+// that the region starts at an even column; TX = 128 - 2 HX interior columns.
+__host__ __device__ constexpr int tile_hx(int H) { return H + (H & 1); }
+__host__ __device__ constexpr int tile_tx(int H) { return 128 - 2 * tile_hx(H); }
+// FP32 tiles keep two blocks resident per CU (<= 64 VGPRs) so that one block's
+// loads overlap the other's LDS sweeps; FP64 tiles need the registers.
+template <class T>
+__host__ __device__ constexpr int tile_min_blocks() { return sizeof(T) == 4 ? 2 : 1; }
 constexpr int DTY = 16;                    // rows per block of the direction kernel
 constexpr int SMALL_MAX_CELLS = 1024;      // top level of the single-workgroup V-cycle at most (32^2)
 constexpr int MAXL = 24;                   // multigrid levels at most
@@ -299,21 +309,79 @@ __global__ void k_inner_continue(Scal* s, double ncells) {
   s->done = (met || s->iter >= s->max_iter) ? 1 : 0;
 }
 
-// Shared staging of a pre/post tile: thread (c, w) owns column c of the
-// 64-wide region and rows w, w + TWAVES, ...; it keeps r, the four face
-// coefficients and 1/diag of its cells in registers.
+// --------------------------------------------------------------------------- pair-layout tiles
+// A tile region is 128 columns by RY = TY + 2H rows.  Lane c of every
+// wavefront holds the column pair (2c, 2c+1): both children of one coarse
+// column, since the region starts at an even column.  Wave w holds rows w,
+// w + TWAVES, ...  The two cells of a pair have opposite colours, so a
+// red-black half-sweep updates exactly one cell per pair and row, picked per
+// row (the same for the whole wavefront): no lane idles on the colour test.
 template <class T>
-struct TileCells {
-  T rr[MAXR], bW[MAXR], bE[MAXR], bS[MAXR], bN[MAXR], inv[MAXR];
-  bool val[MAXR];
+struct alignas(2 * sizeof(T)) Pair {
+  T a, b;
 };
 
-// One red-black half-sweep over the region held by this thread.  Cells of one
-// colour read only cells of the other colour, so all neighbour reads of the
-// half-sweep are gathered first (one LDS round trip for all rows), then the
-// updated values are written.
+// Per row m of this thread: the pair's right-hand side, face coefficients
+// (wA west of A, wB west of B = east of A, eB east of B, s/n south/north),
+// 1/diag and validity.
 template <class T>
-__device__ __forceinline__ void tile_sweep(TileCells<T>& tc, T (*xs)[66], int c, int w, int RY, int gi, int gy0,
+struct PairRows {
+  T rA[MAXR], rB[MAXR], wA[MAXR], wB[MAXR], eB[MAXR], sA[MAXR], sB[MAXR], nA[MAXR], nB[MAXR], iA[MAXR], iB[MAXR];
+  bool vA[MAXR], vB[MAXR];
+};
+
+// Stages row m (region row r, global row gj) of the pair at global columns
+// (giA, giA + 1), k = ci(giA, gj, nx) (even: nx is even on every tile level).
+template <class T>
+__device__ __forceinline__ void stage_coeffs(PairRows<T>& pr, int m, const Level<T>& F, int giA, int gj, bool row_in,
+                                             size_t k) {
+  const int nx = F.nx, ny = F.ny;
+  const bool vA = row_in && giA >= 0 && giA < nx, vB = row_in && giA + 1 >= 0 && giA + 1 < nx;
+  pr.vA[m] = vA;
+  pr.vB[m] = vB;
+  T wA = 0, wB = 0, eB = 0, sA = 0, sB = 0, nA = 0, nB = 0;
+  if (vA) {
+    const Pair<T> w2 = reinterpret_cast<const Pair<T>*>(F.bw)[k >> 1];
+    const Pair<T> s2 = reinterpret_cast<const Pair<T>*>(F.bs)[k >> 1];
+    wA = w2.a;
+    wB = w2.b;
+    sA = s2.a;
+    sB = s2.b;
+    if (giA + 2 < nx) eB = F.bw[k + 2];
+    if (gj + 1 < ny) {
+      const Pair<T> n2 = reinterpret_cast<const Pair<T>*>(F.bs)[(k + nx) >> 1];
+      nA = n2.a;
+      nB = n2.b;
+    }
+  }
+  pr.wA[m] = wA;
+  pr.wB[m] = wB;
+  pr.eB[m] = eB;
+  pr.sA[m] = sA;
+  pr.sB[m] = sB;
+  pr.nA[m] = nA;
+  pr.nB[m] = nB;
+  pr.iA[m] = vA ? (T)1 / (wA + wB + sA + nA) : (T)0;
+  pr.iB[m] = vB ? (T)1 / (wB + eB + sB + nB) : (T)0;
+}
+
+template <class T>
+__device__ __forceinline__ void zero_ring(Pair<T> (*xs)[66], int RY) {
+  for (int t = threadIdx.x; t < 66; t += TBLOCK) {
+    xs[0][t] = Pair<T>{0, 0};
+    xs[RY + 1][t] = Pair<T>{0, 0};
+  }
+  for (int t = threadIdx.x; t < RY; t += TBLOCK) {
+    xs[t + 1][0] = Pair<T>{0, 0};
+    xs[t + 1][65] = Pair<T>{0, 0};
+  }
+}
+
+// One red-black half-sweep: cells of one colour read only cells of the other,
+// so all neighbour reads are gathered first, then the new values written.
+// Cell (i,j) has colour (i + j) & 1; A (even column) has colour (gy0 + r) & 1.
+template <class T>
+__device__ __forceinline__ void pair_sweep(PairRows<T>& pr, Pair<T> (*xs)[66], int c, int w, int RY, int gy0,
                                            int color, bool zero) {
   T off[MAXR];
 #pragma unroll
@@ -321,44 +389,49 @@ __device__ __forceinline__ void tile_sweep(TileCells<T>& tc, T (*xs)[66], int c,
     const int r = w + TWAVES * m;
     off[m] = 0;
     if (zero || r >= RY) continue;
-    off[m] = tc.bW[m] * xs[r + 1][c] + tc.bE[m] * xs[r + 1][c + 2] + tc.bS[m] * xs[r][c + 1] +
-             tc.bN[m] * xs[r + 2][c + 1];
+    const Pair<T> own = xs[r + 1][c + 1], up = xs[r + 2][c + 1], dn = xs[r][c + 1];
+    if (((gy0 + r) & 1) == color)
+      off[m] = pr.wA[m] * xs[r + 1][c].b + pr.wB[m] * own.b + pr.sA[m] * dn.a + pr.nA[m] * up.a;
+    else
+      off[m] = pr.wB[m] * own.a + pr.eB[m] * xs[r + 1][c + 2].a + pr.sB[m] * dn.b + pr.nB[m] * up.b;
   }
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
     const int r = w + TWAVES * m;
-    if (r >= RY || !tc.val[m] || ((gi + gy0 + r) & 1) != color) continue;
-    xs[r + 1][c + 1] = (tc.rr[m] + off[m]) * tc.inv[m];
+    if (r >= RY) continue;
+    if (((gy0 + r) & 1) == color) {
+      if (pr.vA[m]) xs[r + 1][c + 1].a = (pr.rA[m] + off[m]) * pr.iA[m];
+    } else {
+      if (pr.vB[m]) xs[r + 1][c + 1].b = (pr.rB[m] + off[m]) * pr.iB[m];
+    }
   }
 }
 
 // --------------------------------------------------------------------------- pre-smoothing tile kernel
-// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, TX = 64 - 2H, H = 2 nu + 1.
+// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, H = 2 nu + 1, x halo
+// HX = tile_hx(H) (even), TX = 128 - 2 HX.
 // FUSE (finest level): r_in = rbuf[s->rpar], the pending update r -= a q,
 // x += a p and the residual shift are applied first (tile interiors write r to
-// rbuf[1 - rpar] and x), ||r|| is reduced and the convergence decision taken.
-// Restriction r_H = P^T res: every fine cell splits its residual into the four
-// weight products; the x-direction sums run across lanes (a region row is one
-// wavefront), the y-direction sums through LDS.  No global load depends on
-// another, so a tile costs about one memory round trip.
+// rbuf[1 - rpar] and x), and the block partials of ||r|| are stored.
+// Restriction r_H = P^T res: each cell splits its residual into the four
+// weight products; the x sums run across lanes, the y sums through LDS.
 template <class T, bool FUSE>
-__global__ void __launch_bounds__(TBLOCK) k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu,
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_min_blocks<T>()))) k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu,
                                                    T* rA, T* rB, T* xsol, const T* pA, const T* pB,
-                                                   const T* __restrict__ qv, double* partial,
-                                                   Scal* s) {
+                                                   const T* __restrict__ qv, double* partial, Scal* s) {
   if (s->done) return;
-  __shared__ T xs[RYMAX + 2][66];
-  const int H = 2 * nu + 1, TX = 64 - 2 * H, RY = TY + 2 * H;
+  __shared__ Pair<T> xs[RYMAX + 2][66];
+  const int H = 2 * nu + 1, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
   xcd_tile(bx, by);
   const int tx0 = bx * TX, ty0 = by * TY;
-  const int gx0 = tx0 - H, gy0 = ty0 - H;
+  const int gx0 = tx0 - HX, gy0 = ty0 - H;
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int gi = gx0 + c;
+  const int giA = gx0 + 2 * c;
   const int nx = F.nx, ny = F.ny;
-  const bool col_in = gi >= 0 && gi < nx;
-  const bool col_int = c >= H && c < H + TX;      // tile interior column
-  const bool col_fp = c >= H - 1 && c <= H + TX;  // restriction footprint column
+  const int hc = HX / 2, NCX = TX / 2;
+  const bool pair_int = c >= hc && c < hc + NCX;       // both cells in the tile interior
+  const bool pair_fp = c >= hc - 1 && c <= hc + NCX;   // restriction footprint
   const T* rin = r_lvl;
   T* rout = nullptr;
   const T* pdir = nullptr;
@@ -374,91 +447,106 @@ __global__ void __launch_bounds__(TBLOCK) k_mg_pre(Level<T> F, Level<T> Cc, cons
     pdir = ((s->iter + 1) & 1) ? pB : pA;
   }
   double acc[3] = {0.0, 0.0, 0.0};
-  TileCells<T> tc;
-  T wxr[MAXR], wyr[MAXR];
+  PairRows<T> pr;
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
     const int r = w + TWAVES * m, gj = gy0 + r;
-    const bool v = r < RY && col_in && gj >= 0 && gj < ny;
-    tc.val[m] = v;
-    T rv = 0, bw = 0, be = 0, bsv = 0, bn = 0, wx = 0, wy = 0;
-    if (v) {
-      const size_t k = ci(gi, gj, nx);
-      rv = rin[k];
+    const bool row_in = r < RY && gj >= 0 && gj < ny;
+    const size_t k = row_in && giA >= 0 && giA < nx ? ci(giA, gj, nx) : 0;
+    stage_coeffs<T>(pr, m, F, giA, gj, row_in, k);
+    T ra = 0, rb = 0;
+    if (pr.vA[m]) {
+      const Pair<T> r2 = reinterpret_cast<const Pair<T>*>(rin)[k >> 1];
+      ra = r2.a;
+      rb = r2.b;
       if (FUSE) {
-        rv = (T)(((double)rv - (pend ? alpha * (double)qv[k] : 0.0)) - shift);
-        if (col_int && r >= H && r < H + TY) {
-          rout[k] = rv;
-          if (pend) xsol[k] = (T)((double)xsol[k] + alpha * (double)pdir[k]);
-          acc[0] += (double)rv * (double)rv;
-          acc[1] = fmax(acc[1], fabs((double)rv));
-          acc[2] += (double)rv;
+        double da = (double)ra, db = (double)rb;
+        if (pend) {
+          const Pair<T> q2 = reinterpret_cast<const Pair<T>*>(qv)[k >> 1];
+          da -= alpha * (double)q2.a;
+          db -= alpha * (double)q2.b;
+        }
+        ra = (T)(da - shift);
+        rb = (T)(db - shift);
+        if (pair_int && r >= H && r < H + TY) {
+          reinterpret_cast<Pair<T>*>(rout)[k >> 1] = Pair<T>{ra, rb};
+          if (pend) {
+            Pair<T> x2 = reinterpret_cast<Pair<T>*>(xsol)[k >> 1];
+            const Pair<T> p2 = reinterpret_cast<const Pair<T>*>(pdir)[k >> 1];
+            x2.a = (T)((double)x2.a + alpha * (double)p2.a);
+            x2.b = (T)((double)x2.b + alpha * (double)p2.b);
+            reinterpret_cast<Pair<T>*>(xsol)[k >> 1] = x2;
+          }
+          acc[0] += (double)ra * (double)ra + (double)rb * (double)rb;
+          acc[1] = fmax(acc[1], fmax(fabs((double)ra), fabs((double)rb)));
+          acc[2] += (double)ra + (double)rb;
         }
       }
-      bw = F.bw[k];
-      be = gi + 1 < nx ? F.bw[k + 1] : (T)0;
-      bsv = F.bs[k];
-      bn = gj + 1 < ny ? F.bs[k + nx] : (T)0;
-      if (col_fp && r >= H - 1 && r <= H + TY) {
-        wx = F.wx[k];
-        wy = F.wy[k];
-      }
     }
-    tc.rr[m] = rv;
-    tc.bW[m] = bw;
-    tc.bE[m] = be;
-    tc.bS[m] = bsv;
-    tc.bN[m] = bn;
-    tc.inv[m] = v ? (T)1 / (bw + be + bsv + bn) : (T)0;
-    wxr[m] = wx;
-    wyr[m] = wy;
-    if (r < RY) xs[r + 1][c + 1] = 0;
+    pr.rA[m] = ra;
+    pr.rB[m] = rb;
+    if (r < RY) xs[r + 1][c + 1] = Pair<T>{0, 0};
   }
-  for (int t = threadIdx.x; t < 66; t += TBLOCK) {
-    xs[0][t] = 0;
-    xs[RY + 1][t] = 0;
-  }
-  for (int t = threadIdx.x; t < RY; t += TBLOCK) {
-    xs[t + 1][0] = 0;
-    xs[t + 1][65] = 0;
-  }
+  zero_ring<T>(xs, RY);
   __syncthreads();
   for (int sw = 0; sw < nu; ++sw)
     for (int color = 0; color < 2; ++color) {
-      tile_sweep<T>(tc, xs, c, w, RY, gi, gy0, color, sw == 0 && color == 0);
+      pair_sweep<T>(pr, xs, c, w, RY, gy0, color, sw == 0 && color == 0);
       __syncthreads();
     }
+  // interpolation weights of the restriction footprint (loaded only now: not live in the sweeps)
+  T wxA[MAXR], wxB[MAXR], wyA[MAXR], wyB[MAXR];
+#pragma unroll
+  for (int m = 0; m < MAXR; ++m) {
+    const int r = w + TWAVES * m, gj = gy0 + r;
+    wxA[m] = wxB[m] = wyA[m] = wyB[m] = 0;
+    if (pr.vA[m] && pair_fp && r >= H - 1 && r <= H + TY) {
+      const size_t k = ci(giA, gj, nx);
+      const Pair<T> wx2 = reinterpret_cast<const Pair<T>*>(F.wx)[k >> 1];
+      const Pair<T> wy2 = reinterpret_cast<const Pair<T>*>(F.wy)[k >> 1];
+      wxA[m] = wx2.a;
+      wxB[m] = wx2.b;
+      wyA[m] = wy2.a;
+      wyB[m] = wy2.b;
+    }
+  }
   // residual (reference order: r - (diag x - offsum)) and the interior write of x
-  T res[MAXR];
+  T resA[MAXR], resB[MAXR];
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
     const int r = w + TWAVES * m;
-    res[m] = 0;
-    if (r < RY && tc.val[m]) {
-      const T x = xs[r + 1][c + 1];
-      const T off =
-          tc.bW[m] * xs[r + 1][c] + tc.bE[m] * xs[r + 1][c + 2] + tc.bS[m] * xs[r][c + 1] + tc.bN[m] * xs[r + 2][c + 1];
-      res[m] = tc.rr[m] - ((tc.bW[m] + tc.bE[m] + tc.bS[m] + tc.bN[m]) * x - off);
-      if (col_int && r >= H && r < H + TY) x_lvl[ci(gi, gy0 + r, nx)] = x;
-    }
+    resA[m] = 0;
+    resB[m] = 0;
+    if (r >= RY) continue;
+    const Pair<T> own = xs[r + 1][c + 1], up = xs[r + 2][c + 1], dn = xs[r][c + 1];
+    const T xw = xs[r + 1][c].b, xe = xs[r + 1][c + 2].a;
+    if (pr.vA[m])
+      resA[m] = pr.rA[m] - ((pr.wA[m] + pr.wB[m] + pr.sA[m] + pr.nA[m]) * own.a -
+                            (pr.wA[m] * xw + pr.wB[m] * own.b + pr.sA[m] * dn.a + pr.nA[m] * up.a));
+    if (pr.vB[m])
+      resB[m] = pr.rB[m] - ((pr.wB[m] + pr.eB[m] + pr.sB[m] + pr.nB[m]) * own.b -
+                            (pr.wB[m] * own.a + pr.eB[m] * xe + pr.sB[m] * dn.b + pr.nB[m] * up.b));
+    if (pr.vA[m] && pair_int && r >= H && r < H + TY)
+      reinterpret_cast<Pair<T>*>(x_lvl)[ci(giA, gy0 + r, nx) >> 1] = own;
   }
   __syncthreads();  // xs is reused below for the row sums us[2][RYMAX][NCXMAX]
-  T* us = &xs[0][0];
-  const int NCX = TX / 2, NCY = TY / 2, cx0 = tx0 / 2, cy0 = ty0 / 2;
-  const bool owner = col_int && ((c - H) & 1) == 0;  // fine column 2I of coarse column I
-  const int ic = (c - H) >> 1;
+  T* us = &xs[0][0].a;
+  const int NCY = TY / 2, cx0 = tx0 / 2, cy0 = ty0 / 2;
+  const int ic = c - hc;
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
     const int r = w + TWAVES * m;
     if (r >= RY) continue;  // uniform across the wavefront
-    // A: part of res for the parent row (weight wy), B: for the neighbouring row (1 - wy);
+    // a*: part of res for the parent row (weight wy), b*: for the neighbouring row (1 - wy);
     // p*: part for the parent column (wx), q*: for the neighbouring column (1 - wx).
-    const T A = res[m] * wyr[m], B = res[m] * ((T)1 - wyr[m]);
-    const T pa = wxr[m] * A, qa = ((T)1 - wxr[m]) * A, pb = wxr[m] * B, qb = ((T)1 - wxr[m]) * B;
-    // coarse column I gathers q(2I-1) + p(2I) + p(2I+1) + q(2I+2) at the lane of fine column 2I
-    const T sa = (pa + __shfl_down(pa, 1, 64)) + (__shfl_up(qa, 1, 64) + __shfl_down(qa, 2, 64));
-    const T sb = (pb + __shfl_down(pb, 1, 64)) + (__shfl_up(qb, 1, 64) + __shfl_down(qb, 2, 64));
-    if (owner && r >= H - 1 && r <= H + TY) {
+    const T aA = resA[m] * wyA[m], bA = resA[m] * ((T)1 - wyA[m]);
+    const T aB = resB[m] * wyB[m], bB = resB[m] * ((T)1 - wyB[m]);
+    const T paA = wxA[m] * aA, qaA = ((T)1 - wxA[m]) * aA, pbA = wxA[m] * bA, qbA = ((T)1 - wxA[m]) * bA;
+    const T paB = wxB[m] * aB, qaB = ((T)1 - wxB[m]) * aB, pbB = wxB[m] * bB, qbB = ((T)1 - wxB[m]) * bB;
+    // coarse column I (this lane) gathers q(2I-1) + p(2I) + p(2I+1) + q(2I+2)
+    const T sa = (paA + paB) + (__shfl_up(qaB, 1, 64) + __shfl_down(qaA, 1, 64));
+    const T sb = (pbA + pbB) + (__shfl_up(qbB, 1, 64) + __shfl_down(qbA, 1, 64));
+    if (pair_int && r >= H - 1 && r <= H + TY) {
       us[r * NCXMAX + ic] = sa;
       us[(RYMAX + r) * NCXMAX + ic] = sb;
     }
@@ -502,119 +590,86 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
 }
 
 // --------------------------------------------------------------------------- post-smoothing tile kernel
-// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, TX = 64 - 2H, H = 2 nu.
-// x = x_pre + P x_coarse (the coarse values around the tile are staged in LDS),
-// then nu (black, red) sweeps; the tile interior is written to x_out.
-// LEVEL0: r = rbuf[s->rpar], x_out = z, and r.z, sum z, sum r are reduced:
-// zmean, rz = r.(z - zmean), beta.
-constexpr int PCW = 35;                           // staged coarse columns
+// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, H = 2 nu (even): HX = H,
+// TX = 128 - 2H.  x = x_pre + P x_coarse (the coarse values around the tile are
+// staged in LDS), then nu (black, red) sweeps; the tile interior is written to
+// x_out.  LEVEL0: r = rbuf[s->rpar], x_out = z, and the block partials of r.z,
+// sum z, sum r are stored.
+constexpr int PCW = 66;                           // staged coarse columns
 constexpr int PCH = (TY + 4 * MAX_NU) / 2 + 3;    // staged coarse rows
 template <class T, bool LEVEL0>
-__global__ void __launch_bounds__(TBLOCK) k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA,
-                                                    const T* rB, const T* x_in, T* x_out, int nu, double* partial, Scal* s) {
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_min_blocks<T>()))) k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA,
+                                                    const T* rB, const T* x_in, T* x_out, int nu, double* partial,
+                                                    Scal* s) {
   if (s->done) return;
-  __shared__ T xs[RYMAX + 2][66];
+  __shared__ Pair<T> xs[RYMAX + 2][66];
   __shared__ T xcs[PCH][PCW];
-  const int H = 2 * nu, TX = 64 - 2 * H, RY = TY + 2 * H;
+  const int H = 2 * nu, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
   xcd_tile(bx, by);
   const int tx0 = bx * TX, ty0 = by * TY;
-  const int gx0 = tx0 - H, gy0 = ty0 - H;
+  const int gx0 = tx0 - HX, gy0 = ty0 - H;
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int gi = gx0 + c;
+  const int giA = gx0 + 2 * c;
   const int nx = F.nx, ny = F.ny, cnx = Cc.nx, cny = Cc.ny;
-  const int cxa = (gx0 >> 1) - 1, cya = (gy0 >> 1) - 1;  // first staged coarse column / row (floor)
+  const int hc = HX / 2, NCX = TX / 2;
+  const bool pair_int = c >= hc && c < hc + NCX;
+  const int cxa = gx0 / 2 - 1, cya = (gy0 >> 1) - 1;  // first staged coarse column / row (gx0 is even)
   const T* rin = LEVEL0 ? (s->rpar ? rB : rA) : r_lvl;
-  constexpr int NCS = (PCH * PCW + TBLOCK - 1) / TBLOCK;
-  T cv[NCS];
-#pragma unroll
-  for (int q = 0; q < NCS; ++q) {
-    const int t = (int)threadIdx.x + q * TBLOCK;
-    cv[q] = 0;
-    if (t < PCH * PCW) {
-      const int I = min(max(cxa + t % PCW, 0), cnx - 1), J = min(max(cya + t / PCW, 0), cny - 1);
-      cv[q] = Cc.xb[ci(I, J, cnx)];
-    }
+  // the coarse correction around the tile into LDS first, then each row is
+  // staged and prolongated at once (the row's x_pre and weights die there)
+  for (int t = threadIdx.x; t < PCH * PCW; t += TBLOCK) {
+    const int I = min(max(cxa + t % PCW, 0), cnx - 1), J = min(max(cya + t / PCW, 0), cny - 1);
+    xcs[t / PCW][t % PCW] = Cc.xb[ci(I, J, cnx)];
   }
-  TileCells<T> tc;
-  T x0r[MAXR], wxr[MAXR], wyr[MAXR];
-#pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m, gj = gy0 + r;
-    const bool v = r < RY && gi >= 0 && gi < nx && gj >= 0 && gj < ny;
-    tc.val[m] = v;
-    T rv = 0, bw = 0, be = 0, bsv = 0, bn = 0, x0 = 0, wx = 0, wy = 0;
-    if (v) {
-      const size_t k = ci(gi, gj, nx);
-      rv = rin[k];
-      bw = F.bw[k];
-      be = gi + 1 < nx ? F.bw[k + 1] : (T)0;
-      bsv = F.bs[k];
-      bn = gj + 1 < ny ? F.bs[k + nx] : (T)0;
-      wx = F.wx[k];
-      wy = F.wy[k];
-      x0 = x_in[k];
-    }
-    tc.rr[m] = rv;
-    tc.bW[m] = bw;
-    tc.bE[m] = be;
-    tc.bS[m] = bsv;
-    tc.bN[m] = bn;
-    tc.inv[m] = v ? (T)1 / (bw + be + bsv + bn) : (T)0;
-    x0r[m] = x0;
-    wxr[m] = wx;
-    wyr[m] = wy;
-  }
-#pragma unroll
-  for (int q = 0; q < NCS; ++q) {
-    const int t = (int)threadIdx.x + q * TBLOCK;
-    if (t < PCH * PCW) xcs[t / PCW][t % PCW] = cv[q];
-  }
-  for (int t = threadIdx.x; t < 66; t += TBLOCK) {
-    xs[0][t] = 0;
-    xs[RY + 1][t] = 0;
-  }
-  for (int t = threadIdx.x; t < RY; t += TBLOCK) {
-    xs[t + 1][0] = 0;
-    xs[t + 1][65] = 0;
-  }
+  zero_ring<T>(xs, RY);
   __syncthreads();
-  // prolongation of the coarse correction
+  PairRows<T> pr;
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
     const int r = w + TWAVES * m, gj = gy0 + r;
-    if (r >= RY) continue;
-    T x0 = 0;
-    if (tc.val[m]) {
-      const int I = gi >> 1, J = gj >> 1;
-      int I2 = I + ((gi & 1) ? 1 : -1), J2 = J + ((gj & 1) ? 1 : -1);
-      I2 = I2 < 0 ? 0 : (I2 > cnx - 1 ? cnx - 1 : I2);
+    const bool row_in = r < RY && gj >= 0 && gj < ny;
+    const size_t k = row_in && giA >= 0 && giA < nx ? ci(giA, gj, nx) : 0;
+    stage_coeffs<T>(pr, m, F, giA, gj, row_in, k);
+    Pair<T> rr{0, 0}, x0{0, 0};
+    if (pr.vA[m]) {
+      rr = reinterpret_cast<const Pair<T>*>(rin)[k >> 1];
+      x0 = reinterpret_cast<const Pair<T>*>(x_in)[k >> 1];
+      const Pair<T> wx2 = reinterpret_cast<const Pair<T>*>(F.wx)[k >> 1];
+      const Pair<T> wy2 = reinterpret_cast<const Pair<T>*>(F.wy)[k >> 1];
+      // prolongation: A (column 2I) takes coarse I and I-1, B takes I and I+1
+      const int I = giA >> 1, J = gj >> 1;
+      int J2 = J + ((gj & 1) ? 1 : -1);
       J2 = J2 < 0 ? 0 : (J2 > cny - 1 ? cny - 1 : J2);
-      const int a = I - cxa, a2 = I2 - cxa, b = J - cya, b2 = J2 - cya;
-      const T wx = wxr[m], wy = wyr[m];
-      const T e = wx * wy * xcs[b][a] + ((T)1 - wx) * wy * xcs[b][a2] + wx * ((T)1 - wy) * xcs[b2][a] +
-                  ((T)1 - wx) * ((T)1 - wy) * xcs[b2][a2];
-      x0 = x0r[m] + e;
+      const int IA2 = I - 1 < 0 ? 0 : I - 1, IB2 = I + 1 > cnx - 1 ? cnx - 1 : I + 1;
+      const int a = I - cxa, aA = IA2 - cxa, aB = IB2 - cxa, b = J - cya, b2 = J2 - cya;
+      const T wxa = wx2.a, wya = wy2.a, wxb = wx2.b, wyb = wy2.b;
+      x0.a += wxa * wya * xcs[b][a] + ((T)1 - wxa) * wya * xcs[b][aA] + wxa * ((T)1 - wya) * xcs[b2][a] +
+              ((T)1 - wxa) * ((T)1 - wya) * xcs[b2][aA];
+      x0.b += wxb * wyb * xcs[b][a] + ((T)1 - wxb) * wyb * xcs[b][aB] + wxb * ((T)1 - wyb) * xcs[b2][a] +
+              ((T)1 - wxb) * ((T)1 - wyb) * xcs[b2][aB];
     }
-    xs[r + 1][c + 1] = x0;
+    pr.rA[m] = rr.a;
+    pr.rB[m] = rr.b;
+    if (r < RY) xs[r + 1][c + 1] = x0;
   }
   __syncthreads();
   for (int sw = 0; sw < nu; ++sw)
     for (int cc = 0; cc < 2; ++cc) {
-      tile_sweep<T>(tc, xs, c, w, RY, gi, gy0, 1 - cc, false);
+      pair_sweep<T>(pr, xs, c, w, RY, gy0, 1 - cc, false);
       __syncthreads();
     }
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
     const int r = w + TWAVES * m;
-    if (r >= RY || !tc.val[m] || c < H || c >= H + TX || r < H || r >= H + TY) continue;
-    const T z = xs[r + 1][c + 1];
-    x_out[ci(gi, gy0 + r, nx)] = z;
+    if (r >= RY || !pr.vA[m] || !pair_int || r < H || r >= H + TY) continue;
+    const Pair<T> z = xs[r + 1][c + 1];
+    reinterpret_cast<Pair<T>*>(x_out)[ci(giA, gy0 + r, nx) >> 1] = z;
     if (LEVEL0) {
-      acc[0] += (double)tc.rr[m] * (double)z;
-      acc[1] += (double)z;
-      acc[2] += (double)tc.rr[m];
+      acc[0] += (double)pr.rA[m] * (double)z.a + (double)pr.rB[m] * (double)z.b;
+      acc[1] += (double)z.a + (double)z.b;
+      acc[2] += (double)pr.rA[m] + (double)pr.rB[m];
     }
   }
   if (LEVEL0) block_partial<3, TBLOCK>(acc, 0u, partial);
