@@ -248,6 +248,10 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
       break;
     }
   mg.small_lds = small_bytes(mg.l0);
+  if (mg.small_lds > 65536 &&
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_small<T>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)mg.small_lds) != hipSuccess)
+    return P2D_HIP_ERROR;
   for (int l = 0; l < L; ++l) {
     Level<T>& d = mg.lv[l];
     d.nx = c->nx >> l;
@@ -642,7 +646,9 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
-      c->lds_limit = std::min<size_t>((size_t)lds, 65536) - 1024;  // static LDS of the kernel
+      // The single-workgroup cycle gets at most 64 KB: each level it absorbs runs
+      // slower in one workgroup than as its own pair of tile launches (measured).
+      c->lds_limit = std::min<size_t>((size_t)lds, 65536) - 1024;  // less the kernel's static LDS
   }
   c->stream = c->own;
   const size_t N = c->N;

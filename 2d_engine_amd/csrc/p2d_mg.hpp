@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
     shift = s->rshift;
     pdir = ((s->iter + 1) & 1) ? pB : pA;
   }
-  double acc[3] = {0.0, 0.0, 0.0};
+  const T ta = (T)alpha, tshift = (T)shift;
   PairRows<T> pr;
 #pragma unroll
   for (int m = 0; m < MAXR; ++m) {
@@ -460,26 +460,23 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
       ra = r2.a;
       rb = r2.b;
       if (FUSE) {
-        double da = (double)ra, db = (double)rb;
+        // the inner recursion runs in T: r -= a q, x += a p (a, shift rounded to T)
         if (pend) {
           const Pair<T> q2 = reinterpret_cast<const Pair<T>*>(qv)[k >> 1];
-          da -= alpha * (double)q2.a;
-          db -= alpha * (double)q2.b;
+          ra -= ta * q2.a;
+          rb -= ta * q2.b;
         }
-        ra = (T)(da - shift);
-        rb = (T)(db - shift);
+        ra -= tshift;
+        rb -= tshift;
         if (pair_int && r >= H && r < H + TY) {
           reinterpret_cast<Pair<T>*>(rout)[k >> 1] = Pair<T>{ra, rb};
           if (pend) {
             Pair<T> x2 = reinterpret_cast<Pair<T>*>(xsol)[k >> 1];
             const Pair<T> p2 = reinterpret_cast<const Pair<T>*>(pdir)[k >> 1];
-            x2.a = (T)((double)x2.a + alpha * (double)p2.a);
-            x2.b = (T)((double)x2.b + alpha * (double)p2.b);
+            x2.a += ta * p2.a;
+            x2.b += ta * p2.b;
             reinterpret_cast<Pair<T>*>(xsol)[k >> 1] = x2;
           }
-          acc[0] += (double)ra * (double)ra + (double)rb * (double)rb;
-          acc[1] = fmax(acc[1], fmax(fabs((double)ra), fabs((double)rb)));
-          acc[2] += (double)ra + (double)rb;
         }
       }
     }
@@ -560,7 +557,20 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
     Cc.r[ci(I, J, Cc.nx)] = (us[(RYMAX + jr - 1) * NCXMAX + a] + us[jr * NCXMAX + a]) +
                             (us[(jr + 1) * NCXMAX + a] + us[(RYMAX + jr + 2) * NCXMAX + a]);
   }
-  if (FUSE) block_partial<3, TBLOCK>(acc, 2u, partial);
+  if (FUSE) {
+    // norms of the updated residual over the tile interior (pr.r holds it)
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+      const int r = w + TWAVES * m;
+      if (!pr.vA[m] || !pair_int || r < H || r >= H + TY) continue;
+      const double ra = (double)pr.rA[m], rb = (double)pr.rB[m];
+      acc[0] += ra * ra + rb * rb;
+      acc[1] = fmax(acc[1], fmax(fabs(ra), fabs(rb)));
+      acc[2] += ra + rb;
+    }
+    block_partial<3, TBLOCK>(acc, 2u, partial);
+  }
 }
 
 // Finish of the fused finest pre kernel: residual buffers swap, the shift is
