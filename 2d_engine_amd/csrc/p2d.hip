@@ -74,7 +74,6 @@ struct MG {
   int L = 0, l0 = 0;
   bool valid = false;
   std::vector<Level<T>> lv;
-  Level<T>* d_lv = nullptr;
   T* binv = nullptr;
   size_t small_lds = 0;
 };
@@ -233,20 +232,20 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
   const int L = num_levels(c->nx, c->ny, c->cfg.mg_max_levels);
   mg.L = L;
   mg.lv.resize(L);
+  for (int l = 0; l < L; ++l) {
+    mg.lv[l].nx = c->nx >> l;
+    mg.lv[l].ny = c->ny >> l;
+  }
   // l0: the first level of at most SMALL_MAX_CELLS cells whose single-workgroup
-  // V-cycle fits the LDS limit (per level x, r, bw, bs, wx, wy; 4 arrays on the
-  // bottom; one residual array of level l0)
-  auto small_bytes = [&](int a) {
-    size_t e = (size_t)(c->nx >> a) * (c->ny >> a);
-    for (int l = a; l < L; ++l) e += (size_t)(c->nx >> l) * (c->ny >> l) * (l + 1 < L ? 6 : 4);
-    return e * sizeof(T);
-  };
-  mg.l0 = L - 1;
+  // V-cycle (at most SMALL_LV levels down to the bottom) fits the LDS limit
+  auto small_bytes = [&](int a) { return (size_t)small_layout<T>(mg.lv.data() + a, L - a).total * sizeof(T); };
+  mg.l0 = -1;
   for (int l = 1; l < L; ++l)
-    if ((long)(c->nx >> l) * (c->ny >> l) <= SMALL_MAX_CELLS && small_bytes(l) <= c->lds_limit) {
+    if ((long)(c->nx >> l) * (c->ny >> l) <= SMALL_MAX_CELLS && L - l <= SMALL_LV && small_bytes(l) <= c->lds_limit) {
       mg.l0 = l;
       break;
     }
+  if (mg.l0 < 0) return P2D_BAD_ARG;
   mg.small_lds = small_bytes(mg.l0);
   if (mg.small_lds > 65536 &&
       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_small<T>), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -268,7 +267,7 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
       d.bs = b;
     }
     T *wx = nullptr, *wy = nullptr, *xa, *xb, *r;
-    if (l < mg.l0) {
+    if (l + 1 < L) {
       CHK(dalloc(c, &wx, n));
       CHK(dalloc(c, &wy, n));
     }
@@ -281,8 +280,6 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
     d.xb = xb;
     d.r = r;
   }
-  CHK(dalloc(c, &mg.d_lv, (size_t)L));
-  HIPCHK(hipMemcpy(mg.d_lv, mg.lv.data(), sizeof(Level<T>) * L, hipMemcpyHostToDevice));
   return P2D_OK;
 }
 
@@ -296,8 +293,8 @@ int mg_setup(p2d_ctx* c, MG<T>& mg, bool cast32) {
   const bool common = c->mg_common;
   for (int l = 0; l < m.L; ++l) {
     const Level<double>& f = m.lv[l];
-    T* wx = l < mg.l0 ? const_cast<T*>(mg.lv[l].wx) : nullptr;
-    T* wy = l < mg.l0 ? const_cast<T*>(mg.lv[l].wy) : nullptr;
+    T* wx = const_cast<T*>(mg.lv[l].wx);  // null on the bottom level
+    T* wy = const_cast<T*>(mg.lv[l].wy);
     float* f32bw = cast32 ? const_cast<float*>(c->mg32.lv[l].bw) : nullptr;
     float* f32bs = cast32 ? const_cast<float*>(c->mg32.lv[l].bs) : nullptr;
     double* cbw = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bw) : nullptr;
@@ -347,7 +344,11 @@ void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
     k_mg_pre<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpre), TBLOCK, 0, c->stream>>>(
         lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
         c->s);
-  k_mg_small<T><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(mg.d_lv, mg.l0, mg.L, nu1, nu2, mg.binv, c->s);
+  SmallArgs<T> sa{};
+  sa.nl = mg.L - mg.l0;
+  for (int q = 0; q < sa.nl; ++q) sa.lv[q] = lv[mg.l0 + q];
+  sa.binv = mg.binv;
+  k_mg_small<T><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s);
   for (int l = mg.l0 - 1; l >= 1; --l)
     k_mg_post<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpost), TBLOCK, 0, c->stream>>>(
         lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa, lv[l].xb, nu2, nullptr, c->s);
@@ -646,9 +647,7 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
-      // The single-workgroup cycle gets at most 64 KB: each level it absorbs runs
-      // slower in one workgroup than as its own pair of tile launches (measured).
-      c->lds_limit = std::min<size_t>((size_t)lds, 65536) - 1024;  // less the kernel's static LDS
+      c->lds_limit = (size_t)lds - 1024;  // less the kernel's static LDS
   }
   c->stream = c->own;
   const size_t N = c->N;

@@ -50,24 +50,12 @@ struct Level {
   int nx, ny;
   const T* bw;  // west-face coefficient per cell
   const T* bs;  // south-face coefficient per cell
-  const T* wx;  // parent interpolation weight per cell (x, y); unused on the bottom level
+  const T* wx;  // parent interpolation weight per cell (x, y); none on the bottom level
   const T* wy;
   T* xa;  // pre-smoothed correction
   T* xb;  // post-smoothed correction (the coarse correction read by the finer level)
   T* r;   // right-hand side (restricted residual)
 };
-
-// 1-D weight of coarse index I for fine index i with parent weight w (n = coarse size).
-template <class T>
-__device__ __forceinline__ T w1(int i, int I, int n, T w) {
-  const int P = i >> 1;
-  int Q = P + ((i & 1) ? 1 : -1);
-  Q = Q < 0 ? 0 : (Q > n - 1 ? n - 1 : Q);
-  T s = 0;
-  if (P == I) s += w;
-  if (Q == I) s += (T)1 - w;
-  return s;
-}
 
 // Operator-dependent parent weight of fine index i along a row/column whose face
 // coefficients are face(k), k = 0..nf (walls 0): the resistance ratio
@@ -701,158 +689,205 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_post0(const double* __re
 }
 
 // --------------------------------------------------------------------------- single-workgroup coarse V-cycle
-// Levels l0..L-1 (l0 >= 1) in one workgroup.  LDS per level: x, r, bw, bs and,
-// above the bottom, the interpolation weights wx, wy (computed here from the
-// coefficients); one array of n(l0) for residuals.  The bottom applies the
-// dense pseudo-inverse binv (global); the result goes to Ls[l0].xb.
+// Levels l0..L-1 (at most SMALL_LV, each of at most SMALL_THREADS cells) in
+// one workgroup.  Thread t owns cell t of every level and keeps that cell's
+// coefficients, 1/diag and right-hand side in registers; LDS holds x and r of
+// every level, the interpolation weights above the bottom, the residual and
+// the two restriction row-sum arrays, and the bottom pseudo-inverse when the
+// bottom has at most BOTTOM_LDS_MAX cells.  Every global load is issued in the
+// first phase.  The bottom applies the dense pseudo-inverse; the cycle's
+// result goes to level l0's xb.
+constexpr int SMALL_LV = 4;
+
 template <class T>
-__global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(const Level<T>* __restrict__ Ls, int l0, int L, int nu_pre,
-                                                            int nu_post, const T* __restrict__ binv, const Scal* s) {
+struct SmallArgs {
+  Level<T> lv[SMALL_LV];  // levels l0.. in order; the last used one is the bottom
+  int nl;                 // levels in the cycle
+  const T* binv;          // bottom pseudo-inverse, nb x nb, row-major
+};
+
+// LDS layout of k_mg_small in elements of T (offsets), shared by host and device.
+struct SmallLayout {
+  int x[SMALL_LV], r[SMALL_LV], wx[SMALL_LV], wy[SMALL_LV];
+  int xb, rb, tmp, tp, tq, binv, total;
+};
+template <class T>
+__host__ __device__ inline SmallLayout small_layout(const Level<T>* lv, int nl) {
+  SmallLayout o{};
+  int off = 0;
+#pragma unroll
+  for (int q = 0; q < SMALL_LV; ++q) {
+    if (q >= nl) break;
+    const int n = lv[q].nx * lv[q].ny;
+    o.x[q] = off;
+    off += n;
+    o.r[q] = off;
+    off += n;
+    if (q + 1 < nl) {
+      o.wx[q] = off;
+      off += n;
+      o.wy[q] = off;
+      off += n;
+    } else {
+      o.xb = o.x[q];
+      o.rb = o.r[q];
+    }
+  }
+  const int n0 = lv[0].nx * lv[0].ny, nh = lv[0].ny * (lv[0].nx / 2);
+  o.tmp = off;
+  off += n0;
+  o.tp = off;
+  off += nh;
+  o.tq = off;
+  off += nh;
+  const int nb = lv[nl - 1].nx * lv[nl - 1].ny;
+  o.binv = off;
+  if (nb <= BOTTOM_LDS_MAX) off += nb * nb;
+  o.total = off;
+  return o;
+}
+
+// One red-black half-sweep of cell t (coordinates i, j) of a small level.
+template <class T>
+__device__ __forceinline__ void small_half(T* X, int t, int n, int nx, int ny, int i, int j, T cw, T ce, T cs, T cn,
+                                           T id, T rr, int color, bool zero) {
+  if (t < n && ((i + j) & 1) == color) {
+    T o = 0;
+    if (!zero)
+      o = cw * X[i > 0 ? t - 1 : t] + ce * X[i + 1 < nx ? t + 1 : t] + cs * X[j > 0 ? t - nx : t] +
+          cn * X[j + 1 < ny ? t + nx : t];
+    X[t] = (rr + o) * id;
+  }
+  __syncthreads();
+}
+
+template <class T>
+__global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int nu_pre, int nu_post, const Scal* s) {
   if (s->done) return;
   extern __shared__ unsigned char smem_raw[];
-  T* smem = reinterpret_cast<T*>(smem_raw);
-  __shared__ int dnx[MAXL], dny[MAXL], doff[MAXL];
-  __shared__ int tmp_off;
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    int off = 0;
-    for (int l = l0; l < L; ++l) {
-      dnx[l] = Ls[l].nx;
-      dny[l] = Ls[l].ny;
-      doff[l] = off;
-      off += dnx[l] * dny[l] * (l + 1 < L ? 6 : 4);
-    }
-    tmp_off = off;
-  }
-  __syncthreads();
-  // array a of level l: 0 = x, 1 = r, 2 = bw, 3 = bs, 4 = wx, 5 = wy
-  auto arr = [&](int l, int a) { return smem + doff[l] + a * dnx[l] * dny[l]; };
-  T* tmp = smem + tmp_off;
-  for (int l = l0; l < L; ++l) {
-    const int n = dnx[l] * dny[l];
-    const T* gbw = Ls[l].bw;
-    const T* gbs = Ls[l].bs;
-    const T* gr = Ls[l].r;
-    T* BW = arr(l, 2);
-    T* BS = arr(l, 3);
-    T* R = arr(l, 1);
-    for (int t = tid; t < n; t += SMALL_THREADS) {
-      BW[t] = gbw[t];
-      BS[t] = gbs[t];
-      if (l == l0) R[t] = gr[t];
-    }
-  }
-  __syncthreads();
-  for (int l = l0; l + 1 < L; ++l) {
-    const int nx = dnx[l], ny = dny[l], n = nx * ny;
-    const T* BW = arr(l, 2);
-    const T* BS = arr(l, 3);
-    T* WX = arr(l, 4);
-    T* WY = arr(l, 5);
-    for (int t = tid; t < n; t += SMALL_THREADS) {
-      const int i = t % nx, j = t / nx;
-      auto fx = [&](int k) { return k < nx ? BW[k + j * nx] : (T)0; };
-      auto fy = [&](int k) { return k < ny ? BS[i + k * nx] : (T)0; };
-      T wx = od_weight<T>(fx, i, nx), wy = od_weight<T>(fy, j, ny);
-      const int Q = (i >> 1) + ((i & 1) ? 1 : -1), Qj = (j >> 1) + ((j & 1) ? 1 : -1);
-      if (Q < 0 || Q > (nx >> 1) - 1) wx = (T)1;
-      if (Qj < 0 || Qj > (ny >> 1) - 1) wy = (T)1;
-      WX[t] = wx;
-      WY[t] = wy;
-    }
-  }
-  __syncthreads();
-  auto sweep = [&](int l, int color, bool zero) {
-    const int nx = dnx[l], ny = dny[l], half = (nx + 1) >> 1;
-    T* X = arr(l, 0);
-    const T* R = arr(l, 1);
-    const T* BW = arr(l, 2);
-    const T* BS = arr(l, 3);
-    for (int t = tid; t < half * ny; t += SMALL_THREADS) {
-      const int j = t / half, i = 2 * (t % half) + ((j + color) & 1);
-      if (i >= nx) continue;
-      const int k = i + j * nx;
-      const T e = i + 1 < nx ? BW[k + 1] : (T)0, nn = j + 1 < ny ? BS[k + nx] : (T)0;
-      const T d = BW[k] + e + BS[k] + nn;
-      T o = 0;
-      if (!zero) {
-        if (i > 0) o += BW[k] * X[k - 1];
-        if (i + 1 < nx) o += e * X[k + 1];
-        if (j > 0) o += BS[k] * X[k - nx];
-        if (j + 1 < ny) o += nn * X[k + nx];
+  T* sm = reinterpret_cast<T*>(smem_raw);
+  const int nl = A.nl, t = threadIdx.x;
+  const SmallLayout L = small_layout<T>(A.lv, nl);
+  int nb = 0;
+  int ci_[SMALL_LV], cj_[SMALL_LV];
+  T cw[SMALL_LV], ce[SMALL_LV], cs[SMALL_LV], cn[SMALL_LV], cd[SMALL_LV], rr[SMALL_LV];
+  // every global load of the cycle: coefficients (registers), r of level l0,
+  // weights and the bottom pseudo-inverse (LDS); x of every level cleared
+#pragma unroll
+  for (int q = 0; q < SMALL_LV; ++q) {
+    ci_[q] = cj_[q] = 0;
+    cw[q] = ce[q] = cs[q] = cn[q] = cd[q] = rr[q] = (T)0;
+    if (q >= nl) continue;
+    const Level<T> F = A.lv[q];
+    const int nx = F.nx, ny = F.ny;
+    if (q == nl - 1) nb = nx * ny;
+    if (t < nx * ny) {
+      const int j = t / nx, i = t - j * nx;
+      ci_[q] = i;
+      cj_[q] = j;
+      const T w = F.bw[t], e = i + 1 < nx ? F.bw[t + 1] : (T)0;
+      const T so = F.bs[t], no = j + 1 < ny ? F.bs[t + nx] : (T)0;
+      cd[q] = w + e + so + no;
+      cw[q] = i > 0 ? w : (T)0;
+      ce[q] = e;
+      cs[q] = j > 0 ? so : (T)0;
+      cn[q] = no;
+      if (q == 0) {
+        rr[0] = F.r[t];
+        sm[L.r[0] + t] = rr[0];
       }
-      X[k] = (R[k] + o) / d;
+      if (q + 1 < nl) {
+        sm[L.wx[q] + t] = F.wx[t];
+        sm[L.wy[q] + t] = F.wy[t];
+      }
+      sm[L.x[q] + t] = (T)0;
     }
-    __syncthreads();
-  };
-  for (int l = l0; l + 1 < L; ++l) {
+  }
+  if (nb <= BOTTOM_LDS_MAX)
+    for (int e = t; e < nb * nb; e += SMALL_THREADS) sm[L.binv + e] = A.binv[e];
+  __syncthreads();
+  // pre-smoothing, residual and restriction, level by level
+#pragma unroll
+  for (int q = 0; q + 1 < SMALL_LV; ++q) {
+    if (q + 1 >= nl) break;
+    const int nx = A.lv[q].nx, ny = A.lv[q].ny, n = nx * ny, cnx = nx / 2, cny = ny / 2;
+    T* X = sm + L.x[q];
+    const T id = cd[q] != (T)0 ? (T)1 / cd[q] : (T)0;
+    if (q > 0 && t < n) rr[q] = sm[L.r[q] + t];
     for (int sw = 0; sw < nu_pre; ++sw) {
-      sweep(l, 0, sw == 0);
-      sweep(l, 1, false);
+      small_half<T>(X, t, n, nx, ny, ci_[q], cj_[q], cw[q], ce[q], cs[q], cn[q], id, rr[q], 0, sw == 0);
+      small_half<T>(X, t, n, nx, ny, ci_[q], cj_[q], cw[q], ce[q], cs[q], cn[q], id, rr[q], 1, false);
     }
-    const int nx = dnx[l], ny = dny[l], n = nx * ny, cnx = dnx[l + 1], cny = dny[l + 1];
-    {
-      const T* X = arr(l, 0);
-      const T* R = arr(l, 1);
-      const T* BW = arr(l, 2);
-      const T* BS = arr(l, 3);
-      for (int k = tid; k < n; k += SMALL_THREADS) {
-        const int i = k % nx, j = k / nx;
-        const T e = i + 1 < nx ? BW[k + 1] : (T)0, nn = j + 1 < ny ? BS[k + nx] : (T)0;
-        T o = 0;
-        if (i > 0) o += BW[k] * X[k - 1];
-        if (i + 1 < nx) o += e * X[k + 1];
-        if (j > 0) o += BS[k] * X[k - nx];
-        if (j + 1 < ny) o += nn * X[k + nx];
-        tmp[k] = R[k] - ((BW[k] + e + BS[k] + nn) * X[k] - o);
-      }
+    T* R = sm + L.tmp;
+    if (t < n) {
+      const int i = ci_[q], j = cj_[q];
+      const T o = cw[q] * X[i > 0 ? t - 1 : t] + ce[q] * X[i + 1 < nx ? t + 1 : t] +
+                  cs[q] * X[j > 0 ? t - nx : t] + cn[q] * X[j + 1 < ny ? t + nx : t];
+      R[t] = rr[q] - (cd[q] * X[t] - o);
     }
     __syncthreads();
-    {
-      const T* WX = arr(l, 4);
-      const T* WY = arr(l, 5);
-      T* Rc = arr(l + 1, 1);
-      for (int t = tid; t < cnx * cny; t += SMALL_THREADS) {
-        const int I = t % cnx, J = t / cnx;
-        T sum = 0;
-        for (int j = 2 * J - 1; j <= 2 * J + 2; ++j) {
-          if (j < 0 || j >= ny) continue;
-          for (int i = 2 * I - 1; i <= 2 * I + 2; ++i) {
-            if (i < 0 || i >= nx) continue;
-            const int k = i + j * nx;
-            const T a = w1<T>(j, J, cny, WY[k]), b = w1<T>(i, I, cnx, WX[k]);
-            if (a == (T)0 || b == (T)0) continue;
-            sum += b * a * tmp[k];
-          }
+    // restriction r_H = P^T res in two passes: per fine row and coarse column
+    // the x-weighted sums of res * wy (tp) and res * (1 - wy) (tq), then per
+    // coarse cell the rows 2J-1 (tq), 2J, 2J+1 (tp), 2J+2 (tq)
+    const T* WX = sm + L.wx[q];
+    const T* WY = sm + L.wy[q];
+    if (t < ny * cnx) {
+      const int jj = t / cnx, I = t - jj * cnx;
+      T pa = 0, pb = 0, qa = 0, qb = 0;
+#pragma unroll
+      for (int d = -1; d <= 2; ++d) {
+        const int i = 2 * I + d;
+        if (i < 0 || i >= nx) continue;
+        const int k = i + jj * nx;
+        const T res = R[k], wy = WY[k], wx = WX[k];
+        const T a = res * wy, b = res * ((T)1 - wy);
+        const T fx = (d == 0 || d == 1) ? wx : (T)1 - wx;
+        if (d == 0 || d == 1) {
+          pa += fx * a;
+          pb += fx * b;
+        } else {
+          qa += fx * a;
+          qb += fx * b;
         }
-        Rc[t] = sum;
       }
+      sm[L.tp + t] = pa + qa;
+      sm[L.tq + t] = pb + qb;
+    }
+    __syncthreads();
+    if (t < cnx * cny) {
+      const int J = t / cnx, I = t - J * cnx;
+      const T* TP = sm + L.tp;
+      const T* TQ = sm + L.tq;
+      const T lo = 2 * J - 1 >= 0 ? TQ[(2 * J - 1) * cnx + I] : (T)0;
+      const T hi = 2 * J + 2 < ny ? TQ[(2 * J + 2) * cnx + I] : (T)0;
+      sm[L.r[q + 1] + t] = (lo + TP[2 * J * cnx + I]) + (TP[(2 * J + 1) * cnx + I] + hi);
     }
     __syncthreads();
   }
   {
-    // bottom: x = binv r, one wavefront per row
-    const int l = L - 1, n = dnx[l] * dny[l];
-    const T* R = arr(l, 1);
-    T* X = arr(l, 0);
-    const int lane = tid & 63;
-    for (int a = tid >> 6; a < n; a += SMALL_THREADS / 64) {
+    // bottom: x = binv r, one wavefront per row, FP64 accumulation
+    const T* Rb = sm + L.rb;
+    T* Xb = sm + L.xb;
+    const T* Bi = nb <= BOTTOM_LDS_MAX ? sm + L.binv : A.binv;
+    const int lane = t & 63;
+    for (int a = t >> 6; a < nb; a += SMALL_THREADS / 64) {
       double acc = 0;
-      for (int q = lane; q < n; q += 64) acc += (double)binv[(size_t)a * n + q] * (double)R[q];
+      for (int qq = lane; qq < nb; qq += 64) acc += (double)Bi[(size_t)a * nb + qq] * (double)Rb[qq];
       acc = wave_sum(acc);
-      if (lane == 0) X[a] = (T)acc;
+      if (lane == 0) Xb[a] = (T)acc;
     }
     __syncthreads();
   }
-  for (int l = L - 2; l >= l0; --l) {
-    const int nx = dnx[l], ny = dny[l], cnx = dnx[l + 1], cny = dny[l + 1];
-    T* X = arr(l, 0);
-    const T* WX = arr(l, 4);
-    const T* WY = arr(l, 5);
-    const T* xc = arr(l + 1, 0);
-    for (int t = tid; t < nx * ny; t += SMALL_THREADS) {
-      const int i = t % nx, j = t / nx;
-      const T wx = WX[t], wy = WY[t];
+  // prolongation and post-smoothing, coarse to fine
+#pragma unroll
+  for (int q = SMALL_LV - 2; q >= 0; --q) {
+    if (q + 1 >= nl) continue;
+    const int nx = A.lv[q].nx, ny = A.lv[q].ny, n = nx * ny, cnx = nx / 2, cny = ny / 2;
+    T* X = sm + L.x[q];
+    const T* xc = sm + L.x[q + 1];
+    if (t < n) {
+      const int i = ci_[q], j = cj_[q];
+      const T wx = sm[L.wx[q] + t], wy = sm[L.wy[q] + t];
       const int I = i >> 1, J = j >> 1;
       int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
       I2 = I2 < 0 ? 0 : (I2 > cnx - 1 ? cnx - 1 : I2);
@@ -861,15 +896,14 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(const Level<T>* __re
               ((T)1 - wx) * ((T)1 - wy) * xc[I2 + J2 * cnx];
     }
     __syncthreads();
+    const T id = cd[q] != (T)0 ? (T)1 / cd[q] : (T)0;
     for (int sw = 0; sw < nu_post; ++sw) {
-      sweep(l, 1, false);
-      sweep(l, 0, false);
+      small_half<T>(X, t, n, nx, ny, ci_[q], cj_[q], cw[q], ce[q], cs[q], cn[q], id, rr[q], 1, false);
+      small_half<T>(X, t, n, nx, ny, ci_[q], cj_[q], cw[q], ce[q], cs[q], cn[q], id, rr[q], 0, false);
     }
   }
-  const int n0 = dnx[l0] * dny[l0];
-  const T* X0 = arr(l0, 0);
-  T* out = Ls[l0].xb;
-  for (int t = tid; t < n0; t += SMALL_THREADS) out[t] = X0[t];
+  const int n0 = A.lv[0].nx * A.lv[0].ny;
+  if (t < n0) A.lv[0].xb[t] = sm[L.x[0] + t];
 }
 
 // --------------------------------------------------------------------------- Krylov direction kernel

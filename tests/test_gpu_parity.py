@@ -69,6 +69,32 @@ def test_vcycle_matches_oracle(P2D, O, W, prec, nu):
         assert np.linalg.norm(z - ref) <= 1e-4 * np.linalg.norm(ref)
 
 
+# Rectangular grids: hierarchies whose single-workgroup cycle has three levels
+# (192x128, 160x96), two levels over a 135-cell bottom whose pseudo-inverse
+# stays in global memory (120x72), and only the 576-cell bottom (128x18).
+@pytest.mark.parametrize("nx,ny,prec", [(192, 128, 0), (160, 96, 0), (160, 96, 1), (120, 72, 0), (120, 72, 1),
+                                        (128, 18, 0)])
+def test_vcycle_rectangular_matches_oracle(P2D, O, nx, ny, prec):
+    print(f"case vcycle rectangular {nx}x{ny} prec={prec}", flush=True)
+    h = 1.0 / nx
+    X, Y = np.meshgrid((np.arange(nx) + 0.5) * h, (np.arange(ny) + 0.5) * h)
+    phi = np.hypot(X - 0.4, Y - 0.45 * ny / nx) - 0.3 * ny / nx
+    bx, by = O.face_beta(phi, 1.0, 1000.0)
+    ctx = P2D.Context(nx, ny, h, mg_pre_sweeps=2, mg_post_sweeps=2)
+    ctx.set_coefficients(bx, by, phi, np.zeros_like(phi))
+    mg = O.MG(bx, by, nu_pre=2, nu_post=2)
+    assert mg.levels == ctx.mg_levels
+    r = np.random.default_rng(7).standard_normal((ny, nx))
+    r -= r.mean()
+    ref = mg.vcycle(r)
+    if prec == 0:
+        z = ctx.vcycle(r, prec=0)
+        assert np.linalg.norm(z - ref) <= 1e-11 * np.linalg.norm(ref)
+    else:
+        z = ctx.vcycle(r.astype(np.float32), prec=1)
+        assert np.linalg.norm(z - ref) <= 1e-4 * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("name,n,prec", [("circle", 128, 0), ("circle", 128, 1), ("rt", 256, 0), ("rt", 256, 1),
                                          ("drops64", 256, 0), ("drops64", 256, 1)])
 def test_projection_matches_oracle(P2D, O, W, name, n, prec):
