@@ -12,10 +12,11 @@
 //   dir                p = (z - mean z) + beta p_old, q = A0' p, p.q -> alpha
 // The pre/post kernels are temporally blocked: a 128-column region (one
 // wavefront wide, two columns per lane) of TY output rows plus a halo of H
-// cells is staged once, the sweeps run on LDS, and only the tile interior is
-// written back.  Every global
-// load of a tile is independent of the others, so a tile costs about one
-// memory round trip; coarse levels (small grids) are latency-bound.  Halo widths:
+// cells is staged once into registers, the sweeps run there (strips of
+// consecutive rows per thread, lane shifts across columns, LDS only between
+// strips), and only the tile interior is written back.  The global loads of
+// a tile are independent of each other, so a tile costs about one memory
+// round trip; coarse levels (small grids) are latency-bound.  Halo widths:
 // pre H = 2 nu + 1 (first red sweep from zero needs no neighbours; each later
 // half-sweep and the residual lose one exact ring; restriction reads one ring
 // outside the tile), post H = 2 nu.
@@ -29,7 +30,6 @@ constexpr int TWAVES = TBLOCK / 64;
 constexpr int TY = 32;                     // output rows per pre/post tile
 constexpr int MAX_NU = 3;                  // sweeps per side supported
 constexpr int RYMAX = TY + 2 * (2 * MAX_NU + 1);  // region rows at most
-constexpr int MAXR = (RYMAX + TWAVES - 1) / TWAVES;  // region rows per thread at most
 constexpr int NCXMAX = 64;                 // coarse columns of a tile at most (TX / 2 < 64)
 
 // Tiles are 128 columns wide; the x halo HX is the smallest even width >= H so
@@ -297,102 +297,145 @@ __global__ void k_inner_continue(Scal* s, double ncells) {
   s->done = (met || s->iter >= s->max_iter) ? 1 : 0;
 }
 
-// --------------------------------------------------------------------------- pair-layout tiles
+// --------------------------------------------------------------------------- strip-layout tiles
 // A tile region is 128 columns by RY = TY + 2H rows.  Lane c of every
 // wavefront holds the column pair (2c, 2c+1): both children of one coarse
-// column, since the region starts at an even column.  Wave w holds rows w,
-// w + TWAVES, ...  The two cells of a pair have opposite colours, so a
-// red-black half-sweep updates exactly one cell per pair and row, picked per
-// row (the same for the whole wavefront): no lane idles on the colour test.
+// column, since the region starts at an even column.  Wave w holds the KR
+// consecutive region rows KR w .. KR w + KR - 1 (a strip).  The iterate stays
+// in registers: horizontal neighbours come from the adjacent lanes (DPP
+// wavefront shifts), vertical ones from the thread's own rows, and only the
+// strip's bottom and top rows pass through LDS to the waves below and above
+// (two buffers, one barrier per half-sweep).  The two cells of a pair have
+// opposite colours, so a red-black half-sweep updates exactly one cell per
+// pair and row, the same one for the whole wavefront: no lane idles on the
+// colour test.  Cells outside the grid or the region hold 0 and are never
+// updated.
+constexpr int KR = 3;                                // consecutive region rows per thread
+constexpr int BND = TWAVES * 2 * 64;                 // one boundary buffer: [wave][bottom, top][lane]
+constexpr int TILE_LDS_PAIRS = 2 * BND;              // two buffers (the row sums of the restriction reuse them)
+static_assert(KR * TWAVES >= RYMAX, "a tile region must fit the strips");
+static_assert(2 * RYMAX * NCXMAX <= 2 * TILE_LDS_PAIRS, "restriction row sums must fit the boundary buffers");
+
 template <class T>
 struct alignas(2 * sizeof(T)) Pair {
   T a, b;
 };
 
-// Per row m of this thread: the pair's right-hand side, face coefficients
-// (wA west of A, wB west of B = east of A, eB east of B, s/n south/north),
-// 1/diag and validity.
+// The value held by the previous / next lane of the wavefront (0 for lane 0 /
+// lane 63): DPP wave_shr:1 / wave_shl:1 with bound_ctrl.
+__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ float lane_prev(float v) { return __int_as_float(dpp_prev(__float_as_int(v))); }
+__device__ __forceinline__ float lane_next(float v) { return __int_as_float(dpp_next(__float_as_int(v))); }
+__device__ __forceinline__ double lane_prev(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_prev((int)(unsigned)b), hi = (unsigned)dpp_prev((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ double lane_next(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp_next((int)(unsigned)b), hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Per row m of the strip: right-hand side, faces (wA west of A, wB west of B
+// = east of A, eB east of B, s south; the north faces are the next row's
+// south faces, nt* for the top row), 1/diag, iterate and validity.
 template <class T>
-struct PairRows {
-  T rA[MAXR], rB[MAXR], wA[MAXR], wB[MAXR], eB[MAXR], sA[MAXR], sB[MAXR], nA[MAXR], nB[MAXR], iA[MAXR], iB[MAXR];
-  bool vA[MAXR], vB[MAXR];
+struct Strip {
+  T rA[KR], rB[KR], wA[KR], wB[KR], eB[KR], sA[KR], sB[KR], iA[KR], iB[KR], xA[KR], xB[KR];
+  T ntA, ntB;
+  bool v[KR];
+  __device__ __forceinline__ T nA(int m) const { return m + 1 < KR ? sA[m + 1 < KR ? m + 1 : m] : ntA; }
+  __device__ __forceinline__ T nB(int m) const { return m + 1 < KR ? sB[m + 1 < KR ? m + 1 : m] : ntB; }
 };
 
-// Stages row m (region row r, global row gj) of the pair at global columns
-// (giA, giA + 1), k = ci(giA, gj, nx) (even: nx is even on every tile level).
+// Loads the strip's faces (rows up to RY: the row above the region supplies
+// the top row's north faces) and forms 1/diag; the iterate starts at 0.
 template <class T>
-__device__ __forceinline__ void stage_coeffs(PairRows<T>& pr, int m, const Level<T>& F, int giA, int gj, bool row_in,
-                                             size_t k) {
+__device__ __forceinline__ void strip_stage(Strip<T>& S, const Level<T>& F, int giA, int gy0, int w, int RY) {
   const int nx = F.nx, ny = F.ny;
-  const bool vA = row_in && giA >= 0 && giA < nx, vB = row_in && giA + 1 >= 0 && giA + 1 < nx;
-  pr.vA[m] = vA;
-  pr.vB[m] = vB;
-  T wA = 0, wB = 0, eB = 0, sA = 0, sB = 0, nA = 0, nB = 0;
-  if (vA) {
-    const Pair<T> w2 = reinterpret_cast<const Pair<T>*>(F.bw)[k >> 1];
-    const Pair<T> s2 = reinterpret_cast<const Pair<T>*>(F.bs)[k >> 1];
-    wA = w2.a;
-    wB = w2.b;
-    sA = s2.a;
-    sB = s2.b;
-    if (giA + 2 < nx) eB = F.bw[k + 2];
-    if (gj + 1 < ny) {
-      const Pair<T> n2 = reinterpret_cast<const Pair<T>*>(F.bs)[(k + nx) >> 1];
-      nA = n2.a;
-      nB = n2.b;
+  const bool col_in = giA >= 0 && giA < nx;  // nx is even on every tile level: giA + 1 < nx too
+#pragma unroll
+  for (int m = 0; m < KR; ++m) {
+    const int r = KR * w + m, gj = gy0 + r;
+    const bool ld = col_in && r <= RY && gj >= 0 && gj < ny;
+    S.v[m] = ld && r < RY;
+    T wA = 0, wB = 0, eB = 0, sA = 0, sB = 0;
+    if (ld) {
+      const size_t k = ci(giA, gj, nx);
+      const Pair<T> w2 = reinterpret_cast<const Pair<T>*>(F.bw)[k >> 1];
+      const Pair<T> s2 = reinterpret_cast<const Pair<T>*>(F.bs)[k >> 1];
+      wA = w2.a;
+      wB = w2.b;
+      sA = s2.a;
+      sB = s2.b;
+      if (giA + 2 < nx) eB = F.bw[k + 2];
     }
+    S.wA[m] = wA;
+    S.wB[m] = wB;
+    S.eB[m] = eB;
+    S.sA[m] = sA;
+    S.sB[m] = sB;
+    S.xA[m] = 0;
+    S.xB[m] = 0;
   }
-  pr.wA[m] = wA;
-  pr.wB[m] = wB;
-  pr.eB[m] = eB;
-  pr.sA[m] = sA;
-  pr.sB[m] = sB;
-  pr.nA[m] = nA;
-  pr.nB[m] = nB;
-  pr.iA[m] = vA ? (T)1 / (wA + wB + sA + nA) : (T)0;
-  pr.iB[m] = vB ? (T)1 / (wB + eB + sB + nB) : (T)0;
-}
-
-template <class T>
-__device__ __forceinline__ void zero_ring(Pair<T> (*xs)[66], int RY) {
-  for (int t = threadIdx.x; t < 66; t += TBLOCK) {
-    xs[0][t] = Pair<T>{0, 0};
-    xs[RY + 1][t] = Pair<T>{0, 0};
-  }
-  for (int t = threadIdx.x; t < RY; t += TBLOCK) {
-    xs[t + 1][0] = Pair<T>{0, 0};
-    xs[t + 1][65] = Pair<T>{0, 0};
-  }
-}
-
-// One red-black half-sweep: cells of one colour read only cells of the other,
-// so all neighbour reads are gathered first, then the new values written.
-// Cell (i,j) has colour (i + j) & 1; A (even column) has colour (gy0 + r) & 1.
-template <class T>
-__device__ __forceinline__ void pair_sweep(PairRows<T>& pr, Pair<T> (*xs)[66], int c, int w, int RY, int gy0,
-                                           int color, bool zero) {
-  T off[MAXR];
-#pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m;
-    off[m] = 0;
-    if (zero || r >= RY) continue;
-    const Pair<T> own = xs[r + 1][c + 1], up = xs[r + 2][c + 1], dn = xs[r][c + 1];
-    if (((gy0 + r) & 1) == color)
-      off[m] = pr.wA[m] * xs[r + 1][c].b + pr.wB[m] * own.b + pr.sA[m] * dn.a + pr.nA[m] * up.a;
-    else
-      off[m] = pr.wB[m] * own.a + pr.eB[m] * xs[r + 1][c + 2].a + pr.sB[m] * dn.b + pr.nB[m] * up.b;
+  {
+    const int r = KR * w + KR - 1, gj = gy0 + r + 1;
+    T a = 0, b = 0;
+    if (col_in && r + 1 <= RY && gj >= 0 && gj < ny) {
+      const Pair<T> n2 = reinterpret_cast<const Pair<T>*>(F.bs)[ci(giA, gj, nx) >> 1];
+      a = n2.a;
+      b = n2.b;
+    }
+    S.ntA = a;
+    S.ntB = b;
   }
 #pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m;
-    if (r >= RY) continue;
-    if (((gy0 + r) & 1) == color) {
-      if (pr.vA[m]) xs[r + 1][c + 1].a = (pr.rA[m] + off[m]) * pr.iA[m];
+  for (int m = 0; m < KR; ++m) {
+    S.iA[m] = S.v[m] ? (T)1 / (S.wA[m] + S.wB[m] + S.sA[m] + S.nA(m)) : (T)0;
+    S.iB[m] = S.v[m] ? (T)1 / (S.wB[m] + S.eB[m] + S.sB[m] + S.nB(m)) : (T)0;
+  }
+}
+
+// Boundary rows of the neighbouring strips from buffer rd: the top row of
+// wave w - 1 (below this strip) and the bottom row of wave w + 1 (above).
+template <class T>
+__device__ __forceinline__ void strip_halo(const Pair<T>* rd, int c, int w, Pair<T>& dn, Pair<T>& up) {
+  dn = w > 0 ? rd[((w - 1) * 2 + 1) * 64 + c] : Pair<T>{0, 0};
+  up = w + 1 < TWAVES ? rd[(w + 1) * 2 * 64 + c] : Pair<T>{0, 0};
+}
+
+// Half-sweep k of colour `color` (cells (i + j) & 1 == color; A, the even
+// column, has colour (gy0 + r) & 1): reads the boundary buffer (k + 1) & 1,
+// publishes the strip's boundary rows to buffer k & 1.  A cell of the swept
+// colour reads only cells of the other colour, which this half-sweep leaves
+// unchanged, so the rows update in place in any order.  zero: the iterate is
+// 0 (first pre-smoothing half-sweep), no neighbour is read.
+template <class T>
+__device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int w, int gy0, int color, bool zero,
+                                           int k) {
+  Pair<T> dn{0, 0}, up{0, 0};
+  if (!zero) strip_halo<T>(bnd + ((k + 1) & 1) * BND, c, w, dn, up);
+#pragma unroll
+  for (int m = 0; m < KR; ++m) {
+    const bool doA = ((gy0 + KR * w + m) & 1) == color;
+    const T xl = lane_prev(S.xB[m]), xr = lane_next(S.xA[m]);
+    const T dA = m > 0 ? S.xA[m > 0 ? m - 1 : 0] : dn.a, dB = m > 0 ? S.xB[m > 0 ? m - 1 : 0] : dn.b;
+    const T uA = m + 1 < KR ? S.xA[m + 1 < KR ? m + 1 : m] : up.a;
+    const T uB = m + 1 < KR ? S.xB[m + 1 < KR ? m + 1 : m] : up.b;
+    if (doA) {
+      const T o = zero ? (T)0 : S.wA[m] * xl + S.wB[m] * S.xB[m] + S.sA[m] * dA + S.nA(m) * uA;
+      if (S.v[m]) S.xA[m] = (S.rA[m] + o) * S.iA[m];
     } else {
-      if (pr.vB[m]) xs[r + 1][c + 1].b = (pr.rB[m] + off[m]) * pr.iB[m];
+      const T o = zero ? (T)0 : S.wB[m] * S.xA[m] + S.eB[m] * xr + S.sB[m] * dB + S.nB(m) * uB;
+      if (S.v[m]) S.xB[m] = (S.rB[m] + o) * S.iB[m];
     }
   }
+  Pair<T>* wr = bnd + (k & 1) * BND;
+  wr[(w * 2) * 64 + c] = Pair<T>{S.xA[0], S.xB[0]};
+  wr[(w * 2 + 1) * 64 + c] = Pair<T>{S.xA[KR - 1], S.xB[KR - 1]};
+  __syncthreads();
 }
 
 // --------------------------------------------------------------------------- pre-smoothing tile kernel
@@ -404,11 +447,11 @@ __device__ __forceinline__ void pair_sweep(PairRows<T>& pr, Pair<T> (*xs)[66], i
 // Restriction r_H = P^T res: each cell splits its residual into the four
 // weight products; the x sums run across lanes, the y sums through LDS.
 template <class T, bool FUSE>
-__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_min_blocks<T>()))) k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu,
-                                                   T* rA, T* rB, T* xsol, const T* pA, const T* pB,
-                                                   const T* __restrict__ qv, double* partial, Scal* s) {
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_min_blocks<T>())))
+k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB, T* xsol, const T* pA, const T* pB,
+         const T* __restrict__ qv, double* partial, Scal* s) {
   if (s->done) return;
-  __shared__ Pair<T> xs[RYMAX + 2][66];
+  __shared__ Pair<T> lds[TILE_LDS_PAIRS];
   const int H = 2 * nu + 1, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
   xcd_tile(bx, by);
@@ -416,7 +459,7 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
   const int gx0 = tx0 - HX, gy0 = ty0 - H;
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int giA = gx0 + 2 * c;
-  const int nx = F.nx, ny = F.ny;
+  const int nx = F.nx;
   const int hc = HX / 2, NCX = TX / 2;
   const bool pair_int = c >= hc && c < hc + NCX;       // both cells in the tile interior
   const bool pair_fp = c >= hc - 1 && c <= hc + NCX;   // restriction footprint
@@ -435,15 +478,34 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
     pdir = ((s->iter + 1) & 1) ? pB : pA;
   }
   const T ta = (T)alpha, tshift = (T)shift;
-  PairRows<T> pr;
+  Strip<T> S;
+  strip_stage<T>(S, F, giA, gy0, w, RY);
+  // interpolation weights of the restriction footprint: loaded with the rest on
+  // coarse levels, after the sweeps on the finest (not live during them there)
+  T wxA[KR], wxB[KR], wyA[KR], wyB[KR];
+  auto load_weights = [&]() {
 #pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m, gj = gy0 + r;
-    const bool row_in = r < RY && gj >= 0 && gj < ny;
-    const size_t k = row_in && giA >= 0 && giA < nx ? ci(giA, gj, nx) : 0;
-    stage_coeffs<T>(pr, m, F, giA, gj, row_in, k);
+    for (int m = 0; m < KR; ++m) {
+      const int r = KR * w + m;
+      wxA[m] = wxB[m] = wyA[m] = wyB[m] = 0;
+      if (S.v[m] && pair_fp && r >= H - 1 && r <= H + TY) {
+        const size_t k = ci(giA, gy0 + r, nx);
+        const Pair<T> wx2 = reinterpret_cast<const Pair<T>*>(F.wx)[k >> 1];
+        const Pair<T> wy2 = reinterpret_cast<const Pair<T>*>(F.wy)[k >> 1];
+        wxA[m] = wx2.a;
+        wxB[m] = wx2.b;
+        wyA[m] = wy2.a;
+        wyB[m] = wy2.b;
+      }
+    }
+  };
+  if (!FUSE) load_weights();
+#pragma unroll
+  for (int m = 0; m < KR; ++m) {
+    const int r = KR * w + m;
     T ra = 0, rb = 0;
-    if (pr.vA[m]) {
+    if (S.v[m]) {
+      const size_t k = ci(giA, gy0 + r, nx);
       const Pair<T> r2 = reinterpret_cast<const Pair<T>*>(rin)[k >> 1];
       ra = r2.a;
       rb = r2.b;
@@ -468,60 +530,42 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
         }
       }
     }
-    pr.rA[m] = ra;
-    pr.rB[m] = rb;
-    if (r < RY) xs[r + 1][c + 1] = Pair<T>{0, 0};
+    S.rA[m] = ra;
+    S.rB[m] = rb;
   }
-  zero_ring<T>(xs, RY);
-  __syncthreads();
-  for (int sw = 0; sw < nu; ++sw)
-    for (int color = 0; color < 2; ++color) {
-      pair_sweep<T>(pr, xs, c, w, RY, gy0, color, sw == 0 && color == 0);
-      __syncthreads();
-    }
-  // interpolation weights of the restriction footprint (loaded only now: not live in the sweeps)
-  T wxA[MAXR], wxB[MAXR], wyA[MAXR], wyB[MAXR];
+  for (int k = 0; k < 2 * nu; ++k) strip_half<T>(S, lds, c, w, gy0, k & 1, k == 0, k);
+  if (FUSE) load_weights();
+  // residual (reference order: r - (diag x - offsum)) with the last published
+  // boundary rows (buffer (2 nu - 1) & 1), and the interior write of x
+  T resA[KR], resB[KR];
+  {
+    Pair<T> dn, up;
+    strip_halo<T>(lds + BND, c, w, dn, up);
 #pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m, gj = gy0 + r;
-    wxA[m] = wxB[m] = wyA[m] = wyB[m] = 0;
-    if (pr.vA[m] && pair_fp && r >= H - 1 && r <= H + TY) {
-      const size_t k = ci(giA, gj, nx);
-      const Pair<T> wx2 = reinterpret_cast<const Pair<T>*>(F.wx)[k >> 1];
-      const Pair<T> wy2 = reinterpret_cast<const Pair<T>*>(F.wy)[k >> 1];
-      wxA[m] = wx2.a;
-      wxB[m] = wx2.b;
-      wyA[m] = wy2.a;
-      wyB[m] = wy2.b;
+    for (int m = 0; m < KR; ++m) {
+      const int r = KR * w + m;
+      const T xl = lane_prev(S.xB[m]), xr = lane_next(S.xA[m]);
+      const T dA = m > 0 ? S.xA[m > 0 ? m - 1 : 0] : dn.a, dB = m > 0 ? S.xB[m > 0 ? m - 1 : 0] : dn.b;
+      const T uA = m + 1 < KR ? S.xA[m + 1 < KR ? m + 1 : m] : up.a;
+      const T uB = m + 1 < KR ? S.xB[m + 1 < KR ? m + 1 : m] : up.b;
+      resA[m] = 0;
+      resB[m] = 0;
+      if (!S.v[m]) continue;
+      resA[m] = S.rA[m] - ((S.wA[m] + S.wB[m] + S.sA[m] + S.nA(m)) * S.xA[m] -
+                           (S.wA[m] * xl + S.wB[m] * S.xB[m] + S.sA[m] * dA + S.nA(m) * uA));
+      resB[m] = S.rB[m] - ((S.wB[m] + S.eB[m] + S.sB[m] + S.nB(m)) * S.xB[m] -
+                           (S.wB[m] * S.xA[m] + S.eB[m] * xr + S.sB[m] * dB + S.nB(m) * uB));
+      if (pair_int && r >= H && r < H + TY)
+        reinterpret_cast<Pair<T>*>(x_lvl)[ci(giA, gy0 + r, nx) >> 1] = Pair<T>{S.xA[m], S.xB[m]};
     }
   }
-  // residual (reference order: r - (diag x - offsum)) and the interior write of x
-  T resA[MAXR], resB[MAXR];
-#pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m;
-    resA[m] = 0;
-    resB[m] = 0;
-    if (r >= RY) continue;
-    const Pair<T> own = xs[r + 1][c + 1], up = xs[r + 2][c + 1], dn = xs[r][c + 1];
-    const T xw = xs[r + 1][c].b, xe = xs[r + 1][c + 2].a;
-    if (pr.vA[m])
-      resA[m] = pr.rA[m] - ((pr.wA[m] + pr.wB[m] + pr.sA[m] + pr.nA[m]) * own.a -
-                            (pr.wA[m] * xw + pr.wB[m] * own.b + pr.sA[m] * dn.a + pr.nA[m] * up.a));
-    if (pr.vB[m])
-      resB[m] = pr.rB[m] - ((pr.wB[m] + pr.eB[m] + pr.sB[m] + pr.nB[m]) * own.b -
-                            (pr.wB[m] * own.a + pr.eB[m] * xe + pr.sB[m] * dn.b + pr.nB[m] * up.b));
-    if (pr.vA[m] && pair_int && r >= H && r < H + TY)
-      reinterpret_cast<Pair<T>*>(x_lvl)[ci(giA, gy0 + r, nx) >> 1] = own;
-  }
-  __syncthreads();  // xs is reused below for the row sums us[2][RYMAX][NCXMAX]
-  T* us = &xs[0][0].a;
+  __syncthreads();  // the boundary buffers are reused below for the row sums us[2][RYMAX][NCXMAX]
+  T* us = &lds[0].a;
   const int NCY = TY / 2, cx0 = tx0 / 2, cy0 = ty0 / 2;
   const int ic = c - hc;
 #pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m;
-    if (r >= RY) continue;  // uniform across the wavefront
+  for (int m = 0; m < KR; ++m) {
+    const int r = KR * w + m;
     // a*: part of res for the parent row (weight wy), b*: for the neighbouring row (1 - wy);
     // p*: part for the parent column (wx), q*: for the neighbouring column (1 - wx).
     const T aA = resA[m] * wyA[m], bA = resA[m] * ((T)1 - wyA[m]);
@@ -529,8 +573,8 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
     const T paA = wxA[m] * aA, qaA = ((T)1 - wxA[m]) * aA, pbA = wxA[m] * bA, qbA = ((T)1 - wxA[m]) * bA;
     const T paB = wxB[m] * aB, qaB = ((T)1 - wxB[m]) * aB, pbB = wxB[m] * bB, qbB = ((T)1 - wxB[m]) * bB;
     // coarse column I (this lane) gathers q(2I-1) + p(2I) + p(2I+1) + q(2I+2)
-    const T sa = (paA + paB) + (__shfl_up(qaB, 1, 64) + __shfl_down(qaA, 1, 64));
-    const T sb = (pbA + pbB) + (__shfl_up(qbB, 1, 64) + __shfl_down(qbA, 1, 64));
+    const T sa = (paA + paB) + (lane_prev(qaB) + lane_next(qaA));
+    const T sb = (pbA + pbB) + (lane_prev(qbB) + lane_next(qbA));
     if (pair_int && r >= H - 1 && r <= H + TY) {
       us[r * NCXMAX + ic] = sa;
       us[(RYMAX + r) * NCXMAX + ic] = sb;
@@ -546,13 +590,13 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
                             (us[(jr + 1) * NCXMAX + a] + us[(RYMAX + jr + 2) * NCXMAX + a]);
   }
   if (FUSE) {
-    // norms of the updated residual over the tile interior (pr.r holds it)
+    // norms of the updated residual over the tile interior (S.r holds it)
     double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-    for (int m = 0; m < MAXR; ++m) {
-      const int r = w + TWAVES * m;
-      if (!pr.vA[m] || !pair_int || r < H || r >= H + TY) continue;
-      const double ra = (double)pr.rA[m], rb = (double)pr.rB[m];
+    for (int m = 0; m < KR; ++m) {
+      const int r = KR * w + m;
+      if (!S.v[m] || !pair_int || r < H || r >= H + TY) continue;
+      const double ra = (double)S.rA[m], rb = (double)S.rB[m];
       acc[0] += ra * ra + rb * rb;
       acc[1] = fmax(acc[1], fmax(fabs(ra), fabs(rb)));
       acc[2] += ra + rb;
@@ -590,17 +634,18 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
 // --------------------------------------------------------------------------- post-smoothing tile kernel
 // Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, H = 2 nu (even): HX = H,
 // TX = 128 - 2H.  x = x_pre + P x_coarse (the coarse values around the tile are
-// staged in LDS), then nu (black, red) sweeps; the tile interior is written to
-// x_out.  LEVEL0: r = rbuf[s->rpar], x_out = z, and the block partials of r.z,
-// sum z, sum r are stored.
+// staged in LDS while the strip's own loads are in flight), then nu (black,
+// red) sweeps; the tile interior is written to x_out.  LEVEL0: r =
+// rbuf[s->rpar], x_out = z, and the block partials of r.z, sum z, sum r are
+// stored.
 constexpr int PCW = 66;                           // staged coarse columns
 constexpr int PCH = (TY + 4 * MAX_NU) / 2 + 3;    // staged coarse rows
 template <class T, bool LEVEL0>
-__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_min_blocks<T>()))) k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA,
-                                                    const T* rB, const T* x_in, T* x_out, int nu, double* partial,
-                                                    Scal* s) {
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_min_blocks<T>())))
+k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, const T* x_in, T* x_out, int nu,
+          double* partial, Scal* s) {
   if (s->done) return;
-  __shared__ Pair<T> xs[RYMAX + 2][66];
+  __shared__ Pair<T> lds[TILE_LDS_PAIRS];
   __shared__ T xcs[PCH][PCW];
   const int H = 2 * nu, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
@@ -609,65 +654,69 @@ __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 *
   const int gx0 = tx0 - HX, gy0 = ty0 - H;
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int giA = gx0 + 2 * c;
-  const int nx = F.nx, ny = F.ny, cnx = Cc.nx, cny = Cc.ny;
+  const int nx = F.nx, cnx = Cc.nx, cny = Cc.ny;
   const int hc = HX / 2, NCX = TX / 2;
   const bool pair_int = c >= hc && c < hc + NCX;
   const int cxa = gx0 / 2 - 1, cya = (gy0 >> 1) - 1;  // first staged coarse column / row (gx0 is even)
   const T* rin = LEVEL0 ? (s->rpar ? rB : rA) : r_lvl;
-  // the coarse correction around the tile into LDS first, then each row is
-  // staged and prolongated at once (the row's x_pre and weights die there)
   for (int t = threadIdx.x; t < PCH * PCW; t += TBLOCK) {
     const int I = min(max(cxa + t % PCW, 0), cnx - 1), J = min(max(cya + t / PCW, 0), cny - 1);
     xcs[t / PCW][t % PCW] = Cc.xb[ci(I, J, cnx)];
   }
-  zero_ring<T>(xs, RY);
-  __syncthreads();
-  PairRows<T> pr;
+  Strip<T> S;
+  strip_stage<T>(S, F, giA, gy0, w, RY);
+  T wxA[KR], wxB[KR], wyA[KR], wyB[KR];
 #pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m, gj = gy0 + r;
-    const bool row_in = r < RY && gj >= 0 && gj < ny;
-    const size_t k = row_in && giA >= 0 && giA < nx ? ci(giA, gj, nx) : 0;
-    stage_coeffs<T>(pr, m, F, giA, gj, row_in, k);
-    Pair<T> rr{0, 0}, x0{0, 0};
-    if (pr.vA[m]) {
+  for (int m = 0; m < KR; ++m) {
+    Pair<T> rr{0, 0}, x0{0, 0}, wx2{0, 0}, wy2{0, 0};
+    if (S.v[m]) {
+      const size_t k = ci(giA, gy0 + KR * w + m, nx);
       rr = reinterpret_cast<const Pair<T>*>(rin)[k >> 1];
       x0 = reinterpret_cast<const Pair<T>*>(x_in)[k >> 1];
-      const Pair<T> wx2 = reinterpret_cast<const Pair<T>*>(F.wx)[k >> 1];
-      const Pair<T> wy2 = reinterpret_cast<const Pair<T>*>(F.wy)[k >> 1];
-      // prolongation: A (column 2I) takes coarse I and I-1, B takes I and I+1
-      const int I = giA >> 1, J = gj >> 1;
-      int J2 = J + ((gj & 1) ? 1 : -1);
-      J2 = J2 < 0 ? 0 : (J2 > cny - 1 ? cny - 1 : J2);
-      const int IA2 = I - 1 < 0 ? 0 : I - 1, IB2 = I + 1 > cnx - 1 ? cnx - 1 : I + 1;
-      const int a = I - cxa, aA = IA2 - cxa, aB = IB2 - cxa, b = J - cya, b2 = J2 - cya;
-      const T wxa = wx2.a, wya = wy2.a, wxb = wx2.b, wyb = wy2.b;
-      x0.a += wxa * wya * xcs[b][a] + ((T)1 - wxa) * wya * xcs[b][aA] + wxa * ((T)1 - wya) * xcs[b2][a] +
-              ((T)1 - wxa) * ((T)1 - wya) * xcs[b2][aA];
-      x0.b += wxb * wyb * xcs[b][a] + ((T)1 - wxb) * wyb * xcs[b][aB] + wxb * ((T)1 - wyb) * xcs[b2][a] +
-              ((T)1 - wxb) * ((T)1 - wyb) * xcs[b2][aB];
+      wx2 = reinterpret_cast<const Pair<T>*>(F.wx)[k >> 1];
+      wy2 = reinterpret_cast<const Pair<T>*>(F.wy)[k >> 1];
     }
-    pr.rA[m] = rr.a;
-    pr.rB[m] = rr.b;
-    if (r < RY) xs[r + 1][c + 1] = x0;
+    S.rA[m] = rr.a;
+    S.rB[m] = rr.b;
+    S.xA[m] = x0.a;
+    S.xB[m] = x0.b;
+    wxA[m] = wx2.a;
+    wxB[m] = wx2.b;
+    wyA[m] = wy2.a;
+    wyB[m] = wy2.b;
   }
+  __syncthreads();  // xcs
+  // prolongation: A (column 2I) takes coarse I and I-1, B takes I and I+1
+#pragma unroll
+  for (int m = 0; m < KR; ++m) {
+    if (!S.v[m]) continue;
+    const int gj = gy0 + KR * w + m;
+    const int I = giA >> 1, J = gj >> 1;
+    int J2 = J + ((gj & 1) ? 1 : -1);
+    J2 = J2 < 0 ? 0 : (J2 > cny - 1 ? cny - 1 : J2);
+    const int IA2 = I - 1 < 0 ? 0 : I - 1, IB2 = I + 1 > cnx - 1 ? cnx - 1 : I + 1;
+    const int a = I - cxa, aA = IA2 - cxa, aB = IB2 - cxa, b = J - cya, b2 = J2 - cya;
+    const T wxa = wxA[m], wya = wyA[m], wxb = wxB[m], wyb = wyB[m];
+    S.xA[m] += wxa * wya * xcs[b][a] + ((T)1 - wxa) * wya * xcs[b][aA] + wxa * ((T)1 - wya) * xcs[b2][a] +
+               ((T)1 - wxa) * ((T)1 - wya) * xcs[b2][aA];
+    S.xB[m] += wxb * wyb * xcs[b][a] + ((T)1 - wxb) * wyb * xcs[b][aB] + wxb * ((T)1 - wyb) * xcs[b2][a] +
+               ((T)1 - wxb) * ((T)1 - wyb) * xcs[b2][aB];
+  }
+  // publish the boundary rows for the first half-sweep (it reads buffer 1)
+  lds[BND + (w * 2) * 64 + c] = Pair<T>{S.xA[0], S.xB[0]};
+  lds[BND + (w * 2 + 1) * 64 + c] = Pair<T>{S.xA[KR - 1], S.xB[KR - 1]};
   __syncthreads();
-  for (int sw = 0; sw < nu; ++sw)
-    for (int cc = 0; cc < 2; ++cc) {
-      pair_sweep<T>(pr, xs, c, w, RY, gy0, 1 - cc, false);
-      __syncthreads();
-    }
+  for (int k = 0; k < 2 * nu; ++k) strip_half<T>(S, lds, c, w, gy0, 1 - (k & 1), false, k);
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int m = 0; m < MAXR; ++m) {
-    const int r = w + TWAVES * m;
-    if (r >= RY || !pr.vA[m] || !pair_int || r < H || r >= H + TY) continue;
-    const Pair<T> z = xs[r + 1][c + 1];
-    reinterpret_cast<Pair<T>*>(x_out)[ci(giA, gy0 + r, nx) >> 1] = z;
+  for (int m = 0; m < KR; ++m) {
+    const int r = KR * w + m;
+    if (!S.v[m] || !pair_int || r < H || r >= H + TY) continue;
+    reinterpret_cast<Pair<T>*>(x_out)[ci(giA, gy0 + r, nx) >> 1] = Pair<T>{S.xA[m], S.xB[m]};
     if (LEVEL0) {
-      acc[0] += (double)pr.rA[m] * (double)z.a + (double)pr.rB[m] * (double)z.b;
-      acc[1] += (double)z.a + (double)z.b;
-      acc[2] += (double)pr.rA[m] + (double)pr.rB[m];
+      acc[0] += (double)S.rA[m] * (double)S.xA[m] + (double)S.rB[m] * (double)S.xB[m];
+      acc[1] += (double)S.xA[m] + (double)S.xB[m];
+      acc[2] += (double)S.rA[m] + (double)S.rB[m];
     }
   }
   if (LEVEL0) block_partial<3, TBLOCK>(acc, 0u, partial);
