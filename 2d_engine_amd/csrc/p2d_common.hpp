@@ -89,18 +89,37 @@ struct Scal {
 };
 
 // --------------------------------------------------------------------------- reductions
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
+// Cross-lane moves of a 64-bit value (two 32-bit DPP moves); lanes without a
+// source receive 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
-  return v;
+// Wavefront sum / max (of non-negative values: the empty lanes enter as 0),
+// returned to every lane: quad permutations, row shifts by 4 and 8, then the
+// row broadcasts of lanes 15 and 31 leave the total in lane 63.
+template <bool MAX>
+__device__ __forceinline__ double wave_reduce(double v) {
+  auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+  v = op(v, dpp_move<0xb1>(v));   // quad_perm [1,0,3,2]
+  v = op(v, dpp_move<0x4e>(v));   // quad_perm [2,3,0,1]
+  v = op(v, dpp_move<0x114>(v));  // row_shr:4
+  v = op(v, dpp_move<0x118>(v));  // row_shr:8
+  v = op(v, dpp_move<0x142>(v));  // row_bcast:15
+  v = op(v, dpp_move<0x143>(v));  // row_bcast:31
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
+__device__ __forceinline__ double wave_sum(double v) { return wave_reduce<false>(v); }
+__device__ __forceinline__ double wave_max(double v) { return wave_reduce<true>(v); }
 
 // Block reduction; bit k of maxmask selects max (else sum).  Result in thread 0.
+// Called by every thread of the block (the cross-lane moves need whole wavefronts).
 template <int K, int NT>
 __device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
   __shared__ double sh[K][NT / 64];
@@ -113,12 +132,11 @@ __device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
     for (int k = 0; k < K; ++k) sh[k][w] = v[k];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (w == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      double a = sh[k][0];
-      for (int q = 1; q < NT / 64; ++q) a = ((maxmask >> k) & 1) ? fmax(a, sh[k][q]) : a + sh[k][q];
-      v[k] = a;
+      const double a = lane < NT / 64 ? sh[k][lane] : 0.0;
+      v[k] = ((maxmask >> k) & 1) ? wave_max(a) : wave_sum(a);
     }
   }
 }

@@ -36,10 +36,12 @@ constexpr int NCXMAX = 64;                 // coarse columns of a tile at most (
 // that the region starts at an even column; TX = 128 - 2 HX interior columns.
 __host__ __device__ constexpr int tile_hx(int H) { return H + (H & 1); }
 __host__ __device__ constexpr int tile_tx(int H) { return 128 - 2 * tile_hx(H); }
-// FP32 tiles keep two blocks resident per CU (<= 64 VGPRs) so that one block's
-// loads overlap the other's LDS sweeps; FP64 tiles need the registers.
-template <class T>
-__host__ __device__ constexpr int tile_min_blocks() { return sizeof(T) == 4 ? 2 : 1; }
+// Blocks per CU a tile kernel is compiled for: the finest level's FP32 tiles
+// keep two resident (<= 64 VGPRs) so that one block's loads overlap the
+// other's sweeps; FP64 tiles, and the coarse levels (a few blocks per CU at
+// most), take the registers of one.
+template <class T, bool FINEST>
+__host__ __device__ constexpr int tile_blocks() { return sizeof(T) == 4 && FINEST ? 2 : 1; }
 constexpr int DTY = 16;                    // rows per block of the direction kernel
 constexpr int SMALL_MAX_CELLS = 1024;      // top level of the single-workgroup V-cycle at most (32^2)
 constexpr int MAXL = 24;                   // multigrid levels at most
@@ -351,7 +353,7 @@ struct Strip {
 };
 
 // Loads the strip's faces (rows up to RY: the row above the region supplies
-// the top row's north faces) and forms 1/diag; the iterate starts at 0.
+// the top row's north faces); the iterate starts at 0.
 template <class T>
 __device__ __forceinline__ void strip_stage(Strip<T>& S, const Level<T>& F, int giA, int gy0, int w, int RY) {
   const int nx = F.nx, ny = F.ny;
@@ -391,6 +393,11 @@ __device__ __forceinline__ void strip_stage(Strip<T>& S, const Level<T>& F, int 
     S.ntA = a;
     S.ntB = b;
   }
+}
+// 1/diag of the strip's cells (after strip_stage; kept out of it so that the
+// post kernel forms them only once the prolongation's registers are free).
+template <class T>
+__device__ __forceinline__ void strip_diag(Strip<T>& S) {
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
     S.iA[m] = S.v[m] ? (T)1 / (S.wA[m] + S.wB[m] + S.sA[m] + S.nA(m)) : (T)0;
@@ -447,7 +454,7 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
 // Restriction r_H = P^T res: each cell splits its residual into the four
 // weight products; the x sums run across lanes, the y sums through LDS.
 template <class T, bool FUSE>
-__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_min_blocks<T>())))
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE>())))
 k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB, T* xsol, const T* pA, const T* pB,
          const T* __restrict__ qv, double* partial, Scal* s) {
   if (s->done) return;
@@ -480,6 +487,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
   const T ta = (T)alpha, tshift = (T)shift;
   Strip<T> S;
   strip_stage<T>(S, F, giA, gy0, w, RY);
+  strip_diag<T>(S);
   // interpolation weights of the restriction footprint: loaded with the rest on
   // coarse levels, after the sweeps on the finest (not live during them there)
   T wxA[KR], wxB[KR], wyA[KR], wyB[KR];
@@ -641,7 +649,7 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
 constexpr int PCW = 66;                           // staged coarse columns
 constexpr int PCH = (TY + 4 * MAX_NU) / 2 + 3;    // staged coarse rows
 template <class T, bool LEVEL0>
-__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_min_blocks<T>())))
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, LEVEL0>())))
 k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, const T* x_in, T* x_out, int nu,
           double* partial, Scal* s) {
   if (s->done) return;
@@ -702,6 +710,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
     S.xB[m] += wxb * wyb * xcs[b][a] + ((T)1 - wxb) * wyb * xcs[b][aB] + wxb * ((T)1 - wyb) * xcs[b2][a] +
                ((T)1 - wxb) * ((T)1 - wyb) * xcs[b2][aB];
   }
+  strip_diag<T>(S);
   // publish the boundary rows for the first half-sweep (it reads buffer 1)
   lds[BND + (w * 2) * 64 + c] = Pair<T>{S.xA[0], S.xB[0]};
   lds[BND + (w * 2 + 1) * 64 + c] = Pair<T>{S.xA[KR - 1], S.xB[KR - 1]};
