@@ -43,7 +43,7 @@ __host__ __device__ constexpr int tile_tx(int H) { return 128 - 2 * tile_hx(H); 
 template <class T, bool FINEST>
 __host__ __device__ constexpr int tile_blocks() { return sizeof(T) == 4 && FINEST ? 2 : 1; }
 constexpr int DTY = 16;                    // rows per block of the direction kernel
-constexpr int SMALL_MAX_CELLS = 1024;      // top level of the single-workgroup V-cycle at most (32^2)
+constexpr int SMALL_MAX_CELLS = 4096;      // top level of the single-workgroup V-cycle at most (64^2)
 constexpr int MAXL = 24;                   // multigrid levels at most
 constexpr int BOTTOM_LDS_MAX = 64;         // bottom sizes inverted in LDS
 
@@ -747,15 +747,24 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_post0(const double* __re
 }
 
 // --------------------------------------------------------------------------- single-workgroup coarse V-cycle
-// Levels l0..L-1 (at most SMALL_LV, each of at most SMALL_THREADS cells) in
-// one workgroup.  Thread t owns cell t of every level and keeps that cell's
-// coefficients, 1/diag and right-hand side in registers; LDS holds x and r of
-// every level, the interpolation weights above the bottom, the residual and
-// the two restriction row-sum arrays, and the bottom pseudo-inverse when the
-// bottom has at most BOTTOM_LDS_MAX cells.  Every global load is issued in the
-// first phase.  The bottom applies the dense pseudo-inverse; the cycle's
-// result goes to level l0's xb.
+// Levels l0..L-1 (at most SMALL_LV; the top one of at most SMALL_MAX_CELLS
+// cells, the others of at most SMALL_THREADS) in one workgroup.  Thread t owns
+// cells t + SMALL_THREADS u (u < SMALL_CPT) of the top level and cell t of
+// every level below, and keeps each owned cell's coefficients, diagonal and
+// right-hand side in registers ("slots"); LDS holds x of every level, r of
+// the levels below the top (filled by the restriction), the interpolation
+// weights above the bottom, the residual and the two restriction row-sum
+// arrays, and the bottom pseudo-inverse when the bottom has at most
+// BOTTOM_LDS_MAX cells.  Every global load is issued in the first phase.  The
+// bottom applies the dense pseudo-inverse; the cycle's result goes to level
+// l0's xb.
 constexpr int SMALL_LV = 4;
+constexpr int SMALL_CPT = SMALL_MAX_CELLS / SMALL_THREADS;  // top-level cells per thread
+constexpr int SMALL_SLOTS = SMALL_CPT + SMALL_LV - 1;
+static_assert(SMALL_CPT * SMALL_THREADS == SMALL_MAX_CELLS && SMALL_MAX_CELLS <= 4 * SMALL_THREADS,
+              "levels below the top must have at most one cell per thread");
+__host__ __device__ constexpr int small_cpt(int q) { return q == 0 ? SMALL_CPT : 1; }
+__host__ __device__ constexpr int small_slot(int q, int u) { return q == 0 ? u : SMALL_CPT - 1 + q; }
 
 template <class T>
 struct SmallArgs {
@@ -779,8 +788,8 @@ __host__ __device__ inline SmallLayout small_layout(const Level<T>* lv, int nl) 
     const int n = lv[q].nx * lv[q].ny;
     o.x[q] = off;
     off += n;
-    o.r[q] = off;
-    off += n;
+    o.r[q] = off;  // the top level's r lives in registers unless it is the bottom
+    if (q > 0 || nl == 1) off += n;
     if (q + 1 < nl) {
       o.wx[q] = off;
       off += n;
@@ -805,20 +814,6 @@ __host__ __device__ inline SmallLayout small_layout(const Level<T>* lv, int nl) 
   return o;
 }
 
-// One red-black half-sweep of cell t (coordinates i, j) of a small level.
-template <class T>
-__device__ __forceinline__ void small_half(T* X, int t, int n, int nx, int ny, int i, int j, T cw, T ce, T cs, T cn,
-                                           T id, T rr, int color, bool zero) {
-  if (t < n && ((i + j) & 1) == color) {
-    T o = 0;
-    if (!zero)
-      o = cw * X[i > 0 ? t - 1 : t] + ce * X[i + 1 < nx ? t + 1 : t] + cs * X[j > 0 ? t - nx : t] +
-          cn * X[j + 1 < ny ? t + nx : t];
-    X[t] = (rr + o) * id;
-  }
-  __syncthreads();
-}
-
 template <class T>
 __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int nu_pre, int nu_post, const Scal* s) {
   if (s->done) return;
@@ -827,61 +822,85 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
   const int nl = A.nl, t = threadIdx.x;
   const SmallLayout L = small_layout<T>(A.lv, nl);
   int nb = 0;
-  int ci_[SMALL_LV], cj_[SMALL_LV];
-  T cw[SMALL_LV], ce[SMALL_LV], cs[SMALL_LV], cn[SMALL_LV], cd[SMALL_LV], rr[SMALL_LV];
-  // every global load of the cycle: coefficients (registers), r of level l0,
-  // weights and the bottom pseudo-inverse (LDS); x of every level cleared
+  int ci_[SMALL_SLOTS], cj_[SMALL_SLOTS];
+  T cw[SMALL_SLOTS], ce[SMALL_SLOTS], cs[SMALL_SLOTS], cn[SMALL_SLOTS], cd[SMALL_SLOTS], id[SMALL_SLOTS],
+      rr[SMALL_SLOTS];
+  // every global load of the cycle: coefficients and the top level's r
+  // (registers), weights and the bottom pseudo-inverse (LDS); x cleared
 #pragma unroll
   for (int q = 0; q < SMALL_LV; ++q) {
-    ci_[q] = cj_[q] = 0;
-    cw[q] = ce[q] = cs[q] = cn[q] = cd[q] = rr[q] = (T)0;
-    if (q >= nl) continue;
+    if (q >= nl) break;
     const Level<T> F = A.lv[q];
-    const int nx = F.nx, ny = F.ny;
-    if (q == nl - 1) nb = nx * ny;
-    if (t < nx * ny) {
-      const int j = t / nx, i = t - j * nx;
-      ci_[q] = i;
-      cj_[q] = j;
-      const T w = F.bw[t], e = i + 1 < nx ? F.bw[t + 1] : (T)0;
-      const T so = F.bs[t], no = j + 1 < ny ? F.bs[t + nx] : (T)0;
-      cd[q] = w + e + so + no;
-      cw[q] = i > 0 ? w : (T)0;
-      ce[q] = e;
-      cs[q] = j > 0 ? so : (T)0;
-      cn[q] = no;
+    const int nx = F.nx, ny = F.ny, n = nx * ny;
+    if (q == nl - 1) nb = n;
+#pragma unroll
+    for (int u = 0; u < small_cpt(q); ++u) {
+      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS;
+      ci_[sl] = cj_[sl] = 0;
+      cw[sl] = ce[sl] = cs[sl] = cn[sl] = cd[sl] = id[sl] = rr[sl] = (T)0;
+      if (e >= n) continue;
+      const int j = e / nx, i = e - j * nx;
+      ci_[sl] = i;
+      cj_[sl] = j;
+      const T w = F.bw[e], ea = i + 1 < nx ? F.bw[e + 1] : (T)0;
+      const T so = F.bs[e], no = j + 1 < ny ? F.bs[e + nx] : (T)0;
+      cd[sl] = w + ea + so + no;
+      cw[sl] = i > 0 ? w : (T)0;
+      ce[sl] = ea;
+      cs[sl] = j > 0 ? so : (T)0;
+      cn[sl] = no;
       if (q == 0) {
-        rr[0] = F.r[t];
-        sm[L.r[0] + t] = rr[0];
+        rr[sl] = F.r[e];
+        if (nl == 1) sm[L.r[0] + e] = rr[sl];
       }
       if (q + 1 < nl) {
-        sm[L.wx[q] + t] = F.wx[t];
-        sm[L.wy[q] + t] = F.wy[t];
+        sm[L.wx[q] + e] = F.wx[e];
+        sm[L.wy[q] + e] = F.wy[e];
       }
-      sm[L.x[q] + t] = (T)0;
+      sm[L.x[q] + e] = (T)0;
     }
   }
   if (nb <= BOTTOM_LDS_MAX)
     for (int e = t; e < nb * nb; e += SMALL_THREADS) sm[L.binv + e] = A.binv[e];
+#pragma unroll
+  for (int sl = 0; sl < SMALL_SLOTS; ++sl) id[sl] = cd[sl] != (T)0 ? (T)1 / cd[sl] : (T)0;
   __syncthreads();
+  // one red-black half-sweep of level q (cells (i + j) & 1 == color)
+  auto half = [&](int q, int color, bool zero) {
+    const int nx = A.lv[q].nx, ny = A.lv[q].ny, n = nx * ny;
+    T* X = sm + L.x[q];
+#pragma unroll
+    for (int u = 0; u < small_cpt(q); ++u) {
+      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS, i = ci_[sl], j = cj_[sl];
+      if (e < n && ((i + j) & 1) == color) {
+        T o = 0;
+        if (!zero)
+          o = cw[sl] * X[i > 0 ? e - 1 : e] + ce[sl] * X[i + 1 < nx ? e + 1 : e] + cs[sl] * X[j > 0 ? e - nx : e] +
+              cn[sl] * X[j + 1 < ny ? e + nx : e];
+        X[e] = (rr[sl] + o) * id[sl];
+      }
+    }
+    __syncthreads();
+  };
   // pre-smoothing, residual and restriction, level by level
 #pragma unroll
   for (int q = 0; q + 1 < SMALL_LV; ++q) {
     if (q + 1 >= nl) break;
     const int nx = A.lv[q].nx, ny = A.lv[q].ny, n = nx * ny, cnx = nx / 2, cny = ny / 2;
     T* X = sm + L.x[q];
-    const T id = cd[q] != (T)0 ? (T)1 / cd[q] : (T)0;
-    if (q > 0 && t < n) rr[q] = sm[L.r[q] + t];
+    if (q > 0 && t < n) rr[small_slot(q, 0)] = sm[L.r[q] + t];
     for (int sw = 0; sw < nu_pre; ++sw) {
-      small_half<T>(X, t, n, nx, ny, ci_[q], cj_[q], cw[q], ce[q], cs[q], cn[q], id, rr[q], 0, sw == 0);
-      small_half<T>(X, t, n, nx, ny, ci_[q], cj_[q], cw[q], ce[q], cs[q], cn[q], id, rr[q], 1, false);
+      half(q, 0, sw == 0);
+      half(q, 1, false);
     }
     T* R = sm + L.tmp;
-    if (t < n) {
-      const int i = ci_[q], j = cj_[q];
-      const T o = cw[q] * X[i > 0 ? t - 1 : t] + ce[q] * X[i + 1 < nx ? t + 1 : t] +
-                  cs[q] * X[j > 0 ? t - nx : t] + cn[q] * X[j + 1 < ny ? t + nx : t];
-      R[t] = rr[q] - (cd[q] * X[t] - o);
+#pragma unroll
+    for (int u = 0; u < small_cpt(q); ++u) {
+      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS, i = ci_[sl], j = cj_[sl];
+      if (e >= n) continue;
+      const T o = cw[sl] * X[i > 0 ? e - 1 : e] + ce[sl] * X[i + 1 < nx ? e + 1 : e] +
+                  cs[sl] * X[j > 0 ? e - nx : e] + cn[sl] * X[j + 1 < ny ? e + nx : e];
+      R[e] = rr[sl] - (cd[sl] * X[e] - o);
     }
     __syncthreads();
     // restriction r_H = P^T res in two passes: per fine row and coarse column
@@ -889,8 +908,8 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     // coarse cell the rows 2J-1 (tq), 2J, 2J+1 (tp), 2J+2 (tq)
     const T* WX = sm + L.wx[q];
     const T* WY = sm + L.wy[q];
-    if (t < ny * cnx) {
-      const int jj = t / cnx, I = t - jj * cnx;
+    for (int e = t; e < ny * cnx; e += SMALL_THREADS) {
+      const int jj = e / cnx, I = e - jj * cnx;
       T pa = 0, pb = 0, qa = 0, qb = 0;
 #pragma unroll
       for (int d = -1; d <= 2; ++d) {
@@ -899,26 +918,25 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
         const int k = i + jj * nx;
         const T res = R[k], wy = WY[k], wx = WX[k];
         const T a = res * wy, b = res * ((T)1 - wy);
-        const T fx = (d == 0 || d == 1) ? wx : (T)1 - wx;
         if (d == 0 || d == 1) {
-          pa += fx * a;
-          pb += fx * b;
+          pa += wx * a;
+          pb += wx * b;
         } else {
-          qa += fx * a;
-          qb += fx * b;
+          qa += ((T)1 - wx) * a;
+          qb += ((T)1 - wx) * b;
         }
       }
-      sm[L.tp + t] = pa + qa;
-      sm[L.tq + t] = pb + qb;
+      sm[L.tp + e] = pa + qa;
+      sm[L.tq + e] = pb + qb;
     }
     __syncthreads();
-    if (t < cnx * cny) {
-      const int J = t / cnx, I = t - J * cnx;
+    for (int e = t; e < cnx * cny; e += SMALL_THREADS) {
+      const int J = e / cnx, I = e - J * cnx;
       const T* TP = sm + L.tp;
       const T* TQ = sm + L.tq;
       const T lo = 2 * J - 1 >= 0 ? TQ[(2 * J - 1) * cnx + I] : (T)0;
       const T hi = 2 * J + 2 < ny ? TQ[(2 * J + 2) * cnx + I] : (T)0;
-      sm[L.r[q + 1] + t] = (lo + TP[2 * J * cnx + I]) + (TP[(2 * J + 1) * cnx + I] + hi);
+      sm[L.r[q + 1] + e] = (lo + TP[2 * J * cnx + I]) + (TP[(2 * J + 1) * cnx + I] + hi);
     }
     __syncthreads();
   }
@@ -940,28 +958,29 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 #pragma unroll
   for (int q = SMALL_LV - 2; q >= 0; --q) {
     if (q + 1 >= nl) continue;
-    const int nx = A.lv[q].nx, ny = A.lv[q].ny, n = nx * ny, cnx = nx / 2, cny = ny / 2;
+    const int nx = A.lv[q].nx, n = nx * A.lv[q].ny, cnx = nx / 2, cny = A.lv[q].ny / 2;
     T* X = sm + L.x[q];
     const T* xc = sm + L.x[q + 1];
-    if (t < n) {
-      const int i = ci_[q], j = cj_[q];
-      const T wx = sm[L.wx[q] + t], wy = sm[L.wy[q] + t];
+#pragma unroll
+    for (int u = 0; u < small_cpt(q); ++u) {
+      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS, i = ci_[sl], j = cj_[sl];
+      if (e >= n) continue;
+      const T wx = sm[L.wx[q] + e], wy = sm[L.wy[q] + e];
       const int I = i >> 1, J = j >> 1;
       int I2 = I + ((i & 1) ? 1 : -1), J2 = J + ((j & 1) ? 1 : -1);
       I2 = I2 < 0 ? 0 : (I2 > cnx - 1 ? cnx - 1 : I2);
       J2 = J2 < 0 ? 0 : (J2 > cny - 1 ? cny - 1 : J2);
-      X[t] += wx * wy * xc[I + J * cnx] + ((T)1 - wx) * wy * xc[I2 + J * cnx] + wx * ((T)1 - wy) * xc[I + J2 * cnx] +
+      X[e] += wx * wy * xc[I + J * cnx] + ((T)1 - wx) * wy * xc[I2 + J * cnx] + wx * ((T)1 - wy) * xc[I + J2 * cnx] +
               ((T)1 - wx) * ((T)1 - wy) * xc[I2 + J2 * cnx];
     }
     __syncthreads();
-    const T id = cd[q] != (T)0 ? (T)1 / cd[q] : (T)0;
     for (int sw = 0; sw < nu_post; ++sw) {
-      small_half<T>(X, t, n, nx, ny, ci_[q], cj_[q], cw[q], ce[q], cs[q], cn[q], id, rr[q], 1, false);
-      small_half<T>(X, t, n, nx, ny, ci_[q], cj_[q], cw[q], ce[q], cs[q], cn[q], id, rr[q], 0, false);
+      half(q, 1, false);
+      half(q, 0, false);
     }
   }
   const int n0 = A.lv[0].nx * A.lv[0].ny;
-  if (t < n0) A.lv[0].xb[t] = sm[L.x[0] + t];
+  for (int e = t; e < n0; e += SMALL_THREADS) A.lv[0].xb[e] = sm[L.x[0] + e];
 }
 
 // --------------------------------------------------------------------------- Krylov direction kernel
