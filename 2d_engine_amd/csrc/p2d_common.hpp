@@ -94,8 +94,8 @@ struct Scal {
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
-  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, true);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 // Wavefront sum / max (of non-negative values: the empty lanes enter as 0),

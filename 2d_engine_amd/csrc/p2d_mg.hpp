@@ -325,8 +325,8 @@ struct alignas(2 * sizeof(T)) Pair {
 
 // The value held by the previous / next lane of the wavefront (0 for lane 0 /
 // lane 63): DPP wave_shr:1 / wave_shl:1 with bound_ctrl.
-__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true); }
-__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
 __device__ __forceinline__ float lane_prev(float v) { return __int_as_float(dpp_prev(__float_as_int(v))); }
 __device__ __forceinline__ float lane_next(float v) { return __int_as_float(dpp_next(__float_as_int(v))); }
 __device__ __forceinline__ double lane_prev(double v) {
@@ -338,6 +338,19 @@ __device__ __forceinline__ double lane_next(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
   const unsigned lo = (unsigned)dpp_next((int)(unsigned)b), hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Tile kernels address their level's arrays with 32-bit byte offsets (a level
+// holds fewer than 2^32 bytes per array; p2d_create checks it), so a load or
+// store takes the array's base in scalar registers and one VGPR offset.
+__device__ __forceinline__ unsigned cell_u(int i, int j, int nx) { return (unsigned)i + (unsigned)j * (unsigned)nx; }
+template <class V, class T>
+__device__ __forceinline__ V ld_at(const T* base, unsigned cell) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + cell * (unsigned)sizeof(T));
+}
+template <class V, class T>
+__device__ __forceinline__ void st_at(T* base, unsigned cell, V v) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + cell * (unsigned)sizeof(T)) = v;
 }
 
 // Per row m of the strip: right-hand side, faces (wA west of A, wB west of B
@@ -365,14 +378,14 @@ __device__ __forceinline__ void strip_stage(Strip<T>& S, const Level<T>& F, int 
     S.v[m] = ld && r < RY;
     T wA = 0, wB = 0, eB = 0, sA = 0, sB = 0;
     if (ld) {
-      const size_t k = ci(giA, gj, nx);
-      const Pair<T> w2 = reinterpret_cast<const Pair<T>*>(F.bw)[k >> 1];
-      const Pair<T> s2 = reinterpret_cast<const Pair<T>*>(F.bs)[k >> 1];
+      const unsigned k = cell_u(giA, gj, nx);
+      const Pair<T> w2 = ld_at<Pair<T>>(F.bw, k);
+      const Pair<T> s2 = ld_at<Pair<T>>(F.bs, k);
       wA = w2.a;
       wB = w2.b;
       sA = s2.a;
       sB = s2.b;
-      if (giA + 2 < nx) eB = F.bw[k + 2];
+      if (giA + 2 < nx) eB = ld_at<T>(F.bw, k + 2);
     }
     S.wA[m] = wA;
     S.wB[m] = wB;
@@ -386,7 +399,7 @@ __device__ __forceinline__ void strip_stage(Strip<T>& S, const Level<T>& F, int 
     const int r = KR * w + KR - 1, gj = gy0 + r + 1;
     T a = 0, b = 0;
     if (col_in && r + 1 <= RY && gj >= 0 && gj < ny) {
-      const Pair<T> n2 = reinterpret_cast<const Pair<T>*>(F.bs)[ci(giA, gj, nx) >> 1];
+      const Pair<T> n2 = ld_at<Pair<T>>(F.bs, cell_u(giA, gj, nx));
       a = n2.a;
       b = n2.b;
     }
@@ -426,7 +439,7 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
   if (!zero) strip_halo<T>(bnd + ((k + 1) & 1) * BND, c, w, dn, up);
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
-    const bool doA = ((gy0 + KR * w + m) & 1) == color;
+    const bool doA = ((gy0 + KR * w + m) & 1) == color;  // the same for the whole wavefront
     const T xl = lane_prev(S.xB[m]), xr = lane_next(S.xA[m]);
     const T dA = m > 0 ? S.xA[m > 0 ? m - 1 : 0] : dn.a, dB = m > 0 ? S.xB[m > 0 ? m - 1 : 0] : dn.b;
     const T uA = m + 1 < KR ? S.xA[m + 1 < KR ? m + 1 : m] : up.a;
@@ -497,9 +510,9 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
       const int r = KR * w + m;
       wxA[m] = wxB[m] = wyA[m] = wyB[m] = 0;
       if (S.v[m] && pair_fp && r >= H - 1 && r <= H + TY) {
-        const size_t k = ci(giA, gy0 + r, nx);
-        const Pair<T> wx2 = reinterpret_cast<const Pair<T>*>(F.wx)[k >> 1];
-        const Pair<T> wy2 = reinterpret_cast<const Pair<T>*>(F.wy)[k >> 1];
+        const unsigned k = cell_u(giA, gy0 + r, nx);
+        const Pair<T> wx2 = ld_at<Pair<T>>(F.wx, k);
+        const Pair<T> wy2 = ld_at<Pair<T>>(F.wy, k);
         wxA[m] = wx2.a;
         wxB[m] = wx2.b;
         wyA[m] = wy2.a;
@@ -513,27 +526,27 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
     const int r = KR * w + m;
     T ra = 0, rb = 0;
     if (S.v[m]) {
-      const size_t k = ci(giA, gy0 + r, nx);
-      const Pair<T> r2 = reinterpret_cast<const Pair<T>*>(rin)[k >> 1];
+      const unsigned k = cell_u(giA, gy0 + r, nx);
+      const Pair<T> r2 = ld_at<Pair<T>>(rin, k);
       ra = r2.a;
       rb = r2.b;
       if (FUSE) {
         // the inner recursion runs in T: r -= a q, x += a p (a, shift rounded to T)
         if (pend) {
-          const Pair<T> q2 = reinterpret_cast<const Pair<T>*>(qv)[k >> 1];
+          const Pair<T> q2 = ld_at<Pair<T>>(qv, k);
           ra -= ta * q2.a;
           rb -= ta * q2.b;
         }
         ra -= tshift;
         rb -= tshift;
         if (pair_int && r >= H && r < H + TY) {
-          reinterpret_cast<Pair<T>*>(rout)[k >> 1] = Pair<T>{ra, rb};
+          st_at(rout, k, Pair<T>{ra, rb});
           if (pend) {
-            Pair<T> x2 = reinterpret_cast<Pair<T>*>(xsol)[k >> 1];
-            const Pair<T> p2 = reinterpret_cast<const Pair<T>*>(pdir)[k >> 1];
+            Pair<T> x2 = ld_at<Pair<T>>(xsol, k);
+            const Pair<T> p2 = ld_at<Pair<T>>(pdir, k);
             x2.a += ta * p2.a;
             x2.b += ta * p2.b;
-            reinterpret_cast<Pair<T>*>(xsol)[k >> 1] = x2;
+            st_at(xsol, k, x2);
           }
         }
       }
@@ -564,7 +577,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
       resB[m] = S.rB[m] - ((S.wB[m] + S.eB[m] + S.sB[m] + S.nB(m)) * S.xB[m] -
                            (S.wB[m] * S.xA[m] + S.eB[m] * xr + S.sB[m] * dB + S.nB(m) * uB));
       if (pair_int && r >= H && r < H + TY)
-        reinterpret_cast<Pair<T>*>(x_lvl)[ci(giA, gy0 + r, nx) >> 1] = Pair<T>{S.xA[m], S.xB[m]};
+        st_at(x_lvl, cell_u(giA, gy0 + r, nx), Pair<T>{S.xA[m], S.xB[m]});
     }
   }
   __syncthreads();  // the boundary buffers are reused below for the row sums us[2][RYMAX][NCXMAX]
@@ -594,8 +607,8 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
     const int a = t % NCX, b = t / NCX, I = cx0 + a, J = cy0 + b;
     if (I >= Cc.nx || J >= Cc.ny) continue;
     const int jr = 2 * b + H;  // region row of fine row 2J
-    Cc.r[ci(I, J, Cc.nx)] = (us[(RYMAX + jr - 1) * NCXMAX + a] + us[jr * NCXMAX + a]) +
-                            (us[(jr + 1) * NCXMAX + a] + us[(RYMAX + jr + 2) * NCXMAX + a]);
+    st_at(Cc.r, cell_u(I, J, Cc.nx), (us[(RYMAX + jr - 1) * NCXMAX + a] + us[jr * NCXMAX + a]) +
+                            (us[(jr + 1) * NCXMAX + a] + us[(RYMAX + jr + 2) * NCXMAX + a]));
   }
   if (FUSE) {
     // norms of the updated residual over the tile interior (S.r holds it)
@@ -669,7 +682,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
   const T* rin = LEVEL0 ? (s->rpar ? rB : rA) : r_lvl;
   for (int t = threadIdx.x; t < PCH * PCW; t += TBLOCK) {
     const int I = min(max(cxa + t % PCW, 0), cnx - 1), J = min(max(cya + t / PCW, 0), cny - 1);
-    xcs[t / PCW][t % PCW] = Cc.xb[ci(I, J, cnx)];
+    xcs[t / PCW][t % PCW] = ld_at<T>(Cc.xb, cell_u(I, J, cnx));
   }
   Strip<T> S;
   strip_stage<T>(S, F, giA, gy0, w, RY);
@@ -678,11 +691,11 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
   for (int m = 0; m < KR; ++m) {
     Pair<T> rr{0, 0}, x0{0, 0}, wx2{0, 0}, wy2{0, 0};
     if (S.v[m]) {
-      const size_t k = ci(giA, gy0 + KR * w + m, nx);
-      rr = reinterpret_cast<const Pair<T>*>(rin)[k >> 1];
-      x0 = reinterpret_cast<const Pair<T>*>(x_in)[k >> 1];
-      wx2 = reinterpret_cast<const Pair<T>*>(F.wx)[k >> 1];
-      wy2 = reinterpret_cast<const Pair<T>*>(F.wy)[k >> 1];
+      const unsigned k = cell_u(giA, gy0 + KR * w + m, nx);
+      rr = ld_at<Pair<T>>(rin, k);
+      x0 = ld_at<Pair<T>>(x_in, k);
+      wx2 = ld_at<Pair<T>>(F.wx, k);
+      wy2 = ld_at<Pair<T>>(F.wy, k);
     }
     S.rA[m] = rr.a;
     S.rB[m] = rr.b;
@@ -721,7 +734,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
   for (int m = 0; m < KR; ++m) {
     const int r = KR * w + m;
     if (!S.v[m] || !pair_int || r < H || r >= H + TY) continue;
-    reinterpret_cast<Pair<T>*>(x_out)[ci(giA, gy0 + r, nx) >> 1] = Pair<T>{S.xA[m], S.xB[m]};
+    st_at(x_out, cell_u(giA, gy0 + r, nx), Pair<T>{S.xA[m], S.xB[m]});
     if (LEVEL0) {
       acc[0] += (double)S.rA[m] * (double)S.xA[m] + (double)S.rB[m] * (double)S.xB[m];
       acc[1] += (double)S.xA[m] + (double)S.xB[m];
