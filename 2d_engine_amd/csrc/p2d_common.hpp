@@ -159,6 +159,32 @@ __device__ __forceinline__ void block_partial(double (&v)[K], unsigned maxmask, 
   }
 }
 
+// The same for large blocks with LDS to spare: every thread stores its values
+// in sh (K * NT doubles, free for the caller's use again after the call), the
+// first wavefront sums the NT / 64 values of each lane in thread order and
+// reduces across lanes.  One cross-lane reduction per block instead of one
+// per wavefront.
+template <int K, int NT>
+__device__ __forceinline__ void block_partial_lds(double (&v)[K], unsigned maxmask, double* partial, double* sh) {
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) sh[k * NT + threadIdx.x] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 64) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const bool mx = (maxmask >> k) & 1;
+      double a = sh[k * NT + threadIdx.x];
+      for (int q = 1; q < NT / 64; ++q) {
+        const double b = sh[k * NT + q * 64 + threadIdx.x];
+        a = mx ? fmax(a, b) : a + b;
+      }
+      a = mx ? wave_max(a) : wave_sum(a);
+      if (threadIdx.x == 0) partial[(size_t)k * nblocks() + block_id()] = a;
+    }
+  }
+}
+
 // In a FINISH_THREADS-thread block: totals of G partials into thread 0's a.
 // Maxima are of non-negative quantities (start at 0).
 template <int K>

@@ -314,9 +314,10 @@ __global__ void k_inner_continue(Scal* s, double ncells) {
 // updated.
 constexpr int KR = 3;                                // consecutive region rows per thread
 constexpr int BND = TWAVES * 2 * 64;                 // one boundary buffer: [wave][bottom, top][lane]
-constexpr int TILE_LDS_PAIRS = 2 * BND;              // two buffers (the row sums of the restriction reuse them)
+constexpr int TILE_LDS_PAIRS = 2 * BND;              // two buffers (the restriction's row sums and the block reduction reuse them)
 static_assert(KR * TWAVES >= RYMAX, "a tile region must fit the strips");
 static_assert(2 * RYMAX * NCXMAX <= 2 * TILE_LDS_PAIRS, "restriction row sums must fit the boundary buffers");
+static_assert(3 * TBLOCK * sizeof(double) <= TILE_LDS_PAIRS * 2 * sizeof(float), "block reduction must fit them");
 
 template <class T>
 struct alignas(2 * sizeof(T)) Pair {
@@ -409,12 +410,16 @@ __device__ __forceinline__ void strip_stage(Strip<T>& S, const Level<T>& F, int 
 }
 // 1/diag of the strip's cells (after strip_stage; kept out of it so that the
 // post kernel forms them only once the prolongation's registers are free).
+// FP32 uses the hardware reciprocal (1 ulp): the smoother only needs a close
+// 1/diag, and the IEEE division is ten instructions per cell.
+__device__ __forceinline__ float smoother_inv(float d) { return __builtin_amdgcn_rcpf(d); }
+__device__ __forceinline__ double smoother_inv(double d) { return 1.0 / d; }
 template <class T>
 __device__ __forceinline__ void strip_diag(Strip<T>& S) {
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
-    S.iA[m] = S.v[m] ? (T)1 / (S.wA[m] + S.wB[m] + S.sA[m] + S.nA(m)) : (T)0;
-    S.iB[m] = S.v[m] ? (T)1 / (S.wB[m] + S.eB[m] + S.sB[m] + S.nB(m)) : (T)0;
+    S.iA[m] = S.v[m] ? smoother_inv(S.wA[m] + S.wB[m] + S.sA[m] + S.nA(m)) : (T)0;
+    S.iB[m] = S.v[m] ? smoother_inv(S.wB[m] + S.eB[m] + S.sB[m] + S.nB(m)) : (T)0;
   }
 }
 
@@ -589,10 +594,11 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
     const int r = KR * w + m;
     // a*: part of res for the parent row (weight wy), b*: for the neighbouring row (1 - wy);
     // p*: part for the parent column (wx), q*: for the neighbouring column (1 - wx).
-    const T aA = resA[m] * wyA[m], bA = resA[m] * ((T)1 - wyA[m]);
-    const T aB = resB[m] * wyB[m], bB = resB[m] * ((T)1 - wyB[m]);
-    const T paA = wxA[m] * aA, qaA = ((T)1 - wxA[m]) * aA, pbA = wxA[m] * bA, qbA = ((T)1 - wxA[m]) * bA;
-    const T paB = wxB[m] * aB, qaB = ((T)1 - wxB[m]) * aB, pbB = wxB[m] * bB, qbB = ((T)1 - wxB[m]) * bB;
+    // (the (1 - w) shares as differences: res = a + b, a = pa + qa, b = pb + qb)
+    const T aA = resA[m] * wyA[m], bA = resA[m] - aA;
+    const T aB = resB[m] * wyB[m], bB = resB[m] - aB;
+    const T paA = wxA[m] * aA, qaA = aA - paA, pbA = wxA[m] * bA, qbA = bA - pbA;
+    const T paB = wxB[m] * aB, qaB = aB - paB, pbB = wxB[m] * bB, qbB = bB - pbB;
     // coarse column I (this lane) gathers q(2I-1) + p(2I) + p(2I+1) + q(2I+2)
     const T sa = (paA + paB) + (lane_prev(qaB) + lane_next(qaA));
     const T sb = (pbA + pbB) + (lane_prev(qbB) + lane_next(qbA));
@@ -622,7 +628,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
       acc[1] = fmax(acc[1], fmax(fabs(ra), fabs(rb)));
       acc[2] += ra + rb;
     }
-    block_partial<3, TBLOCK>(acc, 2u, partial);
+    block_partial_lds<3, TBLOCK>(acc, 2u, partial, reinterpret_cast<double*>(lds));
   }
 }
 
@@ -741,7 +747,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
       acc[2] += (double)S.rA[m] + (double)S.rB[m];
     }
   }
-  if (LEVEL0) block_partial<3, TBLOCK>(acc, 0u, partial);
+  if (LEVEL0) block_partial_lds<3, TBLOCK>(acc, 0u, partial, reinterpret_cast<double*>(lds));
 }
 
 // Finish of the finest post kernel: zmean, rz = r.(z - zmean), beta.
