@@ -75,6 +75,8 @@ def lib():
             "orc_adjoint": (_i, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _d, _pd, _pd, _pd, _pd, _pd]),
             "orc_project_beta": (_i, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _d, _pd]),
             "orc_num_threads": (_i, []),
+            "orc_weno5_derivatives": (None, [_i, _i, _d, _pd, _d, _pd, _pd, _pd, _pd]),
+            "orc_reinit_sussman": (None, [_i, _i, _d, _pd, _i, _d, _d, _pd]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -133,6 +135,24 @@ def normals(phi, h):
     a, b = np.empty_like(phi), np.empty_like(phi)
     lib().orc_normals(nx, ny, h, _p(phi), _p(a), _p(b))
     return a, b
+
+
+def weno5_derivatives(phi, h, epsilon=1e-6):
+    """phi_x^-, phi_x^+, phi_y^-, phi_y^+ of the re-initialisation step (reference ghost rule)."""
+    phi = _f64(phi)
+    ny, nx = phi.shape
+    out = [np.empty_like(phi) for _ in range(4)]
+    lib().orc_weno5_derivatives(nx, ny, h, _p(phi), epsilon, *[_p(o) for o in out])
+    return tuple(out)
+
+
+def reinit_sussman(phi, h, iterations=25, scaling=5.0, epsilon=1e-6):
+    """Sussman WENO5 re-initialisation (sussmanstyle_reinitialization) of the level set."""
+    phi = _f64(phi)
+    ny, nx = phi.shape
+    out = np.empty_like(phi)
+    lib().orc_reinit_sussman(nx, ny, h, _p(phi), int(iterations), float(scaling), float(epsilon), _p(out))
+    return out
 
 
 def face_beta(phi, rho_minus, rho_plus):

@@ -216,9 +216,10 @@ void orc_normals(int nx, int ny, double h, const double* phi_in, double* nxo, do
         a = (phi(i + 1, j) - phi(i, j)) * one_over_dx;
         b = (phi(i, j + 1) - phi(i, j)) * one_over_dx;
       }
-      const double m = std::sqrt(a * a + b * b);
-      nxo[C(i, j, nx)] = m > 0 ? a / m : 0.0;
-      nyo[C(i, j, nx)] = m > 0 ? b / m : 0.0;
+      // VECTOR_2D::MakeThisUnit (VECTOR_2D.h:136-150): scaled by 1/|n|, (0,0) when |n| = 0
+      const double m = std::sqrt(a * a + b * b), inv = 1.0 / m;
+      nxo[C(i, j, nx)] = m != 0 ? a * inv : 0.0;
+      nyo[C(i, j, nx)] = m != 0 ? b * inv : 0.0;
     }
 }
 
@@ -1156,6 +1157,149 @@ int orc_project_beta(int nx, int ny, double h, double dt, const double* phi, con
   orc_update_consistent(nx, ny, h, dt, phi, jc, bx, by, x.data(), u, v, nullptr, nullptr);
   if (p_out) std::memcpy(p_out, x.data(), sizeof(double) * N);
   return it;
+}
+
+// ---------------------------------------------------------------------------
+// Level-set re-initialisation, the sussmanstyle_reinitialization branch of
+// EULERIAN_FLUID_SOLVER_2D::Reinitialization (EULERIAN_FLUID_SOLVER_2D.h:2365-2400):
+//   s = phi0 / sqrt(phi0^2 + dx^2) from the level set at entry (:2380-2383),
+//   fictitious dt = dx / scaling_number (:2374), then `iterations` calls of
+//   ADVECTION_2D::ReinitializationBySussman (ADVECTION_2D.cpp:415-421) =
+//   ADVECTION_METHOD_2D::WENO5thReinitialization (ADVECTION_METHOD_2D.h:805-1076).
+// One step reads a ghost copy g of phi (FIELD_STRUCTURE_2D::FillGhostCellsFrom,
+// FIELD_STRUCTURE_2D.h:528-582: all three ghosts of a face row take the second
+// interior value, e.g. g(-3..-1, j) = phi(1, j); corners are never read), forms
+// the WENO5 one-sided derivatives, applies the reference's Godunov branch
+// selection, updates phi -= dt s (H - 1), and clamps a sign flip to
+// +-epsilon.  The arithmetic keeps the reference's expression forms (e.g.
+// v*7/6 is (v*7)/6) so that a build without contraction reproduces it.
+namespace {
+struct Pad3 {
+  int nx, ny;
+  std::vector<double> a;
+  Pad3(int nx_, int ny_) : nx(nx_), ny(ny_), a((size_t)(nx_ + 6) * (ny_ + 6), 0.0) {}
+  double& operator()(int i, int j) { return a[(size_t)(i + 3) + (size_t)(j + 3) * (nx + 6)]; }
+  double operator()(int i, int j) const { return a[(size_t)(i + 3) + (size_t)(j + 3) * (nx + 6)]; }
+};
+Pad3 reinit_ghosts(int nx, int ny, const double* phi) {
+  Pad3 g(nx, ny);
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) g(i, j) = phi[C(i, j, nx)];
+  for (int j = 0; j < ny; ++j)
+    for (int k = 1; k <= 3; ++k) {
+      g(-k, j) = phi[C(1, j, nx)];
+      g(nx - 1 + k, j) = phi[C(nx - 2, j, nx)];
+    }
+  for (int i = 0; i < nx; ++i)
+    for (int k = 1; k <= 3; ++k) {
+      g(i, ny - 1 + k) = phi[C(i, ny - 2, nx)];
+      g(i, -k) = phi[C(i, 1, nx)];
+    }
+  return g;
+}
+// WENO5 derivative from the five one-sided differences v1..v5 (ADVECTION_METHOD_2D.h:844-929).
+double weno5(double v1, double v2, double v3, double v4, double v5, double epsilon) {
+  const double s1 = (double)13 / 12 * (v1 - 2 * v2 + v3) * (v1 - 2 * v2 + v3) +
+                    (double)1 / 4 * (v1 - 4 * v2 + 3 * v3) * (v1 - 4 * v2 + 3 * v3);
+  const double s2 = (double)13 / 12 * (v2 - 2 * v3 + v4) * (v2 - 2 * v3 + v4) + (double)1 / 4 * (v2 - v4) * (v2 - v4);
+  const double s3 = (double)13 / 12 * (v3 - 2 * v4 + v5) * (v3 - 2 * v4 + v5) +
+                    (double)1 / 4 * (3 * v3 - 4 * v4 + v5) * (3 * v3 - 4 * v4 + v5);
+  const double a1 = (double)1 / 10 * (double)1 / ((epsilon + s1) * (epsilon + s1));
+  const double a2 = (double)6 / 10 * (double)1 / ((epsilon + s2) * (epsilon + s2));
+  const double a3 = (double)3 / 10 * (double)1 / ((epsilon + s3) * (epsilon + s3));
+  const double sum = a1 + a2 + a3;
+  const double w1 = a1 / sum, w2 = a2 / sum, w3 = a3 / sum;
+  return w1 * (v1 * (double)1 / 3 - v2 * (double)7 / 6 + v3 * (double)11 / 6) +
+         w2 * (v2 * (-(double)1 / 6) + v3 * (double)5 / 6 + v4 * (double)1 / 3) +
+         w3 * (v3 * (double)1 / 3 + v4 * (double)5 / 6 - v5 * (double)1 / 6);
+}
+// phi_x^-, phi_x^+, phi_y^-, phi_y^+ at (i, j) of the ghost copy (ADVECTION_METHOD_2D.h:817-841).
+void weno5_derivs(const Pad3& g, int i, int j, double ih, double epsilon, double* d) {
+  const double mx1 = ih * (g(i - 2, j) - g(i - 3, j)), mx2 = ih * (g(i - 1, j) - g(i - 2, j)),
+               mx3 = ih * (g(i, j) - g(i - 1, j)), mx4 = ih * (g(i + 1, j) - g(i, j)),
+               mx5 = ih * (g(i + 2, j) - g(i + 1, j));
+  const double px1 = ih * (g(i + 3, j) - g(i + 2, j)), px2 = ih * (g(i + 2, j) - g(i + 1, j)),
+               px3 = ih * (g(i + 1, j) - g(i, j)), px4 = ih * (g(i, j) - g(i - 1, j)),
+               px5 = ih * (g(i - 1, j) - g(i - 2, j));
+  const double my1 = ih * (g(i, j - 2) - g(i, j - 3)), my2 = ih * (g(i, j - 1) - g(i, j - 2)),
+               my3 = ih * (g(i, j) - g(i, j - 1)), my4 = ih * (g(i, j + 1) - g(i, j)),
+               my5 = ih * (g(i, j + 2) - g(i, j + 1));
+  const double py1 = ih * (g(i, j + 3) - g(i, j + 2)), py2 = ih * (g(i, j + 2) - g(i, j + 1)),
+               py3 = ih * (g(i, j + 1) - g(i, j)), py4 = ih * (g(i, j) - g(i, j - 1)),
+               py5 = ih * (g(i, j - 1) - g(i, j - 2));
+  d[0] = weno5(mx1, mx2, mx3, mx4, mx5, epsilon);
+  d[1] = weno5(px1, px2, px3, px4, px5, epsilon);
+  d[2] = weno5(my1, my2, my3, my4, my5, epsilon);
+  d[3] = weno5(py1, py2, py3, py4, py5, epsilon);
+}
+// The reference's Godunov branch selection (ADVECTION_METHOD_2D.h:936-1060): the
+// Hamiltonian |grad phi| from the one-sided derivatives and the sign s; NaN
+// when no branch applies (then the reference leaves phi unchanged).
+double godunov_h(double s, double mxd, double pxd, double myd, double pyd) {
+  const double smx = s * mxd, spx = s * pxd, smy = s * myd, spy = s * pyd;
+  // the y part given the x derivative used (xd) and whether x enters at all
+  auto ypart = [&](bool hasx, double xd) -> double {
+    const double x2 = hasx ? xd * xd : 0.0;
+    if ((spy >= 0) && (smy >= 0)) return hasx ? std::sqrt(x2 + myd * myd) : std::sqrt(myd * myd);
+    if ((spy <= 0) && (smy <= 0)) return hasx ? std::sqrt(x2 + pyd * pyd) : std::sqrt(pyd * pyd);
+    if ((spy > 0) && (smy < 0)) return hasx ? std::sqrt(x2) : 0.0;
+    if ((spy < 0) && (smy > 0)) {
+      const double ss = s * (std::fabs(pyd) - std::fabs(myd)) / (pyd - myd);
+      if (ss > 0) return hasx ? std::sqrt(x2 + myd * myd) : std::sqrt(myd * myd);
+      return hasx ? std::sqrt(x2 + pyd * pyd) : std::sqrt(pyd * pyd);
+    }
+    return NAN;
+  };
+  if ((spx >= 0) && (smx >= 0)) return ypart(true, mxd);
+  if ((spx <= 0) && (smx <= 0)) return ypart(true, pxd);
+  if ((spx > 0) && (smx < 0)) return ypart(false, 0.0);
+  if ((spx < 0) && (smx > 0)) {
+    const double sss = s * (std::fabs(pxd) - std::fabs(mxd)) / (pxd - mxd);
+    return sss > 0 ? ypart(true, mxd) : ypart(true, pxd);
+  }
+  return NAN;
+}
+}  // namespace
+
+void orc_weno5_derivatives(int nx, int ny, double h, const double* phi, double epsilon, double* dxm, double* dxp,
+                           double* dym, double* dyp) {
+  const Pad3 g = reinit_ghosts(nx, ny, phi);
+  const double ih = 1.0 / h;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double d[4];
+      weno5_derivs(g, i, j, ih, epsilon, d);
+      const size_t k = C(i, j, nx);
+      dxm[k] = d[0];
+      dxp[k] = d[1];
+      dym[k] = d[2];
+      dyp[k] = d[3];
+    }
+}
+
+void orc_reinit_sussman(int nx, int ny, double h, const double* phi_in, int iterations, double scaling,
+                        double epsilon, double* phi_out) {
+  const size_t N = (size_t)nx * ny;
+  std::vector<double> sgn(N), phi(phi_in, phi_in + N);
+  for (size_t k = 0; k < N; ++k) sgn[k] = phi[k] / std::sqrt(phi[k] * phi[k] + h * h);
+  const double dt = h / scaling, ih = 1.0 / h;
+  for (int it = 0; it < iterations; ++it) {
+    const Pad3 g = reinit_ghosts(nx, ny, phi.data());
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const size_t k = C(i, j, nx);
+        double d[4];
+        weno5_derivs(g, i, j, ih, epsilon, d);
+        const double s = sgn[k];
+        const double H = godunov_h(s, d[0], d[1], d[2], d[3]);
+        if (H == H) phi[k] = phi[k] - dt * s * (H - 1);
+        if (s * phi[k] < 0) {
+          if (s < 0) phi[k] = -epsilon;
+          if (s > 0) phi[k] = epsilon;
+        }
+      }
+  }
+  std::copy(phi.begin(), phi.end(), phi_out);
 }
 
 int orc_num_threads(void) {
