@@ -33,12 +33,14 @@ KERNEL_CORRECT_RESID = 4
 KERNEL_CORRECT_RESID_F64 = 5
 KERNEL_PRE0_F64 = 6
 KERNEL_POST0_F64 = 7
+# p2d_time_kernel only: one level-set re-initialisation step
+TIME_REINIT_STEP = 5
 
 EXPORTS = [
     "p2d_config_default", "p2d_create", "p2d_destroy", "p2d_set_stream", "p2d_set_fluid",
     "p2d_set_coefficients", "p2d_project", "p2d_get_pressure", "p2d_get_coefficients", "p2d_adjoint",
     "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_kernel_timing",
-    "p2d_kernel_time", "p2d_status_string",
+    "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals",
 ]
 
 
@@ -94,6 +96,8 @@ def _load():
         "p2d_kernel_timing": (i, [vp, i, i]),
         "p2d_kernel_time": (i, [vp, i, ctypes.POINTER(d), ctypes.POINTER(i)]),
         "p2d_status_string": (ctypes.c_char_p, [i]),
+        "p2d_reinitialize": (i, [vp, vp, i, d, d, i]),
+        "p2d_levelset_normals": (i, [vp, vp, vp, vp, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -268,6 +272,31 @@ class Context:
         bz = _Buf(z, self.cell_shape, dtype=dt, writable=True)
         _check(lib.p2d_vcycle(self._h, br.ptr, bz.ptr, prec, _mem(br, bz)), "p2d_vcycle")
         return z
+
+    def reinitialize(self, phi, iterations=25, scaling=5.0, epsilon=1e-6):
+        """Sussman WENO5 re-initialisation of phi (EULERIAN_FLUID_SOLVER_2D.h:2365-2400).
+
+        A torch CUDA tensor is updated in place and returned; a numpy array gives a new array."""
+        if _is_torch(phi):
+            b = _Buf(phi, self.cell_shape, writable=True)
+            out = phi
+        else:
+            out = np.array(phi, dtype=np.float64, order="C", copy=True)
+            b = _Buf(out, self.cell_shape, writable=True)
+        _check(lib.p2d_reinitialize(self._h, b.ptr, int(iterations), float(scaling), float(epsilon), b.mem),
+               "p2d_reinitialize")
+        return out
+
+    def normals(self, phi):
+        """Unit normals (nx, ny) of the level set phi (LEVELSET_2D.h:593-620)."""
+        bp = _Buf(phi, self.cell_shape)
+        if bp.mem == MEM_HOST:
+            a, c = np.empty(self.cell_shape), np.empty(self.cell_shape)
+        else:
+            a, c = phi.new_empty(phi.shape), phi.new_empty(phi.shape)
+        ba, bc = _Buf(a, self.cell_shape, writable=True), _Buf(c, self.cell_shape, writable=True)
+        _check(lib.p2d_levelset_normals(self._h, bp.ptr, ba.ptr, bc.ptr, _mem(bp, ba, bc)), "p2d_levelset_normals")
+        return a, c
 
     @property
     def mg_levels(self):

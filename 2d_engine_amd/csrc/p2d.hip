@@ -27,6 +27,7 @@
 #include "p2d_common.hpp"
 #include "p2d_field.hpp"
 #include "p2d_mg.hpp"
+#include "p2d_levelset.hpp"
 
 using namespace p2d;
 
@@ -970,6 +971,39 @@ int p2d_vcycle(p2d_ctx* c, const void* r, void* z, int prec, int mem) {
 
 int p2d_mg_levels(p2d_ctx* c) { return c ? c->mg64.L : -1; }
 
+// Level-set re-initialisation and normals on the context's grid.  They use the
+// FP64 Krylov vectors x, z, q of the context as scratch (free between solves:
+// the pressure and the coefficients are kept elsewhere).
+int p2d_reinitialize(p2d_ctx* c, double* phi, int iterations, double scaling, double epsilon, int mem) {
+  if (!c || !phi || iterations < 0 || !(scaling > 0) || !(epsilon >= 0) || c->nx < 3 || c->ny < 3)
+    return P2D_BAD_ARG;
+  double* a = c->v64.x;
+  double* b = c->v64.z;
+  double* sg = c->v64.q;
+  CHK(stage_in(c, a, phi, c->N, mem));
+  k_reinit_sign<<<blocks_for(c->N), BLOCK, 0, c->stream>>>(c->N, c->h, a, sg);
+  const double dt = c->h / scaling;
+  for (int it = 0; it < iterations; ++it) {
+    k_reinit_step<<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, epsilon, a, sg, b);
+    std::swap(a, b);
+  }
+  CHK(launched_ok());
+  CHK(stage_out(c, phi, a, c->N, mem));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return P2D_OK;
+}
+
+int p2d_levelset_normals(p2d_ctx* c, const double* phi, double* normal_x, double* normal_y, int mem) {
+  if (!c || !phi || !normal_x || !normal_y || c->nx < 2 || c->ny < 2) return P2D_BAD_ARG;
+  CHK(stage_in(c, c->v64.x, phi, c->N, mem));
+  k_normals<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, c->v64.x, c->v64.z, c->v64.q);
+  CHK(launched_ok());
+  CHK(stage_out(c, normal_x, c->v64.z, c->N, mem));
+  CHK(stage_out(c, normal_y, c->v64.q, c->N, mem));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return P2D_OK;
+}
+
 int p2d_kernel_timing(p2d_ctx* c, int which, int enable) {
   if (!c || which < 0 || which >= KT_COUNT) return P2D_BAD_ARG;
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -992,7 +1026,7 @@ int p2d_kernel_time(p2d_ctx* c, int which, double* total_ms, int* launches) {
 }
 
 int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
-  if (!c || !avg_ms || reps < 1 || which < 0 || which > 4 || !c->coeff_ready) return P2D_BAD_ARG;
+  if (!c || !avg_ms || reps < 1 || which < 0 || which > 5 || !c->coeff_ready) return P2D_BAD_ARG;
   const size_t N = c->N;
   CHK(ensure_mg(c, P2D_PREC_F32));
   CHK(ensure_mg(c, P2D_PREC_F64));
@@ -1033,6 +1067,10 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
             L32[0], L32[1], nullptr, c->v32.r[0], c->v32.r[1], L32[0].xa, c->v32.z, nu2, c->partial,
             c->s);
         break;
+      case 5:  // one re-initialisation step of the fluid's level set
+        k_reinit_step<<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, c->h / 5.0, 1e-6, c->phi,
+                                                                    c->v64.q, c->v64.z);
+        break;
       default:
         k_correct_resid<float, float><<<fgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
             c->nx, c->ny, c->bw, c->bs, nullptr, nullptr, c->b, c->ph[0], c->pl[0], c->v32.x, c->ph[1], c->pl[1],
@@ -1040,6 +1078,7 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
     }
   };
   k_scal_arm<<<1, 1, 0, c->stream>>>(c->s, 1, 0.25, 1e-3, 1);
+  if (which == 5) k_reinit_sign<<<blocks_for(N), BLOCK, 0, c->stream>>>(N, c->h, c->phi, c->v64.q);
   launch();  // warm
   HIPCHK(hipEventRecord(e0, c->stream));
   for (int k = 0; k < reps; ++k) launch();

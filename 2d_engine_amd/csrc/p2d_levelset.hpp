@@ -1,0 +1,157 @@
+// p2d level-set kernels: the Sussman re-initialisation (WENO5 one-sided
+// derivatives + Godunov Hamiltonian) and unit normals.  FP64 with
+// contraction off: the arithmetic keeps the reference's operation order
+// (ADVECTION_METHOD_2D.h, LEVELSET_2D.h), so the results are reproducible
+// operation by operation on the CPU.
+#pragma once
+#include "p2d_common.hpp"
+
+namespace p2d {
+
+constexpr int RI_W = 64, RI_H = 16, RI_G = 3;  // re-initialisation tile: 64 x 16 cells, 3-cell ghost ring
+
+// Sign function of the re-initialisation, s = phi0 / sqrt(phi0^2 + dx^2)
+// (EULERIAN_FLUID_SOLVER_2D.h:2380-2383), from the level set at entry.
+__global__ void __launch_bounds__(BLOCK) k_reinit_sign(size_t N, double h, const double* __restrict__ phi,
+                                                       double* __restrict__ s) {
+#pragma clang fp contract(off)
+  for (size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x; k < N; k += (size_t)gridDim.x * BLOCK)
+    s[k] = phi[k] / sqrt(phi[k] * phi[k] + h * h);
+}
+
+// WENO5 derivative from the five one-sided differences v1..v5: smoothness
+// indicators, nonlinear weights with epsilon, the three candidate stencils
+// (ADVECTION_METHOD_2D.h:844-929, in its expression forms).
+__device__ __forceinline__ double weno5_d(double v1, double v2, double v3, double v4, double v5, double eps) {
+#pragma clang fp contract(off)
+  const double s1 = (double)13 / 12 * (v1 - 2 * v2 + v3) * (v1 - 2 * v2 + v3) +
+                    (double)1 / 4 * (v1 - 4 * v2 + 3 * v3) * (v1 - 4 * v2 + 3 * v3);
+  const double s2 = (double)13 / 12 * (v2 - 2 * v3 + v4) * (v2 - 2 * v3 + v4) + (double)1 / 4 * (v2 - v4) * (v2 - v4);
+  const double s3 = (double)13 / 12 * (v3 - 2 * v4 + v5) * (v3 - 2 * v4 + v5) +
+                    (double)1 / 4 * (3 * v3 - 4 * v4 + v5) * (3 * v3 - 4 * v4 + v5);
+  const double a1 = (double)1 / 10 * (double)1 / ((eps + s1) * (eps + s1));
+  const double a2 = (double)6 / 10 * (double)1 / ((eps + s2) * (eps + s2));
+  const double a3 = (double)3 / 10 * (double)1 / ((eps + s3) * (eps + s3));
+  const double sum = a1 + a2 + a3;
+  const double w1 = a1 / sum, w2 = a2 / sum, w3 = a3 / sum;
+  return w1 * (v1 * (double)1 / 3 - v2 * (double)7 / 6 + v3 * (double)11 / 6) +
+         w2 * (v2 * (-(double)1 / 6) + v3 * (double)5 / 6 + v4 * (double)1 / 3) +
+         w3 * (v3 * (double)1 / 3 + v4 * (double)5 / 6 - v5 * (double)1 / 6);
+}
+
+// |grad phi| by the reference's selection among the one-sided derivatives
+// (ADVECTION_METHOD_2D.h:936-1060): upwind by the sign of s times each
+// derivative, the larger magnitude at a shock, none at an expansion.  NaN when
+// no case applies (NaN derivatives): the cell is then left unchanged.
+__device__ __forceinline__ double godunov_h(double s, double mx, double px, double my, double py) {
+#pragma clang fp contract(off)
+  const double smx = s * mx, spx = s * px, smy = s * my, spy = s * py;
+  bool hasx = true;
+  double xd = 0.0;
+  if ((spx >= 0) && (smx >= 0)) {
+    xd = mx;
+  } else if ((spx <= 0) && (smx <= 0)) {
+    xd = px;
+  } else if ((spx > 0) && (smx < 0)) {
+    hasx = false;
+  } else if ((spx < 0) && (smx > 0)) {
+    const double sss = s * (fabs(px) - fabs(mx)) / (px - mx);
+    xd = sss > 0 ? mx : px;
+  } else {
+    return __longlong_as_double(0x7ff8000000000000LL);
+  }
+  const double x2 = xd * xd;
+  if ((spy >= 0) && (smy >= 0)) return hasx ? sqrt(x2 + my * my) : sqrt(my * my);
+  if ((spy <= 0) && (smy <= 0)) return hasx ? sqrt(x2 + py * py) : sqrt(py * py);
+  if ((spy > 0) && (smy < 0)) return hasx ? sqrt(x2) : 0.0;
+  if ((spy < 0) && (smy > 0)) {
+    const double ss = s * (fabs(py) - fabs(my)) / (py - my);
+    if (ss > 0) return hasx ? sqrt(x2 + my * my) : sqrt(my * my);
+    return hasx ? sqrt(x2 + py * py) : sqrt(py * py);
+  }
+  return __longlong_as_double(0x7ff8000000000000LL);
+}
+
+// One re-initialisation step, ADVECTION_METHOD_2D::WENO5thReinitialization
+// (ADVECTION_METHOD_2D.h:805-1076): out = phi - dt s (H - 1), a sign flip
+// clamped to +-eps.  The 64 x 16 tile plus a 3-cell ring is staged in LDS with
+// the reference's ghost rule (FIELD_STRUCTURE_2D.h:528-582: every ghost of a
+// face row takes the second interior value; corners are never read).
+// Grid dim3(ceil(nx/64), ceil(ny/16)), BLOCK threads.
+__global__ void __launch_bounds__(BLOCK) k_reinit_step(int nx, int ny, double h, double dt, double eps,
+                                                       const double* __restrict__ phi, const double* __restrict__ sg,
+                                                       double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double g[RI_H + 2 * RI_G][RI_W + 2 * RI_G];
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * RI_W, j0 = by * RI_H;
+  for (int t = threadIdx.x; t < (RI_H + 2 * RI_G) * (RI_W + 2 * RI_G); t += BLOCK) {
+    const int a = t % (RI_W + 2 * RI_G), b = t / (RI_W + 2 * RI_G);
+    const int i = i0 - RI_G + a, j = j0 - RI_G + b;
+    const bool xin = i >= 0 && i < nx, yin = j >= 0 && j < ny;
+    double v = 0.0;
+    if (xin && yin) v = phi[ci(i, j, nx)];
+    else if (yin) v = phi[ci(i < 0 ? 1 : nx - 2, j, nx)];
+    else if (xin) v = phi[ci(i, j < 0 ? 1 : ny - 2, nx)];
+    g[b][a] = v;
+  }
+  __syncthreads();
+  const double ih = 1.0 / h;
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < RI_H / (BLOCK / 64); ++m) {
+    const int r = w + (BLOCK / 64) * m, i = i0 + c, j = j0 + r;
+    if (i >= nx || j >= ny) continue;
+    const int a = c + RI_G, b = r + RI_G;
+    auto X = [&](int d) { return g[b][a + d]; };
+    auto Y = [&](int d) { return g[b + d][a]; };
+    const double mx = weno5_d(ih * (X(-2) - X(-3)), ih * (X(-1) - X(-2)), ih * (X(0) - X(-1)), ih * (X(1) - X(0)),
+                              ih * (X(2) - X(1)), eps);
+    const double px = weno5_d(ih * (X(3) - X(2)), ih * (X(2) - X(1)), ih * (X(1) - X(0)), ih * (X(0) - X(-1)),
+                              ih * (X(-1) - X(-2)), eps);
+    const double my = weno5_d(ih * (Y(-2) - Y(-3)), ih * (Y(-1) - Y(-2)), ih * (Y(0) - Y(-1)), ih * (Y(1) - Y(0)),
+                              ih * (Y(2) - Y(1)), eps);
+    const double py = weno5_d(ih * (Y(3) - Y(2)), ih * (Y(2) - Y(1)), ih * (Y(1) - Y(0)), ih * (Y(0) - Y(-1)),
+                              ih * (Y(-1) - Y(-2)), eps);
+    const size_t k = ci(i, j, nx);
+    const double s = sg[k];
+    const double H = godunov_h(s, mx, px, my, py);
+    double v = g[b][a];
+    if (H == H) v = v - dt * s * (H - 1);
+    if (s * v < 0) {
+      if (s < 0) v = -eps;
+      if (s > 0) v = eps;
+    }
+    out[k] = v;
+  }
+}
+
+// Unit normals, LEVELSET_2D::ComputeNormals (LEVELSET_2D.h:593-620): central
+// differences over the linear-extrapolation ghosts, one-sided forward
+// differences where the central gradient vanishes, scaled by 1/|n|
+// (VECTOR_2D::MakeThisUnit), (0, 0) when |n| = 0.  Grid dim3(ceil(nx/BLOCK),
+// ceil(ny/BAND)): one column of BAND rows per thread.
+__global__ void __launch_bounds__(BLOCK) k_normals(int nx, int ny, double h, const double* __restrict__ phi,
+                                                   double* __restrict__ onx, double* __restrict__ ony) {
+#pragma clang fp contract(off)
+  const int i = (int)(blockIdx.x * BLOCK + threadIdx.x);
+  if (i >= nx) return;
+  const double one_over_2dx = 1.0 / (2.0 * h), one_over_dx = 1.0 / h;
+  const int j1 = min(ny, (int)(blockIdx.y + 1) * BAND);
+  for (int j = (int)blockIdx.y * BAND; j < j1; ++j) {
+    const double c = phi_at(phi, nx, ny, i, j), r = phi_at(phi, nx, ny, i + 1, j), t = phi_at(phi, nx, ny, i, j + 1);
+    double a = (r - phi_at(phi, nx, ny, i - 1, j)) * one_over_2dx;
+    double b = (t - phi_at(phi, nx, ny, i, j - 1)) * one_over_2dx;
+    if (sqrt(a * a + b * b) == 0) {
+      a = (r - c) * one_over_dx;
+      b = (t - c) * one_over_dx;
+    }
+    const double mag = sqrt(a * a + b * b), inv = 1.0 / mag;
+    const size_t k = ci(i, j, nx);
+    onx[k] = mag != 0 ? a * inv : 0.0;
+    ony[k] = mag != 0 ? b * inv : 0.0;
+  }
+}
+
+}  // namespace p2d

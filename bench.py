@@ -106,6 +106,32 @@ def time_adjoint(ctx, torch, w, uv, steps, dev):
     return statistics.median(times), st
 
 
+def time_reinit(P2D, torch, ctx, w, dev, reps=5):
+    """Level-set re-initialisation of the workload's level set (scaled by 2, so it is not a
+    distance): 25 Sussman WENO5 steps, dt = h/5, device-resident, timed like a projection;
+    plus the per-step kernel alone (p2d_time_kernel)."""
+    src = torch.from_numpy(2.0 * w.phi).to(dev)
+    phi = src.clone()
+    stream = torch.cuda.current_stream(dev)
+    times = []
+    for k in range(reps + 1):
+        phi.copy_(src)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.reinitialize(phi, iterations=25, scaling=5.0, epsilon=1e-6)
+        e1.record(stream)
+        e1.synchronize()
+        if k:
+            times.append(e0.elapsed_time(e1))
+    step_ms = ctx.time_kernel(P2D.TIME_REINIT_STEP, 50)
+    N = w.n * w.n
+    return {"ms_25_steps": round(statistics.median(times), 4), "step_avg_launch_ms": round(step_ms, 5),
+            "step_bytes_per_cell": 24, "step_achieved_GBs": round(24 * N / (step_ms * 1e-3) / 1e9, 1),
+            "accounting": "per step: reads phi and the sign function, writes phi (24 B/cell, FP64); "
+                          "WENO5 + Godunov is ~300 FP64 VALU instructions per cell"}
+
+
 def read_traffic():
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, or None."""
     try:
@@ -176,6 +202,7 @@ def main():
         kernels[name] = {"avg_launch_ms": round(kms, 5), "bytes_per_launch": kbytes,
                          "achieved_GBs": round(kbytes / (kms * 1e-3) / 1e9, 1), "bytes_per_cell": bpc,
                          "accounting": what}
+    levelset = time_reinit(P2D, torch, ctx_m, w, dev)
     dom = kernels[DOMINANT]
     traffic = read_traffic()
     roofline = {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -196,6 +223,7 @@ def main():
         "adjoint": adjoint,
         "roofline": roofline,
         "kernels": kernels,
+        "levelset_reinit": levelset,
     }
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, serial)")

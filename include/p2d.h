@@ -118,6 +118,20 @@ int p2d_get_coefficients(p2d_ctx* ctx, double* beta_x, double* beta_y, double* j
 int p2d_adjoint(p2d_ctx* ctx, const double* gu, const double* gv, double* du, double* dv, double* dbeta_x,
                 double* dbeta_y, p2d_stats* stats, int mem);
 
+/* Level-set re-initialisation, in place on phi (nx*ny cells, FP64): the
+ * sussmanstyle_reinitialization branch of EULERIAN_FLUID_SOLVER_2D::Reinitialization
+ * (EULERIAN_FLUID_SOLVER_2D.h:2365-2400): s = phi0/sqrt(phi0^2 + h^2), then `iterations`
+ * steps of ADVECTION_2D::ReinitializationBySussman (ADVECTION_2D.cpp:415-421 ->
+ * ADVECTION_METHOD_2D::WENO5thReinitialization, ADVECTION_METHOD_2D.h:805-1076) with the
+ * fictitious dt = h/scaling and WENO epsilon `epsilon` (the reference's scripts: 25 or 10
+ * iterations, scaling 5, epsilon 1e-6).  Needs nx, ny >= 3; independent of the fluid state. */
+int p2d_reinitialize(p2d_ctx* ctx, double* phi, int iterations, double scaling, double epsilon, int mem);
+
+/* Unit normals of the level set phi (nx*ny), LEVELSET_2D::ComputeNormals (LEVELSET_2D.h:593-620):
+ * central differences over linear-extrapolation ghosts, forward differences where the central
+ * gradient vanishes, (0, 0) where both vanish. */
+int p2d_levelset_normals(p2d_ctx* ctx, const double* phi, double* normal_x, double* normal_y, int mem);
+
 /* Test hook: y = A x with the consistent operator -D B G (no-flux walls, no pin,
  * entries beta/h^2) of the current coefficients, in FP64 or FP32 (prec). */
 int p2d_apply_stencil(p2d_ctx* ctx, const void* x, void* y, int prec, int mem);
@@ -134,7 +148,8 @@ int p2d_mg_levels(p2d_ctx* ctx);
  * which: 0 = FP32 direction update + stencil + p.q (k_dir<float>), 1 = FP64 of the same,
  *        2 = FP32 fused Krylov update + pre-smoothing + restriction (k_mg_pre<float,true>),
  *        3 = FP32 prolongation + post-smoothing + r.z (k_mg_post<float,true>),
- *        4 = true residual in double-double with the FP32 correction (k_correct_resid). */
+ *        4 = true residual in double-double with the FP32 correction (k_correct_resid),
+ *        5 = one level-set re-initialisation step of the fluid's level set (k_reinit_step). */
 int p2d_time_kernel(p2d_ctx* ctx, int which, int reps, double* avg_ms);
 
 /* Bench hook: from now on (enable = 1) time every launch of kernel `which` inside the
