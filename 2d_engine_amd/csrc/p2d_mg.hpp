@@ -507,7 +507,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
   strip_stage<T>(S, F, giA, gy0, w, RY);
   strip_diag<T>(S);
   // interpolation weights of the restriction footprint: loaded with the rest on
-  // coarse levels, after the sweeps on the finest (not live during them there)
+  // coarse levels, late in the sweeps on the finest (registers are short there)
   T wxA[KR], wxB[KR], wyA[KR], wyB[KR];
   auto load_weights = [&]() {
 #pragma unroll
@@ -559,8 +559,11 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
     S.rA[m] = ra;
     S.rB[m] = rb;
   }
-  for (int k = 0; k < 2 * nu; ++k) strip_half<T>(S, lds, c, w, gy0, k & 1, k == 0, k);
+  // on the finest level the weights' loads are issued before the last
+  // half-sweep, whose work covers part of their latency
+  for (int k = 0; k < 2 * nu - 1; ++k) strip_half<T>(S, lds, c, w, gy0, k & 1, k == 0, k);
   if (FUSE) load_weights();
+  strip_half<T>(S, lds, c, w, gy0, 1, false, 2 * nu - 1);
   // residual (reference order: r - (diag x - offsum)) with the last published
   // boundary rows (buffer (2 nu - 1) & 1), and the interior write of x
   T resA[KR], resB[KR];
