@@ -40,7 +40,7 @@ EXPORTS = [
     "p2d_config_default", "p2d_create", "p2d_destroy", "p2d_set_stream", "p2d_set_fluid",
     "p2d_set_coefficients", "p2d_project", "p2d_get_pressure", "p2d_get_coefficients", "p2d_adjoint",
     "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_kernel_timing",
-    "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals",
+    "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals", "p2d_advect_levelset",
 ]
 
 
@@ -98,6 +98,7 @@ def _load():
         "p2d_status_string": (ctypes.c_char_p, [i]),
         "p2d_reinitialize": (i, [vp, vp, i, d, d, i]),
         "p2d_levelset_normals": (i, [vp, vp, vp, vp, i]),
+        "p2d_advect_levelset": (i, [vp, vp, vp, vp, d, d, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -285,6 +286,20 @@ class Context:
             b = _Buf(out, self.cell_shape, writable=True)
         _check(lib.p2d_reinitialize(self._h, b.ptr, int(iterations), float(scaling), float(epsilon), b.mem),
                "p2d_reinitialize")
+        return out
+
+    def advect_levelset(self, phi, u, v, dt, epsilon=1e-6):
+        """One forward-Euler WENO5 advection stage of phi by the MAC velocity (u, v)
+        (ADVECTION_2D.cpp:165-200).  Torch CUDA tensors: phi in place; numpy: a new array."""
+        bu, bv = _Buf(u, self.u_shape), _Buf(v, self.v_shape)
+        if _is_torch(phi):
+            b = _Buf(phi, self.cell_shape, writable=True)
+            out = phi
+        else:
+            out = np.array(phi, dtype=np.float64, order="C", copy=True)
+            b = _Buf(out, self.cell_shape, writable=True)
+        _check(lib.p2d_advect_levelset(self._h, b.ptr, bu.ptr, bv.ptr, float(dt), float(epsilon), _mem(b, bu, bv)),
+               "p2d_advect_levelset")
         return out
 
     def normals(self, phi):

@@ -993,6 +993,26 @@ int p2d_reinitialize(p2d_ctx* c, double* phi, int iterations, double scaling, do
   return P2D_OK;
 }
 
+int p2d_advect_levelset(p2d_ctx* c, double* phi, const double* u, const double* v, double dt, double epsilon,
+                        int mem) {
+  if (!c || !phi || !u || !v || !(epsilon >= 0) || !(dt == dt) || c->nx < 3 || c->ny < 3) return P2D_BAD_ARG;
+  const double* du = u;
+  const double* dv = v;
+  if (mem == P2D_MEM_HOST) {
+    CHK(stage_in(c, c->su, u, c->NU, mem));
+    CHK(stage_in(c, c->sv, v, c->NV, mem));
+    du = c->su;
+    dv = c->sv;
+  }
+  CHK(stage_in(c, c->v64.x, phi, c->N, mem));
+  k_advect_ls<<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, epsilon, c->v64.x, du, dv,
+                                                            c->v64.z);
+  CHK(launched_ok());
+  CHK(stage_out(c, phi, c->v64.z, c->N, mem));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return P2D_OK;
+}
+
 int p2d_levelset_normals(p2d_ctx* c, const double* phi, double* normal_x, double* normal_y, int mem) {
   if (!c || !phi || !normal_x || !normal_y || c->nx < 2 || c->ny < 2) return P2D_BAD_ARG;
   CHK(stage_in(c, c->v64.x, phi, c->N, mem));

@@ -1,5 +1,6 @@
 // p2d level-set kernels: the Sussman re-initialisation (WENO5 one-sided
-// derivatives + Godunov Hamiltonian) and unit normals.  FP64 with
+// derivatives + Godunov Hamiltonian), the WENO5 advection stage and unit
+// normals.  FP64 with
 // contraction off: the arithmetic keeps the reference's operation order
 // (ADVECTION_METHOD_2D.h, LEVELSET_2D.h), so the results are reproducible
 // operation by operation on the CPU.
@@ -72,20 +73,12 @@ __device__ __forceinline__ double godunov_h(double s, double mx, double px, doub
   return __longlong_as_double(0x7ff8000000000000LL);
 }
 
-// One re-initialisation step, ADVECTION_METHOD_2D::WENO5thReinitialization
-// (ADVECTION_METHOD_2D.h:805-1076): out = phi - dt s (H - 1), a sign flip
-// clamped to +-eps.  The 64 x 16 tile plus a 3-cell ring is staged in LDS with
-// the reference's ghost rule (FIELD_STRUCTURE_2D.h:528-582: every ghost of a
-// face row takes the second interior value; corners are never read).
-// Grid dim3(ceil(nx/64), ceil(ny/16)), BLOCK threads.
-__global__ void __launch_bounds__(BLOCK) k_reinit_step(int nx, int ny, double h, double dt, double eps,
-                                                       const double* __restrict__ phi, const double* __restrict__ sg,
-                                                       double* __restrict__ out) {
-#pragma clang fp contract(off)
-  __shared__ double g[RI_H + 2 * RI_G][RI_W + 2 * RI_G];
-  int bx, by;
-  xcd_tile(bx, by);
-  const int i0 = bx * RI_W, j0 = by * RI_H;
+// Stages a 64 x 16 tile of phi plus a 3-cell ring in LDS with the ghost rule
+// of FIELD_STRUCTURE_2D::FillGhostCellsFrom (FIELD_STRUCTURE_2D.h:528-582):
+// every ghost of a face row takes the second interior value; corners, never
+// read by the dimension-by-dimension stencils, are 0.
+__device__ __forceinline__ void stage_weno_tile(double (*g)[RI_W + 2 * RI_G], const double* __restrict__ phi, int nx,
+                                                int ny, int i0, int j0) {
   for (int t = threadIdx.x; t < (RI_H + 2 * RI_G) * (RI_W + 2 * RI_G); t += BLOCK) {
     const int a = t % (RI_W + 2 * RI_G), b = t / (RI_W + 2 * RI_G);
     const int i = i0 - RI_G + a, j = j0 - RI_G + b;
@@ -96,6 +89,21 @@ __global__ void __launch_bounds__(BLOCK) k_reinit_step(int nx, int ny, double h,
     else if (xin) v = phi[ci(i, j < 0 ? 1 : ny - 2, nx)];
     g[b][a] = v;
   }
+}
+
+// One re-initialisation step, ADVECTION_METHOD_2D::WENO5thReinitialization
+// (ADVECTION_METHOD_2D.h:805-1076): out = phi - dt s (H - 1), a sign flip
+// clamped to +-eps.  The tile is staged by stage_weno_tile.
+// Grid dim3(ceil(nx/64), ceil(ny/16)), BLOCK threads.
+__global__ void __launch_bounds__(BLOCK) k_reinit_step(int nx, int ny, double h, double dt, double eps,
+                                                       const double* __restrict__ phi, const double* __restrict__ sg,
+                                                       double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double g[RI_H + 2 * RI_G][RI_W + 2 * RI_G];
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * RI_W, j0 = by * RI_H;
+  stage_weno_tile(g, phi, nx, ny, i0, j0);
   __syncthreads();
   const double ih = 1.0 / h;
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -124,6 +132,46 @@ __global__ void __launch_bounds__(BLOCK) k_reinit_step(int nx, int ny, double h,
       if (s > 0) v = eps;
     }
     out[k] = v;
+  }
+}
+
+// One forward-Euler stage of the level-set advection, ADVECTION_2D::Solve_Levelset
+// with use_mac_grid and use_5th_weno (ADVECTION_2D.cpp:165-200) =
+// ADVECTION_METHOD_2D::WENO5th on a scalar field (ADVECTION_METHOD_2D.h:508-618):
+// the cell velocity is the mean of its two faces, each component picks the
+// upwind one-sided WENO5 derivative (u > 0: phi_x^-), out = phi - dt (u phi_x
+// + v phi_y).  Only the two derivatives used are formed.  u: (nx+1) x ny
+// x-faces, v: nx x (ny+1) y-faces.  Grid dim3(ceil(nx/64), ceil(ny/16)).
+__global__ void __launch_bounds__(BLOCK) k_advect_ls(int nx, int ny, double h, double dt, double eps,
+                                                     const double* __restrict__ phi, const double* __restrict__ u,
+                                                     const double* __restrict__ v, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  __shared__ double g[RI_H + 2 * RI_G][RI_W + 2 * RI_G];
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * RI_W, j0 = by * RI_H;
+  stage_weno_tile(g, phi, nx, ny, i0, j0);
+  __syncthreads();
+  const double ih = 1.0 / h;
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < RI_H / (BLOCK / 64); ++m) {
+    const int r = w + (BLOCK / 64) * m, i = i0 + c, j = j0 + r;
+    if (i >= nx || j >= ny) continue;
+    const int a = c + RI_G, b = r + RI_G;
+    auto X = [&](int d) { return g[b][a + d]; };
+    auto Y = [&](int d) { return g[b + d][a]; };
+    const double uv = (u[fxi(i + 1, j, nx)] + u[fxi(i, j, nx)]) * 0.5;
+    const double vv = (v[fyi(i, j + 1, nx)] + v[fyi(i, j, nx)]) * 0.5;
+    const double dx = uv > 0 ? weno5_d(ih * (X(-2) - X(-3)), ih * (X(-1) - X(-2)), ih * (X(0) - X(-1)),
+                                       ih * (X(1) - X(0)), ih * (X(2) - X(1)), eps)
+                             : weno5_d(ih * (X(3) - X(2)), ih * (X(2) - X(1)), ih * (X(1) - X(0)),
+                                       ih * (X(0) - X(-1)), ih * (X(-1) - X(-2)), eps);
+    const double dy = vv > 0 ? weno5_d(ih * (Y(-2) - Y(-3)), ih * (Y(-1) - Y(-2)), ih * (Y(0) - Y(-1)),
+                                       ih * (Y(1) - Y(0)), ih * (Y(2) - Y(1)), eps)
+                             : weno5_d(ih * (Y(3) - Y(2)), ih * (Y(2) - Y(1)), ih * (Y(1) - Y(0)),
+                                       ih * (Y(0) - Y(-1)), ih * (Y(-1) - Y(-2)), eps);
+    out[ci(i, j, nx)] = g[b][a] - dt * (uv * dx + vv * dy);
   }
 }
 

@@ -127,6 +127,15 @@ int p2d_adjoint(p2d_ctx* ctx, const double* gu, const double* gv, double* du, do
  * iterations, scaling 5, epsilon 1e-6).  Needs nx, ny >= 3; independent of the fluid state. */
 int p2d_reinitialize(p2d_ctx* ctx, double* phi, int iterations, double scaling, double epsilon, int mem);
 
+/* One forward-Euler stage of the level-set advection, in place on phi (nx*ny, FP64):
+ * ADVECTION_2D::Solve_Levelset with use_mac_grid and use_5th_weno (ADVECTION_2D.cpp:165-200 ->
+ * ADVECTION_METHOD_2D::WENO5th, ADVECTION_METHOD_2D.h:508-618) by the MAC velocity u ((nx+1)*ny
+ * x-faces), v (nx*(ny+1) y-faces): phi -= dt (u phi_x + v phi_y), cell velocity = mean of its
+ * faces, upwind WENO5 derivatives with `epsilon` (the scripts: 1e-6).  The caller's RK stages
+ * combine these stages as the reference's time integrator does.  Needs nx, ny >= 3. */
+int p2d_advect_levelset(p2d_ctx* ctx, double* phi, const double* u, const double* v, double dt, double epsilon,
+                        int mem);
+
 /* Unit normals of the level set phi (nx*ny), LEVELSET_2D::ComputeNormals (LEVELSET_2D.h:593-620):
  * central differences over linear-extrapolation ghosts, forward differences where the central
  * gradient vanishes, (0, 0) where both vanish. */

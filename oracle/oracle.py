@@ -77,6 +77,7 @@ def lib():
             "orc_num_threads": (_i, []),
             "orc_weno5_derivatives": (None, [_i, _i, _d, _pd, _d, _pd, _pd, _pd, _pd]),
             "orc_reinit_sussman": (None, [_i, _i, _d, _pd, _i, _d, _d, _pd]),
+            "orc_advect_levelset": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _d, _pd]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -152,6 +153,15 @@ def reinit_sussman(phi, h, iterations=25, scaling=5.0, epsilon=1e-6):
     ny, nx = phi.shape
     out = np.empty_like(phi)
     lib().orc_reinit_sussman(nx, ny, h, _p(phi), int(iterations), float(scaling), float(epsilon), _p(out))
+    return out
+
+
+def advect_levelset(phi, u, v, h, dt, epsilon=1e-6):
+    """One forward-Euler WENO5 advection stage of the level set by the MAC velocity (u, v)."""
+    phi, u, v = _f64(phi), _f64(u), _f64(v)
+    ny, nx = phi.shape
+    out = np.empty_like(phi)
+    lib().orc_advect_levelset(nx, ny, h, dt, _p(phi), _p(u), _p(v), float(epsilon), _p(out))
     return out
 
 

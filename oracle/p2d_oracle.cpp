@@ -1302,6 +1302,29 @@ void orc_reinit_sussman(int nx, int ny, double h, const double* phi_in, int iter
   std::copy(phi.begin(), phi.end(), phi_out);
 }
 
+// Level-set advection, one forward-Euler stage: ADVECTION_2D::Solve_Levelset
+// (ADVECTION_2D.cpp:165-200, use_mac_grid + use_5th_weno) =
+// ADVECTION_METHOD_2D::WENO5th on a scalar field with MAC velocities
+// (ADVECTION_METHOD_2D.h:508-618): the ghost copy and WENO5 derivatives of the
+// re-initialisation, the cell velocity as the mean of its two faces,
+// u = (u(i+1,j) + u(i,j)) * 0.5, v = (v(i,j+1) + v(i,j)) * 0.5, and the upwind
+// side by the sign of each component (u > 0: phi_x^-, else phi_x^+; same in y):
+// phi -= dt (u phi_x + v phi_y).  u is (nx+1) x ny, v nx x (ny+1).
+void orc_advect_levelset(int nx, int ny, double h, double dt, const double* phi, const double* u, const double* v,
+                         double epsilon, double* out) {
+  const Pad3 g = reinit_ghosts(nx, ny, phi);
+  const double ih = 1.0 / h;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double d[4];
+      weno5_derivs(g, i, j, ih, epsilon, d);
+      const double uv = (u[FX(i + 1, j, nx)] + u[FX(i, j, nx)]) * 0.5;
+      const double vv = (v[FY(i, j + 1, nx)] + v[FY(i, j, nx)]) * 0.5;
+      const double dx = uv > 0 ? d[0] : d[1], dy = vv > 0 ? d[2] : d[3];
+      out[C(i, j, nx)] = phi[C(i, j, nx)] - dt * (uv * dx + vv * dy);
+    }
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -1,6 +1,7 @@
 """GPU parity of the level-set kernels against the oracle, through the C ABI.
 
-The re-initialisation (Sussman WENO5, 25 steps of dt = h/5) and the normals run
+The re-initialisation (Sussman WENO5, 25 steps of dt = h/5), the WENO5
+advection stage and the normals run
 in FP64 with the reference's operation order and no contraction on both sides
 (correctly rounded division and square root on both), so they must equal the
 oracle bit for bit.  At 2048^2 (oracle too slow) the checks are the properties
@@ -89,3 +90,27 @@ def test_reinit_2048_properties(P2D, W):
     gy, gx = np.gradient(got, h)
     band = (np.abs(d) < 3 * h) & (np.abs(d) > h)
     assert np.abs(np.hypot(gx, gy)[band] - 1).max() < 0.05
+
+
+@pytest.mark.parametrize("case", ["drops64", "rt", "rect"])
+def test_advection_matches_oracle(P2D, O, W, case):
+    print(f"case advection {case}", flush=True)
+    if case == "rect":
+        phi, h = _circle_scaled(64, 96)
+        ny, nx = phi.shape
+        rng = np.random.default_rng(5)
+        u = rng.uniform(-1, 1, (ny, nx + 1))
+        v = rng.uniform(-1, 1, (ny + 1, nx))
+    else:
+        w = W.make(case, 128)
+        phi, h, u, v = w.phi, w.h, w.u, w.v
+        ny, nx = phi.shape
+    ctx = P2D.Context(nx, ny, h)
+    dt = 0.4 * h
+    got = ctx.advect_levelset(phi, u, v, dt)
+    ref = O.advect_levelset(phi, u, v, h, dt)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    import torch
+    t = torch.from_numpy(phi).cuda()
+    ctx.advect_levelset(t, torch.from_numpy(u).cuda(), torch.from_numpy(v).cuda(), dt)
+    assert np.array_equal(t.cpu().numpy(), ref)

@@ -101,3 +101,38 @@ def test_normals_of_circle_are_radial(O):
         errs.append(max(np.abs(nx - (X - 0.5) / r)[mask].max(), np.abs(ny - (Y - 0.5) / r)[mask].max()))
         assert np.abs(np.hypot(nx, ny) - 1).max() < 1e-14
     assert errs[1] < 1e-3 and errs[0] / errs[1] > 3.0, errs  # second order
+
+
+def test_advection_linear_field_closed_form(O):
+    """phi = a x + b y under a uniform velocity: phi - dt (u a + v b) exactly, away
+    from the walls' ghost band (WENO5 is exact on linear data)."""
+    n = 48
+    h, X, Y = _grid(n)
+    phi = 0.7 * X - 0.3 * Y + 0.1
+    for uu, vv in ((0.5, 0.25), (-0.5, 0.25), (0.5, -0.25), (-0.5, -0.25)):
+        u = np.full((n, n + 1), uu)
+        v = np.full((n + 1, n), vv)
+        got = O.advect_levelset(phi, u, v, h, 0.1 * h)
+        ref = phi - 0.1 * h * (uu * 0.7 + vv * -0.3)
+        band = (slice(3, n - 3), slice(3, n - 3))
+        assert np.abs(got - ref)[band].max() <= 1e-15
+
+
+def test_advection_upwind_side_and_face_average(O):
+    """On a varying face velocity the stage equals -dt (u_c D u + v_c D v) with the
+    cell velocity the mean of its faces and the upwind one-sided WENO5 derivative."""
+    n = 40
+    h, X, Y = _grid(n)
+    phi = np.sin(3 * X) * np.cos(2 * Y) + 0.3 * X
+    xf = np.arange(n + 1) * h
+    yf = (np.arange(n) + 0.5) * h
+    u = np.sin(5 * xf)[None, :] * np.cos(4 * yf)[:, None]                 # (n, n+1)
+    v = np.cos(3 * ((np.arange(n) + 0.5) * h))[None, :] * np.sin(6 * np.arange(n + 1) * h)[:, None] - 0.2  # (n+1, n)
+    dt = 0.3 * h
+    a, b, c, d = O.weno5_derivatives(phi, h)
+    uc = 0.5 * (u[:, 1:] + u[:, :-1])
+    vc = 0.5 * (v[1:, :] + v[:-1, :])
+    ref = phi - dt * (uc * np.where(uc > 0, a, b) + vc * np.where(vc > 0, c, d))
+    got = O.advect_levelset(phi, u, v, h, dt)
+    assert np.abs(got - ref).max() <= 1e-15 * np.abs(phi).max() + 1e-16
+    assert (uc > 0).any() and (uc < 0).any() and (vc > 0).any() and (vc < 0).any()
