@@ -41,6 +41,7 @@ EXPORTS = [
     "p2d_set_coefficients", "p2d_project", "p2d_get_pressure", "p2d_get_coefficients", "p2d_adjoint",
     "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_kernel_timing",
     "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals", "p2d_advect_levelset",
+    "p2d_advect_velocity",
 ]
 
 
@@ -99,6 +100,7 @@ def _load():
         "p2d_reinitialize": (i, [vp, vp, i, d, d, i]),
         "p2d_levelset_normals": (i, [vp, vp, vp, vp, i]),
         "p2d_advect_levelset": (i, [vp, vp, vp, vp, d, d, i]),
+        "p2d_advect_velocity": (i, [vp, vp, vp, d, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -301,6 +303,19 @@ class Context:
         _check(lib.p2d_advect_levelset(self._h, b.ptr, bu.ptr, bv.ptr, float(dt), float(epsilon), _mem(b, bu, bv)),
                "p2d_advect_levelset")
         return out
+
+    def advect_velocity(self, u, v, dt):
+        """One forward-Euler ENO3 advection stage of the MAC velocity by itself
+        (ADVECTION_2D.cpp:95-160).  Torch CUDA tensors: in place; numpy: new arrays."""
+        if _is_torch(u):
+            bu, bv = _Buf(u, self.u_shape, writable=True), _Buf(v, self.v_shape, writable=True)
+            uo, vo = u, v
+        else:
+            uo = np.array(u, dtype=np.float64, order="C", copy=True)
+            vo = np.array(v, dtype=np.float64, order="C", copy=True)
+            bu, bv = _Buf(uo, self.u_shape, writable=True), _Buf(vo, self.v_shape, writable=True)
+        _check(lib.p2d_advect_velocity(self._h, bu.ptr, bv.ptr, float(dt), _mem(bu, bv)), "p2d_advect_velocity")
+        return uo, vo
 
     def normals(self, phi):
         """Unit normals (nx, ny) of the level set phi (LEVELSET_2D.h:593-620)."""

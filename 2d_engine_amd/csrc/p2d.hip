@@ -1013,6 +1013,22 @@ int p2d_advect_levelset(p2d_ctx* c, double* phi, const double* u, const double* 
   return P2D_OK;
 }
 
+int p2d_advect_velocity(p2d_ctx* c, double* u, double* v, double dt, int mem) {
+  if (!c || !u || !v || !(dt == dt) || c->nx < 3 || c->ny < 3) return P2D_BAD_ARG;
+  // old velocity in su/sv, the new components in sa[0]/sa[1], then back
+  CHK(stage_in(c, c->su, u, c->NU, mem));
+  CHK(stage_in(c, c->sv, v, c->NV, mem));
+  const dim3 gu((unsigned)((c->nx + 1 + RI_W - 1) / RI_W), (unsigned)((c->ny + RI_H - 1) / RI_H));
+  const dim3 gv((unsigned)((c->nx + RI_W - 1) / RI_W), (unsigned)((c->ny + 1 + RI_H - 1) / RI_H));
+  k_advect_vel<true><<<gu, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->su, c->sv, c->sa[0]);
+  k_advect_vel<false><<<gv, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->su, c->sv, c->sa[1]);
+  CHK(launched_ok());
+  CHK(stage_out(c, u, c->sa[0], c->NU, mem));
+  CHK(stage_out(c, v, c->sa[1], c->NV, mem));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return P2D_OK;
+}
+
 int p2d_levelset_normals(p2d_ctx* c, const double* phi, double* normal_x, double* normal_y, int mem) {
   if (!c || !phi || !normal_x || !normal_y || c->nx < 2 || c->ny < 2) return P2D_BAD_ARG;
   CHK(stage_in(c, c->v64.x, phi, c->N, mem));

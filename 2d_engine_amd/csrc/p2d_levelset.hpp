@@ -1,6 +1,6 @@
 // p2d level-set kernels: the Sussman re-initialisation (WENO5 one-sided
-// derivatives + Godunov Hamiltonian), the WENO5 advection stage and unit
-// normals.  FP64 with
+// derivatives + Godunov Hamiltonian), the WENO5 level-set and ENO3 velocity
+// advection stages and unit normals.  FP64 with
 // contraction off: the arithmetic keeps the reference's operation order
 // (ADVECTION_METHOD_2D.h, LEVELSET_2D.h), so the results are reproducible
 // operation by operation on the CPU.
@@ -172,6 +172,88 @@ __global__ void __launch_bounds__(BLOCK) k_advect_ls(int nx, int ny, double h, d
                              : weno5_d(ih * (Y(3) - Y(2)), ih * (Y(2) - Y(1)), ih * (Y(1) - Y(0)),
                                        ih * (Y(0) - Y(-1)), ih * (Y(-1) - Y(-2)), eps);
     out[ci(i, j, nx)] = g[b][a] - dt * (uv * dx + vv * dy);
+  }
+}
+
+// ENO3 one-sided derivatives (minus, plus) along a line of seven values
+// f[0..6] = f(-3..3) with spacing dx: divided differences 1/dx, 1/(2dx),
+// (1/dx)/3 and the choice by the smaller |difference|
+// (ADVECTION_METHOD_2D.h:2054-2126, in its expression forms).
+__device__ __forceinline__ void eno3_d(const double* f, double dx, double* m, double* p) {
+#pragma clang fp contract(off)
+  const double o1 = 1.0 / dx, o2 = 1.0 / (2 * dx), o3 = o1 * 1 / 3;
+  const double d1n3 = o1 * (f[1] - f[0]), d1n2 = o1 * (f[2] - f[1]), d1n1 = o1 * (f[3] - f[2]);
+  const double d10 = o1 * (f[4] - f[3]), d1p1 = o1 * (f[5] - f[4]), d1p2 = o1 * (f[6] - f[5]);
+  const double d2n2 = o2 * (d1n2 - d1n3), d2n1 = o2 * (d1n1 - d1n2), d20 = o2 * (d10 - d1n1);
+  const double d2p1 = o2 * (d1p1 - d10), d2p2 = o2 * (d1p2 - d1p1);
+  const double d3n2 = o3 * (d2n1 - d2n2), d3n1 = o3 * (d20 - d2n1), d30 = o3 * (d2p1 - d20), d3p1 = o3 * (d2p2 - d2p1);
+  if (fabs(d2n1) <= fabs(d20)) {
+    *m = fabs(d3n2) <= fabs(d3n1) ? d1n1 + d2n1 * dx + 2 * d3n2 * dx * dx : d1n1 + d2n1 * dx + 2 * d3n1 * dx * dx;
+  } else {
+    *m = fabs(d3n1) <= fabs(d30) ? d1n1 + d20 * dx - d3n1 * dx * dx : d1n1 + d20 * dx - d30 * dx * dx;
+  }
+  if (fabs(d20) <= fabs(d2p1)) {
+    *p = fabs(d3n1) <= fabs(d30) ? d10 - d20 * dx - d3n1 * dx * dx : d10 - d20 * dx - d30 * dx * dx;
+  } else {
+    *p = fabs(d30) <= fabs(d3p1) ? d10 - d2p1 * dx + 2 * d30 * dx * dx : d10 - d2p1 * dx + 2 * d3p1 * dx * dx;
+  }
+}
+
+// A face field of fnx x fny values read through the ghost rule of
+// FillGhostCellsFrom (second interior value beyond each face; corners unused).
+__device__ __forceinline__ double face_ghost_at(const double* f, int fnx, int fny, int i, int j) {
+  i = i < 0 ? 1 : (i >= fnx ? fnx - 2 : i);
+  j = j < 0 ? 1 : (j >= fny ? fny - 2 : j);
+  return f[(size_t)i + (size_t)j * fnx];
+}
+
+// One forward-Euler ENO3 stage of one MAC velocity component (XCOMP: u on the
+// (nx+1) x ny x-faces, else v on the nx x (ny+1) y-faces) by the old velocity,
+// ADVECTION_2D::Solve_Velocity with use_3rd_eno_v (ADVECTION_2D.cpp:95-160) =
+// ADVECTION_METHOD_2D::ENO3rd (ADVECTION_METHOD_2D.h:2038-2232): the advecting
+// velocity at an x-face is (u, mean of the four y-faces around it), at a
+// y-face (mean of the four x-faces, v); upwind ENO3 derivative per component;
+// out = f - dt (u f_x + v f_y).  Grid dim3(ceil(fnx/64), ceil(fny/16)).
+template <bool XCOMP>
+__global__ void __launch_bounds__(BLOCK) k_advect_vel(int nx, int ny, double h, double dt, const double* __restrict__ u,
+                                                      const double* __restrict__ v, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int fnx = XCOMP ? nx + 1 : nx, fny = XCOMP ? ny : ny + 1;
+  const double* f = XCOMP ? u : v;
+  __shared__ double g[RI_H + 2 * RI_G][RI_W + 2 * RI_G];
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * RI_W, j0 = by * RI_H;
+  stage_weno_tile(g, f, fnx, fny, i0, j0);
+  __syncthreads();
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < RI_H / (BLOCK / 64); ++m) {
+    const int r = w + (BLOCK / 64) * m, i = i0 + c, j = j0 + r;
+    if (i >= fnx || j >= fny) continue;
+    const int a = c + RI_G, b = r + RI_G;
+    double lx[7], ly[7], mx, px, my, py;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      lx[k] = g[b][a - 3 + k];
+      ly[k] = g[b - 3 + k][a];
+    }
+    eno3_d(lx, h, &mx, &px);
+    eno3_d(ly, h, &my, &py);
+    double uv, vv;
+    if (XCOMP) {
+      uv = g[b][a];
+      vv = (face_ghost_at(v, nx, ny + 1, i, j) + face_ghost_at(v, nx, ny + 1, i, j + 1) +
+            face_ghost_at(v, nx, ny + 1, i - 1, j) + face_ghost_at(v, nx, ny + 1, i - 1, j + 1)) *
+           0.25;
+    } else {
+      uv = (face_ghost_at(u, nx + 1, ny, i, j - 1) + face_ghost_at(u, nx + 1, ny, i + 1, j - 1) +
+            face_ghost_at(u, nx + 1, ny, i, j) + face_ghost_at(u, nx + 1, ny, i + 1, j)) *
+           0.25;
+      vv = g[b][a];
+    }
+    const double dxv = uv > 0 ? mx : px, dyv = vv > 0 ? my : py;
+    out[(size_t)i + (size_t)j * fnx] = g[b][a] - dt * (uv * dxv + vv * dyv);
   }
 }
 

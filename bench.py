@@ -140,14 +140,28 @@ def time_reinit(P2D, torch, ctx, w, dev, reps=5):
         e1.synchronize()
         if k:
             adv.append(e0.elapsed_time(e1))
+    uu, vv = u.clone(), v.clone()
+    vel = []
+    for k in range(reps + 1):
+        uu.copy_(u)
+        vv.copy_(v)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.advect_velocity(uu, vv, 0.4 * w.h)
+        e1.record(stream)
+        e1.synchronize()
+        if k:
+            vel.append(e0.elapsed_time(e1))
     return {"reinit_ms_25_steps": round(statistics.median(times), 4),
             "reinit_step_avg_launch_ms": round(step_ms, 5), "reinit_step_bytes_per_cell": 24,
             "reinit_step_achieved_GBs": round(24 * N / (step_ms * 1e-3) / 1e9, 1),
             "advect_stage_ms": round(statistics.median(adv), 4),
+            "velocity_advect_stage_ms": round(statistics.median(vel), 4),
             "accounting": "re-initialisation step: reads phi and the sign function, writes phi (24 B/cell, FP64), "
                           "~600 FP64 VALU instructions per cell (four WENO5 derivatives, Godunov); advection stage: "
                           "reads phi and the two face velocities, writes phi (32 B/cell), two WENO5 derivatives "
-                          "per cell, timed with its two device copies"}
+                          "per cell, timed with its two device copies; velocity stage (ENO3, both components): reads u, v, writes u, v, timed with its four device copies"}
 
 
 def read_traffic():

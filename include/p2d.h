@@ -136,6 +136,14 @@ int p2d_reinitialize(p2d_ctx* ctx, double* phi, int iterations, double scaling, 
 int p2d_advect_levelset(p2d_ctx* ctx, double* phi, const double* u, const double* v, double dt, double epsilon,
                         int mem);
 
+/* One forward-Euler stage of the velocity advection, in place on the MAC velocity u
+ * ((nx+1)*ny), v (nx*(ny+1)), both components by the old velocity:
+ * ADVECTION_2D::Solve_Velocity with use_3rd_eno_v (ADVECTION_2D.cpp:95-160 ->
+ * ADVECTION_METHOD_2D::ENO3rd, ADVECTION_METHOD_2D.h:2038-2232).  Every face, walls
+ * included, is advected as in the reference; its boundary conditions are applied after.
+ * Needs nx, ny >= 3. */
+int p2d_advect_velocity(p2d_ctx* ctx, double* u, double* v, double dt, int mem);
+
 /* Unit normals of the level set phi (nx*ny), LEVELSET_2D::ComputeNormals (LEVELSET_2D.h:593-620):
  * central differences over linear-extrapolation ghosts, forward differences where the central
  * gradient vanishes, (0, 0) where both vanish. */

@@ -78,6 +78,7 @@ def lib():
             "orc_weno5_derivatives": (None, [_i, _i, _d, _pd, _d, _pd, _pd, _pd, _pd]),
             "orc_reinit_sussman": (None, [_i, _i, _d, _pd, _i, _d, _d, _pd]),
             "orc_advect_levelset": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _d, _pd]),
+            "orc_advect_velocity": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _pd]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -163,6 +164,17 @@ def advect_levelset(phi, u, v, h, dt, epsilon=1e-6):
     out = np.empty_like(phi)
     lib().orc_advect_levelset(nx, ny, h, dt, _p(phi), _p(u), _p(v), float(epsilon), _p(out))
     return out
+
+
+def advect_velocity(u, v, h, dt):
+    """One forward-Euler ENO3 advection stage of the MAC velocity by itself (both components
+    by the old velocity)."""
+    u, v = _f64(u), _f64(v)
+    ny, nx1 = u.shape
+    nx = nx1 - 1
+    uo, vo = np.empty_like(u), np.empty_like(v)
+    lib().orc_advect_velocity(nx, ny, h, dt, _p(u), _p(v), _p(uo), _p(vo))
+    return uo, vo
 
 
 def face_beta(phi, rho_minus, rho_plus):

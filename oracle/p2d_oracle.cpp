@@ -1325,6 +1325,97 @@ void orc_advect_levelset(int nx, int ny, double h, double dt, const double* phi,
     }
 }
 
+// Velocity advection, one forward-Euler stage: ADVECTION_2D::Solve_Velocity
+// (ADVECTION_2D.cpp:95-160, use_3rd_eno_v, the scripts' choice) =
+// ADVECTION_METHOD_2D::ENO3rd on each MAC component (ADVECTION_METHOD_2D.h:2038-2232).
+// Both components are advected by the old velocities.  Each face field
+// (u: (nx+1) x ny, v: nx x (ny+1)) gets a 3-cell ghost copy by the rule of
+// FillGhostCellsFrom (second interior value, FIELD_STRUCTURE_2D.h:528-582); the
+// one-sided derivatives are third-order ENO (divided differences 1/dx, 1/(2dx),
+// (1/dx)/3, choice by the smaller |difference|); the advecting velocity at an
+// x-face is (u, mean of the four surrounding y-faces), at a y-face (mean of the
+// four surrounding x-faces, v), reading the other component's ghost copy; the
+// upwind side follows each component's sign.  Every face is updated, walls too.
+namespace {
+struct PadG {  // a field of fnx x fny values with a 3-cell ghost ring
+  int nx, ny;
+  std::vector<double> a;
+  PadG(int nx_, int ny_) : nx(nx_), ny(ny_), a((size_t)(nx_ + 6) * (ny_ + 6), 0.0) {}
+  double& operator()(int i, int j) { return a[(size_t)(i + 3) + (size_t)(j + 3) * (nx + 6)]; }
+  double operator()(int i, int j) const { return a[(size_t)(i + 3) + (size_t)(j + 3) * (nx + 6)]; }
+};
+PadG face_ghosts(int fnx, int fny, const double* f) {
+  PadG g(fnx, fny);
+  for (int j = 0; j < fny; ++j)
+    for (int i = 0; i < fnx; ++i) g(i, j) = f[(size_t)i + (size_t)j * fnx];
+  for (int j = 0; j < fny; ++j)
+    for (int k = 1; k <= 3; ++k) {
+      g(-k, j) = f[(size_t)1 + (size_t)j * fnx];
+      g(fnx - 1 + k, j) = f[(size_t)(fnx - 2) + (size_t)j * fnx];
+    }
+  for (int i = 0; i < fnx; ++i)
+    for (int k = 1; k <= 3; ++k) {
+      g(i, fny - 1 + k) = f[(size_t)i + (size_t)(fny - 2) * fnx];
+      g(i, -k) = f[(size_t)i + (size_t)1 * fnx];
+    }
+  return g;
+}
+// ENO3 one-sided derivatives (minus, plus) from six consecutive values
+// f(-3..3) of a line, with spacing dx (ADVECTION_METHOD_2D.h:2054-2126).
+void eno3(const double* f /* f[0] = value at -3 ... f[6] at +3 */, double dx, double* m, double* p) {
+  const double o1 = 1.0 / dx, o2 = 1.0 / (2 * dx), o3 = o1 * 1 / 3;
+  const double d1n3 = o1 * (f[1] - f[0]), d1n2 = o1 * (f[2] - f[1]), d1n1 = o1 * (f[3] - f[2]);
+  const double d10 = o1 * (f[4] - f[3]), d1p1 = o1 * (f[5] - f[4]), d1p2 = o1 * (f[6] - f[5]);
+  const double d2n2 = o2 * (d1n2 - d1n3), d2n1 = o2 * (d1n1 - d1n2), d20 = o2 * (d10 - d1n1);
+  const double d2p1 = o2 * (d1p1 - d10), d2p2 = o2 * (d1p2 - d1p1);
+  const double d3n2 = o3 * (d2n1 - d2n2), d3n1 = o3 * (d20 - d2n1), d30 = o3 * (d2p1 - d20), d3p1 = o3 * (d2p2 - d2p1);
+  if (std::fabs(d2n1) <= std::fabs(d20)) {
+    if (std::fabs(d3n2) <= std::fabs(d3n1)) *m = d1n1 + d2n1 * dx + 2 * d3n2 * dx * dx;
+    else *m = d1n1 + d2n1 * dx + 2 * d3n1 * dx * dx;
+  } else {
+    if (std::fabs(d3n1) <= std::fabs(d30)) *m = d1n1 + d20 * dx - d3n1 * dx * dx;
+    else *m = d1n1 + d20 * dx - d30 * dx * dx;
+  }
+  if (std::fabs(d20) <= std::fabs(d2p1)) {
+    if (std::fabs(d3n1) <= std::fabs(d30)) *p = d10 - d20 * dx - d3n1 * dx * dx;
+    else *p = d10 - d20 * dx - d30 * dx * dx;
+  } else {
+    if (std::fabs(d30) <= std::fabs(d3p1)) *p = d10 - d2p1 * dx + 2 * d30 * dx * dx;
+    else *p = d10 - d2p1 * dx + 2 * d3p1 * dx * dx;
+  }
+}
+void eno3_advect_component(const PadG& g, const PadG& gu, const PadG& gv, bool xcomp, double h, double dt,
+                           double* out) {
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      double lx[7], ly[7], mx, px, my, py;
+      for (int k = 0; k < 7; ++k) {
+        lx[k] = g(i - 3 + k, j);
+        ly[k] = g(i, j - 3 + k);
+      }
+      eno3(lx, h, &mx, &px);
+      eno3(ly, h, &my, &py);
+      double uv, vv;
+      if (xcomp) {
+        uv = gu(i, j);
+        vv = (gv(i, j) + gv(i, j + 1) + gv(i - 1, j) + gv(i - 1, j + 1)) * 0.25;
+      } else {
+        uv = (gu(i, j - 1) + gu(i + 1, j - 1) + gu(i, j) + gu(i + 1, j)) * 0.25;
+        vv = gv(i, j);
+      }
+      const double dx = uv > 0 ? mx : px, dy = vv > 0 ? my : py;
+      out[(size_t)i + (size_t)j * g.nx] = g(i, j) - dt * (uv * dx + vv * dy);
+    }
+}
+}  // namespace
+
+void orc_advect_velocity(int nx, int ny, double h, double dt, const double* u, const double* v, double* u_out,
+                         double* v_out) {
+  const PadG gu = face_ghosts(nx + 1, ny, u), gv = face_ghosts(nx, ny + 1, v);
+  eno3_advect_component(gu, gu, gv, true, h, dt, u_out);
+  eno3_advect_component(gv, gu, gv, false, h, dt, v_out);
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -1,7 +1,7 @@
 """GPU parity of the level-set kernels against the oracle, through the C ABI.
 
 The re-initialisation (Sussman WENO5, 25 steps of dt = h/5), the WENO5
-advection stage and the normals run
+level-set and ENO3 velocity advection stages and the normals run
 in FP64 with the reference's operation order and no contraction on both sides
 (correctly rounded division and square root on both), so they must equal the
 oracle bit for bit.  At 2048^2 (oracle too slow) the checks are the properties
@@ -114,3 +114,27 @@ def test_advection_matches_oracle(P2D, O, W, case):
     t = torch.from_numpy(phi).cuda()
     ctx.advect_levelset(t, torch.from_numpy(u).cuda(), torch.from_numpy(v).cuda(), dt)
     assert np.array_equal(t.cpu().numpy(), ref)
+
+
+@pytest.mark.parametrize("case", ["rt", "drops64", "rect"])
+def test_velocity_advection_matches_oracle(P2D, O, W, case):
+    print(f"case velocity advection {case}", flush=True)
+    if case == "rect":
+        ny, nx = 64, 96
+        h = 1.0 / nx
+        rng = np.random.default_rng(9)
+        u = rng.uniform(-1, 1, (ny, nx + 1))
+        v = rng.uniform(-1, 1, (ny + 1, nx))
+    else:
+        w = W.make(case, 128)
+        nx = ny = 128
+        h, u, v = w.h, w.u, w.v
+    ctx = P2D.Context(nx, ny, h)
+    dt = 0.3 * h
+    gu, gv = ctx.advect_velocity(u, v, dt)
+    ru, rv = O.advect_velocity(u, v, h, dt)
+    assert np.array_equal(gu, ru) and np.array_equal(gv, rv), (np.abs(gu - ru).max(), np.abs(gv - rv).max())
+    import torch
+    tu, tv = torch.from_numpy(u).cuda(), torch.from_numpy(v).cuda()
+    ctx.advect_velocity(tu, tv, dt)
+    assert np.array_equal(tu.cpu().numpy(), ru) and np.array_equal(tv.cpu().numpy(), rv)

@@ -136,3 +136,49 @@ def test_advection_upwind_side_and_face_average(O):
     got = O.advect_levelset(phi, u, v, h, dt)
     assert np.abs(got - ref).max() <= 1e-15 * np.abs(phi).max() + 1e-16
     assert (uc > 0).any() and (uc < 0).any() and (vc > 0).any() and (vc < 0).any()
+
+
+def test_velocity_advection_closed_forms(O):
+    """ENO3 is exact on quadratics; under the uniform field u = U, v = V the stage is
+    u - dt (U u_x + V u_y) exactly, away from the ghost band; a shear u = y, v = 0
+    leaves u unchanged (u_x = 0, v = 0) and moves v by nothing (v = 0 everywhere)."""
+    n = 40
+    h = 1.0 / n
+    xf = np.arange(n + 1) * h
+    yc = (np.arange(n) + 0.5) * h
+    XU, YU = np.meshgrid(xf, yc)                      # x-faces (n, n+1)
+    for U, V in ((0.7, 0.3), (-0.7, 0.3), (0.7, -0.3), (-0.7, -0.3)):
+        u = np.full((n, n + 1), U)
+        v = np.full((n + 1, n), V)
+        uo, vo = O.advect_velocity(u, v, h, 0.2 * h)
+        assert np.abs(uo - U).max() <= 1e-15 and np.abs(vo - V).max() <= 1e-15
+    # a quadratic x-component advected by a uniform field given through v only
+    u = 0.3 + 0.2 * XU + 0.1 * YU ** 2
+    v = np.full((n + 1, n), 0.4)
+    uo, _ = O.advect_velocity(u, v, h, 0.2 * h)
+    ux, uy = 0.2, 0.2 * YU
+    ref = u - 0.2 * h * (u * ux + 0.4 * uy)
+    band = (slice(4, n - 4), slice(4, n - 3))
+    assert np.abs(uo - ref)[band].max() <= 1e-13
+
+
+def test_velocity_advection_eno_is_third_order(O):
+    """Derivative error of the stage on a smooth field: (u - u_out) / dt against
+    u u_x + v_c u_y at the x-faces, third order under refinement."""
+    errs = []
+    for n in (32, 64):
+        h = 1.0 / n
+        xf = np.arange(n + 1) * h
+        yc = (np.arange(n) + 0.5) * h
+        XU, YU = np.meshgrid(xf, yc)
+        f = lambda x, y: 0.5 + 0.2 * np.sin(2 * x + 1) * np.cos(3 * y)
+        u = f(XU, YU)
+        XV, YV = np.meshgrid((np.arange(n) + 0.5) * h, np.arange(n + 1) * h)
+        v = np.full_like(XV, 0.3)
+        dt = 1e-3 * h
+        uo, _ = O.advect_velocity(u, v, h, dt)
+        ux = 0.2 * 2 * np.cos(2 * XU + 1) * np.cos(3 * YU)
+        uy = -0.2 * 3 * np.sin(2 * XU + 1) * np.sin(3 * YU)
+        band = (slice(5, n - 5), slice(5, n - 4))
+        errs.append(np.abs((u - uo) / dt - (u * ux + 0.3 * uy))[band].max())
+    assert errs[0] / errs[1] > 2 ** 2.5, errs
