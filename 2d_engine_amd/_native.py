@@ -41,7 +41,7 @@ EXPORTS = [
     "p2d_set_coefficients", "p2d_project", "p2d_get_pressure", "p2d_get_coefficients", "p2d_adjoint",
     "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_kernel_timing",
     "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals", "p2d_advect_levelset",
-    "p2d_advect_velocity",
+    "p2d_advect_velocity", "p2d_apply_viscosity",
 ]
 
 
@@ -101,6 +101,7 @@ def _load():
         "p2d_levelset_normals": (i, [vp, vp, vp, vp, i]),
         "p2d_advect_levelset": (i, [vp, vp, vp, vp, d, d, i]),
         "p2d_advect_velocity": (i, [vp, vp, vp, d, i]),
+        "p2d_apply_viscosity": (i, [vp, vp, vp, d, d, d, d, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -315,6 +316,22 @@ class Context:
             vo = np.array(v, dtype=np.float64, order="C", copy=True)
             bu, bv = _Buf(uo, self.u_shape, writable=True), _Buf(vo, self.v_shape, writable=True)
         _check(lib.p2d_advect_velocity(self._h, bu.ptr, bv.ptr, float(dt), _mem(bu, bv)), "p2d_advect_velocity")
+        return uo, vo
+
+    def apply_viscosity(self, u, v, dt, mu_minus, mu_plus, epsilon=None):
+        """Explicit air-water viscosity step of the MAC velocity with the fluid of the last
+        set_fluid (VISCOSITY_2D.h:460-548); epsilon defaults to 1.5 h.  Torch CUDA tensors:
+        in place; numpy: new arrays."""
+        eps = 1.5 * self.h if epsilon is None else float(epsilon)
+        if _is_torch(u):
+            bu, bv = _Buf(u, self.u_shape, writable=True), _Buf(v, self.v_shape, writable=True)
+            uo, vo = u, v
+        else:
+            uo = np.array(u, dtype=np.float64, order="C", copy=True)
+            vo = np.array(v, dtype=np.float64, order="C", copy=True)
+            bu, bv = _Buf(uo, self.u_shape, writable=True), _Buf(vo, self.v_shape, writable=True)
+        _check(lib.p2d_apply_viscosity(self._h, bu.ptr, bv.ptr, float(dt), float(mu_minus), float(mu_plus), eps,
+                                       _mem(bu, bv)), "p2d_apply_viscosity")
         return uo, vo
 
     def normals(self, phi):

@@ -1029,6 +1029,27 @@ int p2d_advect_velocity(p2d_ctx* c, double* u, double* v, double dt, int mem) {
   return P2D_OK;
 }
 
+int p2d_apply_viscosity(p2d_ctx* c, double* u, double* v, double dt, double mu_minus, double mu_plus,
+                        double epsilon, int mem) {
+  // the densities come from p2d_set_fluid (explicit coefficients carry none)
+  if (!c || !u || !v || !c->have_fluid || c->explicit_coeff || !(epsilon > 0) || !(dt == dt) || c->nx < 2 ||
+      c->ny < 2)
+    return P2D_BAD_ARG;
+  CHK(stage_in(c, c->su, u, c->NU, mem));
+  CHK(stage_in(c, c->sv, v, c->NV, mem));
+  const dim3 gu(blocks_for((size_t)c->nx + 1), (unsigned)((c->ny + BAND - 1) / BAND));
+  const dim3 gv(blocks_for((size_t)c->nx), (unsigned)((c->ny + 1 + BAND - 1) / BAND));
+  k_viscosity<true><<<gu, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->phi, c->rho_m, c->rho_p, mu_minus, mu_plus,
+                                                 epsilon, c->su, c->sv, c->sa[0]);
+  k_viscosity<false><<<gv, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->phi, c->rho_m, c->rho_p, mu_minus,
+                                                  mu_plus, epsilon, c->su, c->sv, c->sa[1]);
+  CHK(launched_ok());
+  CHK(stage_out(c, u, c->sa[0], c->NU, mem));
+  CHK(stage_out(c, v, c->sa[1], c->NV, mem));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return P2D_OK;
+}
+
 int p2d_levelset_normals(p2d_ctx* c, const double* phi, double* normal_x, double* normal_y, int mem) {
   if (!c || !phi || !normal_x || !normal_y || c->nx < 2 || c->ny < 2) return P2D_BAD_ARG;
   CHK(stage_in(c, c->v64.x, phi, c->N, mem));

@@ -1,6 +1,6 @@
 // p2d level-set kernels: the Sussman re-initialisation (WENO5 one-sided
 // derivatives + Godunov Hamiltonian), the WENO5 level-set and ENO3 velocity
-// advection stages and unit normals.  FP64 with
+// advection stages, the explicit air-water viscosity update and unit normals.  FP64 with
 // contraction off: the arithmetic keeps the reference's operation order
 // (ADVECTION_METHOD_2D.h, LEVELSET_2D.h), so the results are reproducible
 // operation by operation on the CPU.
@@ -254,6 +254,68 @@ __global__ void __launch_bounds__(BLOCK) k_advect_vel(int nx, int ny, double h, 
     }
     const double dxv = uv > 0 ? mx : px, dyv = vv > 0 ? my : py;
     out[(size_t)i + (size_t)j * fnx] = g[b][a] - dt * (uv * dxv + vv * dyv);
+  }
+}
+
+// Mollified Heaviside of the viscosity step, VISCOSITY_2D::HeavisideFunction
+// (VISCOSITY_2D.h:688-707, the reference's PI = 3.141592).
+__device__ __forceinline__ double heaviside_mollified(double phi, double eps) {
+#pragma clang fp contract(off)
+  const double pi = 3.141592, oe = (double)1 / eps, op = (double)1 / pi;
+  if (phi < -eps) return 0;
+  if (phi > eps) return 1;
+  return (double)0.5 + phi * oe * (double)0.5 + (double)0.5 * op * sin(pi * phi * oe);
+}
+
+// Explicit air-water viscosity update of one MAC component from the old
+// velocity (u0, v0, 0 outside the grid), VISCOSITY_2D::ApplyViscosity
+// (VISCOSITY_2D.h:460-548): density and viscosities from the mollified
+// Heaviside of level-set averages (the face, its two corners across the
+// other direction, the two cells), value = minus + (plus - minus) H;
+// x-face: u += 2 coef/dx (mu_r (u_E - u) - mu_c (u - u_W))/dx
+//            + coef/dy (mu_cu (du/dy + dv/dx)_N - mu_cd (du/dy + dv/dx)_S), coef = dt/rho_half;
+// y-face the mirror image.  Every face, walls too.  One thread per face,
+// grid dim3(ceil(fnx/BLOCK), ceil(fny/BAND)).
+template <bool XCOMP>
+__global__ void __launch_bounds__(BLOCK) k_viscosity(int nx, int ny, double h, double dt, const double* __restrict__ phi,
+                                                     double rm, double rp, double mm, double mp, double eps,
+                                                     const double* __restrict__ u0, const double* __restrict__ v0,
+                                                     double* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int fnx = XCOMP ? nx + 1 : nx, fny = XCOMP ? ny : ny + 1;
+  const int i = (int)(blockIdx.x * BLOCK + threadIdx.x);
+  if (i >= fnx) return;
+  auto P = [&](int a, int b) { return phi_at(phi, nx, ny, a, b); };
+  auto U = [&](int a, int b) { return (a < 0 || a > nx || b < 0 || b >= ny) ? 0.0 : u0[fxi(a, b, nx)]; };
+  auto V = [&](int a, int b) { return (a < 0 || a >= nx || b < 0 || b > ny) ? 0.0 : v0[fyi(a, b, nx)]; };
+  auto rho = [&](double p) { return rm + (rp - rm) * heaviside_mollified(p, eps); };
+  auto mu = [&](double p) { return mm + (mp - mm) * heaviside_mollified(p, eps); };
+  const double ox = 1.0 / h, oy = 1.0 / h;
+  const int j1 = min(fny, (int)(blockIdx.y + 1) * BAND);
+  for (int j = (int)blockIdx.y * BAND; j < j1; ++j) {
+    if (XCOMP) {
+      const double ph = (double)0.5 * (P(i - 1, j) + P(i, j));
+      const double pu = (double)0.25 * (P(i - 1, j) + P(i, j) + P(i - 1, j + 1) + P(i, j + 1));
+      const double pd = (double)0.25 * (P(i - 1, j - 1) + P(i, j - 1) + P(i - 1, j) + P(i, j));
+      const double mr = mu(P(i, j)), mc = mu(P(i - 1, j)), mcu = mu(pu), mcd = mu(pd);
+      const double coef = dt * ((double)1 / rho(ph));
+      const double first =
+          (double)2 * coef * ox * (mr * ox * (U(i + 1, j) - U(i, j)) - mc * ox * (U(i, j) - U(i - 1, j)));
+      const double second = coef * oy * (mcu * (oy * (U(i, j + 1) - U(i, j)) + ox * (V(i, j + 1) - V(i - 1, j + 1))) -
+                                         mcd * (oy * (U(i, j) - U(i, j - 1)) + ox * (V(i, j) - V(i - 1, j))));
+      out[fxi(i, j, nx)] = U(i, j) + (first + second);
+    } else {
+      const double ph = (double)0.5 * (P(i, j - 1) + P(i, j));
+      const double pu = (double)0.25 * (P(i, j - 1) + P(i + 1, j - 1) + P(i, j) + P(i + 1, j));
+      const double pl = (double)0.25 * (P(i - 1, j - 1) + P(i, j - 1) + P(i - 1, j) + P(i, j));
+      const double mu_u = mu(P(i, j)), mc = mu(P(i, j - 1)), mcu = mu(pu), mcl = mu(pl);
+      const double coef = dt * ((double)1 / rho(ph));
+      const double first =
+          (double)2 * coef * oy * (mu_u * oy * (V(i, j + 1) - V(i, j)) - mc * oy * (V(i, j) - V(i, j - 1)));
+      const double second = coef * ox * (mcu * (oy * (U(i + 1, j) - U(i + 1, j - 1)) + ox * (V(i + 1, j) - V(i, j))) -
+                                         mcl * (oy * (U(i, j) - U(i, j - 1)) + ox * (V(i, j) - V(i - 1, j))));
+      out[fyi(i, j, nx)] = V(i, j) + (first + second);
+    }
   }
 }
 

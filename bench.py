@@ -107,9 +107,10 @@ def time_adjoint(ctx, torch, w, uv, steps, dev):
 
 
 def time_reinit(P2D, torch, ctx, w, dev, reps=5):
-    """Level-set kernels on the workload's level set (scaled by 2, so it is not a distance):
-    25 Sussman WENO5 re-initialisation steps (dt = h/5) timed like a projection, the step
-    kernel alone (p2d_time_kernel), and one WENO5 advection stage."""
+    """The kernels around the projection on the same workload: 25 Sussman WENO5
+    re-initialisation steps (dt = h/5) of the level set scaled by 2 (not a distance), timed
+    like a projection, and the step kernel alone (p2d_time_kernel); one WENO5 level-set
+    advection stage, one ENO3 velocity advection stage and one explicit viscosity step."""
     src = torch.from_numpy(2.0 * w.phi).to(dev)
     phi = src.clone()
     stream = torch.cuda.current_stream(dev)
@@ -153,15 +154,28 @@ def time_reinit(P2D, torch, ctx, w, dev, reps=5):
         e1.synchronize()
         if k:
             vel.append(e0.elapsed_time(e1))
+    visc = []
+    for k in range(reps + 1):
+        uu.copy_(u)
+        vv.copy_(v)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.apply_viscosity(uu, vv, 0.25 * w.h, 1.78e-5, 1.137e-3)
+        e1.record(stream)
+        e1.synchronize()
+        if k:
+            visc.append(e0.elapsed_time(e1))
     return {"reinit_ms_25_steps": round(statistics.median(times), 4),
             "reinit_step_avg_launch_ms": round(step_ms, 5), "reinit_step_bytes_per_cell": 24,
             "reinit_step_achieved_GBs": round(24 * N / (step_ms * 1e-3) / 1e9, 1),
             "advect_stage_ms": round(statistics.median(adv), 4),
             "velocity_advect_stage_ms": round(statistics.median(vel), 4),
+            "viscosity_step_ms": round(statistics.median(visc), 4),
             "accounting": "re-initialisation step: reads phi and the sign function, writes phi (24 B/cell, FP64), "
                           "~600 FP64 VALU instructions per cell (four WENO5 derivatives, Godunov); advection stage: "
                           "reads phi and the two face velocities, writes phi (32 B/cell), two WENO5 derivatives "
-                          "per cell, timed with its two device copies; velocity stage (ENO3, both components): reads u, v, writes u, v, timed with its four device copies"}
+                          "per cell, timed with its two device copies; velocity stage (ENO3, both components): reads u, v, writes u, v, timed with its four device copies; explicit viscosity step (air/water 1.78e-5/1.137e-3): reads phi, u, v, writes u, v, timed with its four device copies"}
 
 
 def read_traffic():

@@ -144,6 +144,16 @@ int p2d_advect_levelset(p2d_ctx* ctx, double* phi, const double* u, const double
  * Needs nx, ny >= 3. */
 int p2d_advect_velocity(p2d_ctx* ctx, double* u, double* v, double dt, int mem);
 
+/* Explicit viscosity step of the air-water delta-function formulation, in place on the MAC
+ * velocity u ((nx+1)*ny), v (nx*(ny+1)): VISCOSITY_2D::ApplyViscosity (VISCOSITY_2D.h:460-548,
+ * called at EULERIAN_FLUID_SOLVER_2D.h:2086-2089).  Uses the level set and the densities of the
+ * last p2d_set_fluid; mu_minus / mu_plus are the viscosities on the phi <= 0 / phi > 0 sides
+ * (air / water for air_bubble_rising) and epsilon the mollification width (the reference's
+ * epsilon_for_mollification = 1.5 dx, EULERIAN_FLUID_SOLVER_2D.h:347).  Every face is updated,
+ * walls included, as in the reference. */
+int p2d_apply_viscosity(p2d_ctx* ctx, double* u, double* v, double dt, double mu_minus, double mu_plus,
+                        double epsilon, int mem);
+
 /* Unit normals of the level set phi (nx*ny), LEVELSET_2D::ComputeNormals (LEVELSET_2D.h:593-620):
  * central differences over linear-extrapolation ghosts, forward differences where the central
  * gradient vanishes, (0, 0) where both vanish. */

@@ -79,6 +79,7 @@ def lib():
             "orc_reinit_sussman": (None, [_i, _i, _d, _pd, _i, _d, _d, _pd]),
             "orc_advect_levelset": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _d, _pd]),
             "orc_advect_velocity": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _pd]),
+            "orc_apply_viscosity": (None, [_i, _i, _d, _d, _pd, _d, _d, _d, _d, _d, _pd, _pd]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -174,6 +175,17 @@ def advect_velocity(u, v, h, dt):
     nx = nx1 - 1
     uo, vo = np.empty_like(u), np.empty_like(v)
     lib().orc_advect_velocity(nx, ny, h, dt, _p(u), _p(v), _p(uo), _p(vo))
+    return uo, vo
+
+
+def apply_viscosity(phi, u, v, h, dt, rho_minus, rho_plus, mu_minus, mu_plus, epsilon=None):
+    """Explicit air-water viscosity update of the MAC velocity (mollified density and
+    viscosity, width epsilon = 1.5 h by default); returns new arrays."""
+    phi = _f64(phi)
+    ny, nx = phi.shape
+    uo, vo = _f64(u).copy(), _f64(v).copy()
+    eps = 1.5 * h if epsilon is None else epsilon
+    lib().orc_apply_viscosity(nx, ny, h, dt, _p(phi), rho_minus, rho_plus, mu_minus, mu_plus, eps, _p(uo), _p(vo))
     return uo, vo
 
 

@@ -1416,6 +1416,63 @@ void orc_advect_velocity(int nx, int ny, double h, double dt, const double* u, c
   eno3_advect_component(gv, gu, gv, false, h, dt, v_out);
 }
 
+// Viscosity, air-water delta-function formulation: VISCOSITY_2D::ApplyViscosity
+// (VISCOSITY_2D.h:460-548, the threaded form the solver calls at
+// EULERIAN_FLUID_SOLVER_2D.h:2086-2089).  An explicit update of both MAC
+// components from the old velocities (temp copies whose ghosts are 0): at an
+// x-face, level-set averages at the face, its upper/lower corners and the two
+// cells; density and viscosities from the mollified Heaviside
+// (HeavisideFunction VISCOSITY_2D.h:688-707, PI = 3.141592, width epsilon;
+// value = minus + (plus - minus) H, VISCOSITY_2D.h:750-790, 834-871); then
+//   u += 2 coef/dx (mu_r/dx (u_E - u) - mu_c/dx (u - u_W))
+//      + coef/dy (mu_cu (du/dy_N + dv/dx_N) - mu_cd (du/dy_S + dv/dx_S)),  coef = dt/rho_half,
+// and the mirror formula at y-faces.  Every face is updated, walls too.  The
+// level set's ghosts are linear extrapolations (levelset_with_ghosts: the
+// reference leaves the four corner ghosts unfilled).
+namespace {
+double heaviside(double phi, double eps) {
+  const double oe = (double)1 / eps, op = (double)1 / REF_PI;
+  if (phi < -eps) return 0;
+  if (phi > eps) return 1;
+  return (double)0.5 + phi * oe * (double)0.5 + (double)0.5 * op * std::sin(REF_PI * phi * oe);
+}
+}  // namespace
+
+void orc_apply_viscosity(int nx, int ny, double h, double dt, const double* phi_in, double rho_minus,
+                         double rho_plus, double mu_minus, double mu_plus, double epsilon, double* u, double* v) {
+  const Pad phi = levelset_with_ghosts(nx, ny, phi_in);
+  const std::vector<double> u0(u, u + (size_t)(nx + 1) * ny), v0(v, v + (size_t)nx * (ny + 1));
+  auto U = [&](int i, int j) { return (i < 0 || i > nx || j < 0 || j >= ny) ? 0.0 : u0[FX(i, j, nx)]; };
+  auto V = [&](int i, int j) { return (i < 0 || i >= nx || j < 0 || j > ny) ? 0.0 : v0[FY(i, j, nx)]; };
+  auto rho = [&](double p) { return rho_minus + (rho_plus - rho_minus) * heaviside(p, epsilon); };
+  auto mu = [&](double p) { return mu_minus + (mu_plus - mu_minus) * heaviside(p, epsilon); };
+  const double ox = 1.0 / h, oy = 1.0 / h;
+  for (int j = 0; j < ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      const double ph = (double)0.5 * (phi(i - 1, j) + phi(i, j));
+      const double pu = (double)0.25 * (phi(i - 1, j) + phi(i, j) + phi(i - 1, j + 1) + phi(i, j + 1));
+      const double pd = (double)0.25 * (phi(i - 1, j - 1) + phi(i, j - 1) + phi(i - 1, j) + phi(i, j));
+      const double mr = mu(phi(i, j)), mc = mu(phi(i - 1, j)), mcu = mu(pu), mcd = mu(pd);
+      const double coef = dt * ((double)1 / rho(ph));
+      const double first = (double)2 * coef * ox * (mr * ox * (U(i + 1, j) - U(i, j)) - mc * ox * (U(i, j) - U(i - 1, j)));
+      const double second = coef * oy * (mcu * (oy * (U(i, j + 1) - U(i, j)) + ox * (V(i, j + 1) - V(i - 1, j + 1))) -
+                                         mcd * (oy * (U(i, j) - U(i, j - 1)) + ox * (V(i, j) - V(i - 1, j))));
+      u[FX(i, j, nx)] = u0[FX(i, j, nx)] + (first + second);
+    }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 0; i < nx; ++i) {
+      const double ph = (double)0.5 * (phi(i, j - 1) + phi(i, j));
+      const double pu = (double)0.25 * (phi(i, j - 1) + phi(i + 1, j - 1) + phi(i, j) + phi(i + 1, j));
+      const double pl = (double)0.25 * (phi(i - 1, j - 1) + phi(i, j - 1) + phi(i - 1, j) + phi(i, j));
+      const double mu_u = mu(phi(i, j)), mc = mu(phi(i, j - 1)), mcu = mu(pu), mcl = mu(pl);
+      const double coef = dt * ((double)1 / rho(ph));
+      const double first = (double)2 * coef * oy * (mu_u * oy * (V(i, j + 1) - V(i, j)) - mc * oy * (V(i, j) - V(i, j - 1)));
+      const double second = coef * ox * (mcu * (oy * (U(i + 1, j) - U(i + 1, j - 1)) + ox * (V(i + 1, j) - V(i, j))) -
+                                         mcl * (oy * (U(i, j) - U(i, j - 1)) + ox * (V(i, j) - V(i - 1, j))));
+      v[FY(i, j, nx)] = v0[FY(i, j, nx)] + (first + second);
+    }
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

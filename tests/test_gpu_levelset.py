@@ -138,3 +138,27 @@ def test_velocity_advection_matches_oracle(P2D, O, W, case):
     tu, tv = torch.from_numpy(u).cuda(), torch.from_numpy(v).cuda()
     ctx.advect_velocity(tu, tv, dt)
     assert np.array_equal(tu.cpu().numpy(), ru) and np.array_equal(tv.cpu().numpy(), rv)
+
+
+@pytest.mark.parametrize("case", ["circle", "drops64", "rt", "rect"])
+def test_viscosity_matches_oracle(P2D, O, W, case):
+    """Explicit air-water viscosity step (air 1.78e-5, water 1.137e-3, 2D_script.sim:338-342)."""
+    print(f"case viscosity {case}", flush=True)
+    if case == "rect":
+        phi, h = _circle_scaled(64, 96)
+        ny, nx = phi.shape
+        rng = np.random.default_rng(11)
+        u, v = rng.uniform(-1, 1, (ny, nx + 1)), rng.uniform(-1, 1, (ny + 1, nx))
+        rm, rp = 1.0, 1000.0
+    else:
+        w = W.make(case, 128)
+        phi, h, u, v, rm, rp = w.phi, w.h, w.u, w.v, w.rho_minus, w.rho_plus
+        ny, nx = phi.shape
+    ctx = P2D.Context(nx, ny, h)
+    ctx.set_fluid(phi, rm, rp, 0.0)
+    dt, mm, mp = 0.25 * h, 1.78e-5, 1.137e-3
+    gu, gv = ctx.apply_viscosity(u, v, dt, mm, mp)
+    ru, rv = O.apply_viscosity(phi, u, v, h, dt, rm, rp, mm, mp)
+    # the mollified Heaviside's sine may differ by an ulp between the GPU and the host
+    for g, r, a in ((gu, ru, u), (gv, rv, v)):
+        assert np.abs(g - r).max() <= 1e-13 * np.abs(r - a).max() + 2 * np.finfo(float).eps * np.abs(a).max()
