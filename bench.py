@@ -199,6 +199,20 @@ def cpu_baseline(w):
     base = {"value": 1.0 / dt, "unit": "projections/s", "cores": 1, "kind": "port",
             "sample": f"one full projection of the same {w.n}^2 RT workload by the oracle "
                       f"(serial FP64 MG-PCG to relative residual 1e-12, {st['iterations']} iterations), {dt:.1f} s"}
+    # the reference's live solver path (CSR assembly + serial IC(0)-PCG, PCG_METHOD.cpp:176-264),
+    # bounded to 5 and 20 iterations: per-iteration cost at this size (its full solve runs
+    # for thousands of iterations, far beyond a bench's budget)
+    t0 = time.perf_counter()
+    O.project(w.phi, w.rho_minus, w.rho_plus, w.sigma, w.u, w.v, w.h, solver=O.SOLVER_IC0, max_iteration=5)
+    t1 = time.perf_counter()
+    _, _, _, st20 = O.project(w.phi, w.rho_minus, w.rho_plus, w.sigma, w.u, w.v, w.h, solver=O.SOLVER_IC0,
+                              max_iteration=20)
+    t2 = time.perf_counter()
+    base["reference_path_sample"] = {
+        "solver": "oracle restatement of the reference's CSR assembly + serial IC(0)-PCG",
+        "seconds_5_iterations": round(t1 - t0, 3), "seconds_20_iterations": round(t2 - t1, 3),
+        "seconds_per_iteration": round((t2 - t1 - (t1 - t0)) / 15.0, 4),
+        "residual_after_20": float(st20["residual"])}
     return base, (p, u, v)
 
 
