@@ -41,7 +41,7 @@ EXPORTS = [
     "p2d_set_coefficients", "p2d_project", "p2d_get_pressure", "p2d_get_coefficients", "p2d_adjoint",
     "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_kernel_timing",
     "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals", "p2d_advect_levelset",
-    "p2d_advect_velocity", "p2d_apply_viscosity",
+    "p2d_advect_velocity", "p2d_apply_viscosity", "p2d_step_params_default", "p2d_advance",
 ]
 
 
@@ -65,6 +65,15 @@ class P2DStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class P2DStepParams(ctypes.Structure):
+    _fields_ = [
+        ("gravity_x", ctypes.c_double), ("gravity_y", ctypes.c_double),
+        ("rho_minus", ctypes.c_double), ("rho_plus", ctypes.c_double), ("sigma", ctypes.c_double),
+        ("mu_minus", ctypes.c_double), ("mu_plus", ctypes.c_double), ("mollification", ctypes.c_double),
+        ("weno_epsilon", ctypes.c_double), ("reinit_iterations", ctypes.c_int), ("reinit_scaling", ctypes.c_double),
+    ]
 
 
 class P2DError(RuntimeError):
@@ -102,6 +111,8 @@ def _load():
         "p2d_advect_levelset": (i, [vp, vp, vp, vp, d, d, i]),
         "p2d_advect_velocity": (i, [vp, vp, vp, d, i]),
         "p2d_apply_viscosity": (i, [vp, vp, vp, d, d, d, d, i]),
+        "p2d_step_params_default": (None, [ctypes.POINTER(P2DStepParams)]),
+        "p2d_advance": (i, [vp, vp, vp, vp, d, ctypes.POINTER(P2DStepParams), stp, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -333,6 +344,33 @@ class Context:
         _check(lib.p2d_apply_viscosity(self._h, bu.ptr, bv.ptr, float(dt), float(mu_minus), float(mu_plus), eps,
                                        _mem(bu, bv)), "p2d_apply_viscosity")
         return uo, vo
+
+    def advance(self, phi, u, v, dt, allow_not_converged=False, **params):
+        """One TVD-RK3 air-water time step (EULERIAN_FLUID_SOLVER_2D.h:1365-1797): level-set
+        advection and re-initialisation, then three velocity stages (advection, gravity,
+        viscosity, wall velocities, projection).  Keyword parameters are the fields of
+        p2d_step_params (gravity_x, gravity_y, rho_minus, rho_plus, sigma, mu_minus, mu_plus,
+        mollification, weno_epsilon, reinit_iterations, reinit_scaling).  Torch CUDA tensors are
+        advanced in place; numpy arrays are copied.  Returns (phi, u, v, stats of the last
+        projection)."""
+        prm = P2DStepParams()
+        lib.p2d_step_params_default(ctypes.byref(prm))
+        names = {k for k, _ in P2DStepParams._fields_}
+        for k, val in params.items():
+            if k not in names:
+                raise TypeError(f"unknown step parameter {k}")
+            setattr(prm, k, val)
+        if not _is_torch(phi):
+            phi = np.array(phi, dtype=np.float64, order="C", copy=True)
+            u = np.array(u, dtype=np.float64, order="C", copy=True)
+            v = np.array(v, dtype=np.float64, order="C", copy=True)
+        bp = _Buf(phi, self.cell_shape, writable=True)
+        bu, bv = _Buf(u, self.u_shape, writable=True), _Buf(v, self.v_shape, writable=True)
+        st = P2DStats()
+        allow = (P2D_OK, P2D_NOT_CONVERGED) if allow_not_converged else (P2D_OK,)
+        _check(lib.p2d_advance(self._h, bp.ptr, bu.ptr, bv.ptr, float(dt), ctypes.byref(prm), ctypes.byref(st),
+                               _mem(bp, bu, bv)), "p2d_advance", allow)
+        return phi, u, v, st.as_dict()
 
     def normals(self, phi):
         """Unit normals (nx, ny) of the level set phi (LEVELSET_2D.h:593-620)."""

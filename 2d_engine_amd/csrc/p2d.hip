@@ -120,6 +120,8 @@ struct p2d_ctx {
   double* muh[2] = {nullptr, nullptr};
   double* mul[2] = {nullptr, nullptr};
   double *su = nullptr, *sv = nullptr, *sg1 = nullptr, *sg2 = nullptr, *sp = nullptr;
+  // time-step state (allocated by the first p2d_advance): phi, u, v and their stage-start copies
+  double *st_phi = nullptr, *st_phin = nullptr, *st_u = nullptr, *st_v = nullptr, *st_un = nullptr, *st_vn = nullptr;
   double* sa[4] = {nullptr, nullptr, nullptr, nullptr};
   bool have_pressure = false;
   double last_dt = 1.0;
@@ -974,23 +976,174 @@ int p2d_mg_levels(p2d_ctx* c) { return c ? c->mg64.L : -1; }
 // Level-set re-initialisation and normals on the context's grid.  They use the
 // FP64 Krylov vectors x, z, q of the context as scratch (free between solves:
 // the pressure and the coefficients are kept elsewhere).
+// --- device-pointer building blocks of the level-set, advection and viscosity entry points
+// and of p2d_advance (enqueued on the context stream, no synchronisation)
+namespace {
+void ls_advect_stage(p2d_ctx* c, const double* phi, const double* u, const double* v, double dt, double eps,
+                     double* out) {
+  k_advect_ls<<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, eps, phi, u, v, out);
+}
+// `iterations` Sussman steps from phi, ping-ponging between phi and tmp (the sign array in
+// v64.q); returns the buffer that holds the result
+double* reinit_run(p2d_ctx* c, double* phi, double* tmp, int iterations, double scaling, double eps) {
+  k_reinit_sign<<<blocks_for(c->N), BLOCK, 0, c->stream>>>(c->N, c->h, phi, c->v64.q);
+  const double dt = c->h / scaling;
+  for (int it = 0; it < iterations; ++it) {
+    k_reinit_step<<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, eps, phi, c->v64.q, tmp);
+    std::swap(phi, tmp);
+  }
+  return phi;
+}
+// one ENO3 velocity advection stage (u, v) -> (uo, vo)
+void vel_advect_stage(p2d_ctx* c, const double* u, const double* v, double dt, double* uo, double* vo) {
+  const dim3 gu((unsigned)((c->nx + 1 + RI_W - 1) / RI_W), (unsigned)((c->ny + RI_H - 1) / RI_H));
+  const dim3 gv((unsigned)((c->nx + RI_W - 1) / RI_W), (unsigned)((c->ny + 1 + RI_H - 1) / RI_H));
+  k_advect_vel<true><<<gu, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, u, v, uo);
+  k_advect_vel<false><<<gv, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, u, v, vo);
+}
+// one explicit viscosity step (u, v) -> (uo, vo) with the context's fluid
+void viscosity_step(p2d_ctx* c, const double* u, const double* v, double dt, double mm, double mp, double eps,
+                    double* uo, double* vo) {
+  const dim3 gu(blocks_for((size_t)c->nx + 1), (unsigned)((c->ny + BAND - 1) / BAND));
+  const dim3 gv(blocks_for((size_t)c->nx), (unsigned)((c->ny + 1 + BAND - 1) / BAND));
+  k_viscosity<true><<<gu, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->phi, c->rho_m, c->rho_p, mm, mp, eps, u, v,
+                                                 uo);
+  k_viscosity<false><<<gv, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->phi, c->rho_m, c->rho_p, mm, mp, eps, u,
+                                                  v, vo);
+}
+void lincomb(p2d_ctx* c, size_t n, double a, const double* x, double b, const double* y, double* out) {
+  k_lincomb<<<blocks_for(n), BLOCK, 0, c->stream>>>(n, a, x, b, y, out);
+}
+}  // namespace
+
 int p2d_reinitialize(p2d_ctx* c, double* phi, int iterations, double scaling, double epsilon, int mem) {
   if (!c || !phi || iterations < 0 || !(scaling > 0) || !(epsilon >= 0) || c->nx < 3 || c->ny < 3)
     return P2D_BAD_ARG;
-  double* a = c->v64.x;
-  double* b = c->v64.z;
-  double* sg = c->v64.q;
-  CHK(stage_in(c, a, phi, c->N, mem));
-  k_reinit_sign<<<blocks_for(c->N), BLOCK, 0, c->stream>>>(c->N, c->h, a, sg);
-  const double dt = c->h / scaling;
-  for (int it = 0; it < iterations; ++it) {
-    k_reinit_step<<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, epsilon, a, sg, b);
-    std::swap(a, b);
-  }
+  CHK(stage_in(c, c->v64.x, phi, c->N, mem));
+  const double* r = reinit_run(c, c->v64.x, c->v64.z, iterations, scaling, epsilon);
   CHK(launched_ok());
-  CHK(stage_out(c, phi, a, c->N, mem));
+  CHK(stage_out(c, phi, r, c->N, mem));
   HIPCHK(hipStreamSynchronize(c->stream));
   return P2D_OK;
+}
+
+void p2d_step_params_default(p2d_step_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->gravity_x = 0.0;
+  p->gravity_y = -9.8;
+  p->rho_minus = 1.0;
+  p->rho_plus = 1000.0;
+  p->sigma = 0.0;
+  p->mu_minus = 0.0;
+  p->mu_plus = 0.0;
+  p->mollification = 0.0;
+  p->weno_epsilon = 1e-6;
+  p->reinit_iterations = 25;
+  p->reinit_scaling = 5.0;
+}
+
+// One time step of the air-water solver with third-order TVD Runge-Kutta,
+// EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread, order_for_time_advancing == 3,
+// no vortex sheet (EULERIAN_FLUID_SOLVER_2D.h:1365-1797):
+//   level set:  phi <- L(L(phi)); 0.75 phi_n + 0.25 phi; L; 1/3 phi_n + 2/3 phi
+//               (L = one WENO5 advection stage by the current velocity)
+//   re-init:    the same combination of three Reinitialization calls
+//   velocity:   the same combination of three SolveForRK stages, each
+//               ENO3 advection, gravity, viscosity (Sourcing), wall velocities 0,
+//               projection with the new level set's density and jump
+// Returns the last projection's statistics; P2D_NOT_CONVERGED if any did not converge.
+int p2d_advance(p2d_ctx* c, double* phi, double* u, double* v, double dt, const p2d_step_params* prm, p2d_stats* out,
+                int mem) {
+  if (!c || !phi || !u || !v || !prm || !(dt > 0) || c->nx < 3 || c->ny < 3 || !(prm->rho_minus > 0) ||
+      !(prm->rho_plus > 0) || prm->reinit_iterations < 0 || !(prm->reinit_scaling > 0) || !(prm->weno_epsilon >= 0))
+    return P2D_BAD_ARG;
+  if (!c->st_phi) {
+    CHK(dalloc(c, &c->st_phi, c->N));
+    CHK(dalloc(c, &c->st_phin, c->N));
+    CHK(dalloc(c, &c->st_u, c->NU));
+    CHK(dalloc(c, &c->st_v, c->NV));
+    CHK(dalloc(c, &c->st_un, c->NU));
+    CHK(dalloc(c, &c->st_vn, c->NV));
+  }
+  double *P = c->st_phi, *Pn = c->st_phin, *U = c->st_u, *V = c->st_v, *Un = c->st_un, *Vn = c->st_vn;
+  CHK(stage_in(c, P, phi, c->N, mem));
+  CHK(stage_in(c, U, u, c->NU, mem));
+  CHK(stage_in(c, V, v, c->NV, mem));
+  const double eps = prm->weno_epsilon;
+  const double third = 1 / (double)3, two_thirds = 2 / (double)3;
+  // level-set advection, TVD-RK3 (EULERIAN_FLUID_SOLVER_2D.h:1688-1716)
+  HIPCHK(hipMemcpyAsync(Pn, P, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  double* T1 = c->v64.x;
+  ls_advect_stage(c, P, U, V, dt, eps, T1);
+  ls_advect_stage(c, T1, U, V, dt, eps, P);
+  lincomb(c, c->N, 0.75, Pn, 0.25, P, P);
+  ls_advect_stage(c, P, U, V, dt, eps, T1);
+  lincomb(c, c->N, third, Pn, two_thirds, T1, P);
+  // re-initialisation, the same combination (:1718-1741)
+  if (prm->reinit_iterations > 0) {
+    const int it = prm->reinit_iterations;
+    const double sc = prm->reinit_scaling;
+    HIPCHK(hipMemcpyAsync(Pn, P, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    double* r = reinit_run(c, P, T1, it, sc, eps);
+    r = reinit_run(c, r, r == P ? T1 : P, it, sc, eps);
+    lincomb(c, c->N, 0.75, Pn, 0.25, r, P);
+    r = reinit_run(c, P, T1, it, sc, eps);
+    lincomb(c, c->N, third, Pn, two_thirds, r, P);
+  }
+  CHK(launched_ok());
+  // the projection's fluid: the new level set
+  CHK(p2d_set_fluid(c, P, prm->rho_minus, prm->rho_plus, prm->sigma, P2D_MEM_DEVICE));
+  const bool visc = prm->mu_minus != 0.0 || prm->mu_plus != 0.0;
+  const double moll = prm->mollification > 0 ? prm->mollification : 1.5 * c->h;
+  int status = P2D_OK;
+  p2d_stats st{};
+  // SolveForRK (:1857-1878) from the velocity in (cu, cv), alternating with (sa[0], sa[1]);
+  // (cu, cv) is left pointing at the result
+  double *cu = U, *cv = V;
+  auto other_u = [&](const double* p) { return p == U ? c->sa[0] : U; };
+  auto other_v = [&](const double* p) { return p == V ? c->sa[1] : V; };
+  auto stage = [&]() -> int {
+    double *nu = other_u(cu), *nv = other_v(cv);
+    vel_advect_stage(c, cu, cv, dt, nu, nv);
+    cu = nu;
+    cv = nv;
+    k_gravity<<<blocks_for(c->NU + c->NV), BLOCK, 0, c->stream>>>(c->NU, c->NV, dt * prm->gravity_x,
+                                                                   dt * prm->gravity_y, cu, cv);
+    if (visc) {
+      nu = other_u(cu);
+      nv = other_v(cv);
+      viscosity_step(c, cu, cv, dt, prm->mu_minus, prm->mu_plus, moll, nu, nv);
+      cu = nu;
+      cv = nv;
+    }
+    k_wall_zero<<<blocks_for((size_t)std::max(c->nx, c->ny)), BLOCK, 0, c->stream>>>(c->nx, c->ny, cu, cv);
+    CHK(launched_ok());
+    const int rc = p2d_project(c, cu, cv, dt, &st, P2D_MEM_DEVICE);
+    if (rc == P2D_NOT_CONVERGED) status = P2D_NOT_CONVERGED;
+    else if (rc != P2D_OK) return rc;
+    return P2D_OK;
+  };
+  // velocity, TVD-RK3 (:1743-1797)
+  HIPCHK(hipMemcpyAsync(Un, U, c->NU * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(Vn, V, c->NV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  CHK(stage());
+  CHK(stage());
+  lincomb(c, c->NU, 0.75, Un, 0.25, cu, U);
+  lincomb(c, c->NV, 0.75, Vn, 0.25, cv, V);
+  cu = U;
+  cv = V;
+  CHK(stage());
+  lincomb(c, c->NU, third, Un, two_thirds, cu, U);
+  lincomb(c, c->NV, third, Vn, two_thirds, cv, V);
+  CHK(launched_ok());
+  CHK(stage_out(c, phi, P, c->N, mem));
+  CHK(stage_out(c, u, U, c->NU, mem));
+  CHK(stage_out(c, v, V, c->NV, mem));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  st.status = status;
+  if (out) *out = st;
+  return status;
 }
 
 int p2d_advect_levelset(p2d_ctx* c, double* phi, const double* u, const double* v, double dt, double epsilon,
@@ -1005,8 +1158,7 @@ int p2d_advect_levelset(p2d_ctx* c, double* phi, const double* u, const double* 
     dv = c->sv;
   }
   CHK(stage_in(c, c->v64.x, phi, c->N, mem));
-  k_advect_ls<<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, epsilon, c->v64.x, du, dv,
-                                                            c->v64.z);
+  ls_advect_stage(c, c->v64.x, du, dv, dt, epsilon, c->v64.z);
   CHK(launched_ok());
   CHK(stage_out(c, phi, c->v64.z, c->N, mem));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1018,10 +1170,7 @@ int p2d_advect_velocity(p2d_ctx* c, double* u, double* v, double dt, int mem) {
   // old velocity in su/sv, the new components in sa[0]/sa[1], then back
   CHK(stage_in(c, c->su, u, c->NU, mem));
   CHK(stage_in(c, c->sv, v, c->NV, mem));
-  const dim3 gu((unsigned)((c->nx + 1 + RI_W - 1) / RI_W), (unsigned)((c->ny + RI_H - 1) / RI_H));
-  const dim3 gv((unsigned)((c->nx + RI_W - 1) / RI_W), (unsigned)((c->ny + 1 + RI_H - 1) / RI_H));
-  k_advect_vel<true><<<gu, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->su, c->sv, c->sa[0]);
-  k_advect_vel<false><<<gv, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->su, c->sv, c->sa[1]);
+  vel_advect_stage(c, c->su, c->sv, dt, c->sa[0], c->sa[1]);
   CHK(launched_ok());
   CHK(stage_out(c, u, c->sa[0], c->NU, mem));
   CHK(stage_out(c, v, c->sa[1], c->NV, mem));
@@ -1037,12 +1186,7 @@ int p2d_apply_viscosity(p2d_ctx* c, double* u, double* v, double dt, double mu_m
     return P2D_BAD_ARG;
   CHK(stage_in(c, c->su, u, c->NU, mem));
   CHK(stage_in(c, c->sv, v, c->NV, mem));
-  const dim3 gu(blocks_for((size_t)c->nx + 1), (unsigned)((c->ny + BAND - 1) / BAND));
-  const dim3 gv(blocks_for((size_t)c->nx), (unsigned)((c->ny + 1 + BAND - 1) / BAND));
-  k_viscosity<true><<<gu, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->phi, c->rho_m, c->rho_p, mu_minus, mu_plus,
-                                                 epsilon, c->su, c->sv, c->sa[0]);
-  k_viscosity<false><<<gv, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->phi, c->rho_m, c->rho_p, mu_minus,
-                                                  mu_plus, epsilon, c->su, c->sv, c->sa[1]);
+  viscosity_step(c, c->su, c->sv, dt, mu_minus, mu_plus, epsilon, c->sa[0], c->sa[1]);
   CHK(launched_ok());
   CHK(stage_out(c, u, c->sa[0], c->NU, mem));
   CHK(stage_out(c, v, c->sa[1], c->NV, mem));

@@ -319,6 +319,37 @@ __global__ void __launch_bounds__(BLOCK) k_viscosity(int nx, int ny, double h, d
   }
 }
 
+// Time-step helpers.  out = a x + b y (the TVD-RK3 stage combinations,
+// EULERIAN_FLUID_SOLVER_2D.h:1700-1795: 0.75 / 0.25 and 1/3 / 2/3).
+__global__ void __launch_bounds__(BLOCK) k_lincomb(size_t n, double a, const double* __restrict__ x, double b,
+                                                   const double* __restrict__ y, double* __restrict__ out) {
+#pragma clang fp contract(off)
+  for (size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x; k < n; k += (size_t)gridDim.x * BLOCK)
+    out[k] = a * x[k] + b * y[k];
+}
+// Gravity, EULERIAN_FLUID_SOLVER_2D::ApplyGraivity (EULERIAN_FLUID_SOLVER_2D.h:3249-3269):
+// every x-face += dt gx, every y-face += dt gy.
+__global__ void __launch_bounds__(BLOCK) k_gravity(size_t nu, size_t nv, double gxdt, double gydt, double* __restrict__ u,
+                                                   double* __restrict__ v) {
+  for (size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x; k < nu + nv; k += (size_t)gridDim.x * BLOCK) {
+    if (k < nu) u[k] += gxdt;
+    else v[k - nu] += gydt;
+  }
+}
+// Wall velocities to zero, SetupBoundaryConditionForVelocity (air_bubble_rising
+// branch, EULERIAN_FLUID_SOLVER_2D.h:2227-2270): u at i = 0, nx and v at j = 0, ny.
+__global__ void __launch_bounds__(BLOCK) k_wall_zero(int nx, int ny, double* __restrict__ u, double* __restrict__ v) {
+  const int t = (int)(blockIdx.x * BLOCK + threadIdx.x);
+  if (t < ny) {
+    u[fxi(0, t, nx)] = 0.0;
+    u[fxi(nx, t, nx)] = 0.0;
+  }
+  if (t < nx) {
+    v[fyi(t, 0, nx)] = 0.0;
+    v[fyi(t, ny, nx)] = 0.0;
+  }
+}
+
 // Unit normals, LEVELSET_2D::ComputeNormals (LEVELSET_2D.h:593-620): central
 // differences over the linear-extrapolation ghosts, one-sided forward
 // differences where the central gradient vanishes, scaled by 1/|n|

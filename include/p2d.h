@@ -154,6 +154,32 @@ int p2d_advect_velocity(p2d_ctx* ctx, double* u, double* v, double dt, int mem);
 int p2d_apply_viscosity(p2d_ctx* ctx, double* u, double* v, double dt, double mu_minus, double mu_plus,
                         double epsilon, int mem);
 
+/* Parameters of p2d_advance: the AIR_WATER_SIMULATION block of the reference's script
+ * (2D_script.sim:265-345) as the solver reads them. */
+typedef struct p2d_step_params {
+  double gravity_x, gravity_y;  /* ApplyGraivity: every face += dt g (default (0, -9.8)) */
+  double rho_minus, rho_plus;   /* densities where phi <= 0 / phi > 0 */
+  double sigma;                 /* surface tension of the projection's jump */
+  double mu_minus, mu_plus;     /* viscosities; both 0 = no viscosity step */
+  double mollification;         /* epsilon_for_mollification; <= 0: 1.5 h */
+  double weno_epsilon;          /* WENO epsilon of advection and re-initialisation (1e-6) */
+  int reinit_iterations;        /* Sussman steps per Reinitialization call (25); 0 = none */
+  double reinit_scaling;        /* scaling_number_for_reinitialization (5) */
+} p2d_step_params;
+
+void p2d_step_params_default(p2d_step_params* params);
+
+/* One time step of the air-water solver with third-order TVD Runge-Kutta, in place on phi
+ * (nx*ny), u ((nx+1)*ny), v (nx*(ny+1)): EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread
+ * with order_for_time_advancing = 3 (EULERIAN_FLUID_SOLVER_2D.h:1365-1797, no vortex sheet).
+ * Level set: three WENO5 advection stages (p2d_advect_levelset) combined 3/4-1/4 and 1/3-2/3;
+ * re-initialisation: three p2d_reinitialize calls combined the same way; velocity: three
+ * SolveForRK stages (ENO3 advection, gravity, viscosity, wall velocities 0, p2d_project with the
+ * new level set's fluid) combined the same way.  stats: the last projection's (status
+ * P2D_NOT_CONVERGED if any projection did not converge).  The caller chooses dt (CFL). */
+int p2d_advance(p2d_ctx* ctx, double* phi, double* u, double* v, double dt, const p2d_step_params* params,
+                p2d_stats* stats, int mem);
+
 /* Unit normals of the level set phi (nx*ny), LEVELSET_2D::ComputeNormals (LEVELSET_2D.h:593-620):
  * central differences over linear-extrapolation ghosts, forward differences where the central
  * gradient vanishes, (0, 0) where both vanish. */

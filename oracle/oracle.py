@@ -189,6 +189,56 @@ def apply_viscosity(phi, u, v, h, dt, rho_minus, rho_plus, mu_minus, mu_plus, ep
     return uo, vo
 
 
+def rk3(y, stage):
+    """The third-order TVD Runge-Kutta combination the solver uses for the level set, its
+    re-initialisation and the velocity (EULERIAN_FLUID_SOLVER_2D.h:1700-1795):
+    y1 = L(L(y)); y2 = 3/4 y + 1/4 y1; return 1/3 y + 2/3 L(y2).  y is an array or a tuple
+    of arrays; stage maps it to the same structure."""
+    tup = isinstance(y, tuple)
+    ys = y if tup else (y,)
+    comb = lambda a, x, b, z: tuple(a * xi + b * zi for xi, zi in zip(x, z))
+    as_t = lambda r: r if tup else (r,)
+    y1 = as_t(stage(stage(y)))
+    y2 = comb(0.75, ys, 0.25, y1)
+    y3 = as_t(stage(y2 if tup else y2[0]))
+    out = comb(1 / 3, ys, 2 / 3, y3)
+    return out if tup else out[0]
+
+
+def advance(phi, u, v, h, dt, gravity=(0.0, -9.8), rho_minus=1.0, rho_plus=1000.0, sigma=0.0, mu_minus=0.0,
+            mu_plus=0.0, mollification=None, weno_epsilon=1e-6, reinit_iterations=25, reinit_scaling=5.0,
+            solver=None, tol=1e-13):
+    """One TVD-RK3 air-water time step composed of the restated stages,
+    EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread with order 3 and no vortex sheet
+    (EULERIAN_FLUID_SOLVER_2D.h:1688-1797): rk3 of the level-set advection stage by the
+    step's initial velocity, rk3 of the Reinitialization call, then rk3 of SolveForRK
+    (EULERIAN_FLUID_SOLVER_2D.h:1857-1877: ENO3 advection, ApplyGraivity :3249-3269,
+    viscosity, wall velocities 0 :2227-2260, PROJECTION_2D::Solve with the new level set).
+    Returns (phi, u, v)."""
+    u0, v0 = _f64(u).copy(), _f64(v).copy()
+    p = rk3(_f64(phi).copy(), lambda q: advect_levelset(q, u0, v0, h, dt, weno_epsilon))
+    if reinit_iterations > 0:
+        p = rk3(p, lambda q: reinit_sussman(q, h, reinit_iterations, reinit_scaling, weno_epsilon))
+    eps = 1.5 * h if mollification is None or mollification <= 0 else mollification
+    solver = SOLVER_IC0 if solver is None else solver
+
+    def stage(uv):
+        a, b = advect_velocity(uv[0], uv[1], h, dt)
+        a = a + dt * gravity[0]
+        b = b + dt * gravity[1]
+        if mu_minus != 0.0 or mu_plus != 0.0:
+            a, b = apply_viscosity(p, a, b, h, dt, rho_minus, rho_plus, mu_minus, mu_plus, eps)
+        a[:, 0] = 0.0
+        a[:, -1] = 0.0
+        b[0, :] = 0.0
+        b[-1, :] = 0.0
+        _, a, b, _ = project(p, rho_minus, rho_plus, sigma, a, b, h, dt, solver=solver, tol=tol)
+        return a, b
+
+    a, b = rk3((u0, v0), stage)
+    return p, a, b
+
+
 def face_beta(phi, rho_minus, rho_plus):
     phi = _f64(phi)
     ny, nx = phi.shape
