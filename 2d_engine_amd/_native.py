@@ -42,6 +42,7 @@ EXPORTS = [
     "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_kernel_timing",
     "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals", "p2d_advect_levelset",
     "p2d_advect_velocity", "p2d_apply_viscosity", "p2d_step_params_default", "p2d_advance",
+    "p2d_cfl_dt",
 ]
 
 
@@ -113,6 +114,7 @@ def _load():
         "p2d_apply_viscosity": (i, [vp, vp, vp, d, d, d, d, i]),
         "p2d_step_params_default": (None, [ctypes.POINTER(P2DStepParams)]),
         "p2d_advance": (i, [vp, vp, vp, vp, d, ctypes.POINTER(P2DStepParams), stp, i]),
+        "p2d_cfl_dt": (i, [vp, vp, vp, vp, ctypes.POINTER(P2DStepParams), d, ctypes.POINTER(d), i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -345,6 +347,29 @@ class Context:
                                        _mem(bu, bv)), "p2d_apply_viscosity")
         return uo, vo
 
+    @staticmethod
+    def _step_params(params):
+        prm = P2DStepParams()
+        lib.p2d_step_params_default(ctypes.byref(prm))
+        names = {k for k, _ in P2DStepParams._fields_}
+        for k, val in params.items():
+            if k not in names:
+                raise TypeError(f"unknown step parameter {k}")
+            setattr(prm, k, val)
+        return prm
+
+    def cfl_dt(self, phi, u, v, cfl_number=0.5, **params):
+        """The CFL time step (EULERIAN_FLUID_SOLVER_2D.h:3517-3637); 0.0 when no term
+        restricts it.  phi may be None when sigma is 0."""
+        prm = self._step_params(params)
+        bu, bv = _Buf(u, self.u_shape), _Buf(v, self.v_shape)
+        bp = _Buf(phi, self.cell_shape) if phi is not None else None
+        out = ctypes.c_double(0.0)
+        bufs = (bu, bv) if bp is None else (bp, bu, bv)
+        _check(lib.p2d_cfl_dt(self._h, bp.ptr if bp else None, bu.ptr, bv.ptr, ctypes.byref(prm), float(cfl_number),
+                              ctypes.byref(out), _mem(*bufs)), "p2d_cfl_dt")
+        return out.value
+
     def advance(self, phi, u, v, dt, allow_not_converged=False, **params):
         """One TVD-RK3 air-water time step (EULERIAN_FLUID_SOLVER_2D.h:1365-1797): level-set
         advection and re-initialisation, then three velocity stages (advection, gravity,
@@ -353,13 +378,7 @@ class Context:
         mollification, weno_epsilon, reinit_iterations, reinit_scaling).  Torch CUDA tensors are
         advanced in place; numpy arrays are copied.  Returns (phi, u, v, stats of the last
         projection)."""
-        prm = P2DStepParams()
-        lib.p2d_step_params_default(ctypes.byref(prm))
-        names = {k for k, _ in P2DStepParams._fields_}
-        for k, val in params.items():
-            if k not in names:
-                raise TypeError(f"unknown step parameter {k}")
-            setattr(prm, k, val)
+        prm = self._step_params(params)
         if not _is_torch(phi):
             phi = np.array(phi, dtype=np.float64, order="C", copy=True)
             u = np.array(u, dtype=np.float64, order="C", copy=True)

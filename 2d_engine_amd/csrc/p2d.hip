@@ -1027,6 +1027,49 @@ int p2d_reinitialize(p2d_ctx* c, double* phi, int iterations, double scaling, do
   return P2D_OK;
 }
 
+// EULERIAN_FLUID_SOLVER_2D::CFLOneTimeStep for the air-water solver
+// (EULERIAN_FLUID_SOLVER_2D.h:3517-3637): advection c = max|u|/h + max|v|/h, gravity
+// g = sqrt(9.8/h), viscosity w = max(mu-/rho-, mu+/rho+)(2/h^2 + 2/h^2), surface tension
+// s = sqrt(sigma max|kappa| / (min(rho) h^2)); dt = cfl / (0.5 (c + w + sqrt((c + w)^2 + 4 g^2
+// + 4 s^2))), 0 when the denominator is 0.  The curvature is that of phi (the level set the
+// last projection used); phi may be NULL when sigma is 0.
+int p2d_cfl_dt(p2d_ctx* c, const double* phi, const double* u, const double* v, const p2d_step_params* prm,
+               double cfl_number, double* dt, int mem) {
+  if (!c || !u || !v || !prm || !dt || (!phi && prm->sigma != 0.0) || !(prm->rho_minus > 0) ||
+      !(prm->rho_plus > 0) || c->nx < 2 || c->ny < 2)
+    return P2D_BAD_ARG;
+  const double* du = u;
+  const double* dv = v;
+  const double* dp = prm->sigma != 0.0 ? phi : nullptr;
+  if (mem == P2D_MEM_HOST) {
+    CHK(stage_in(c, c->su, u, c->NU, mem));
+    CHK(stage_in(c, c->sv, v, c->NV, mem));
+    du = c->su;
+    dv = c->sv;
+    if (dp) {
+      CHK(stage_in(c, c->v64.x, phi, c->N, mem));
+      dp = c->v64.x;
+    }
+  }
+  c->hs->err = 0;
+  HIPCHK(hipMemsetAsync(&c->s->err, 0, sizeof(int), c->stream));
+  k_cfl_max<<<bgrid(c->nx + 1, c->ny + 1), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dp, du, dv, c->partial, c->s);
+  k_fin_store<3><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx + 1, c->ny + 1)), 7u, 0,
+                                                      c->s);
+  CHK(launched_ok());
+  CHK(sync_scal(c));
+  if (c->hs->err) return P2D_NAN;
+  const double umax = c->hs->red[0], vmax = c->hs->red[1], kmax = c->hs->red[2];
+  const double ih = 1.0 / c->h, ih2 = 1.0 / (c->h * c->h);
+  const double cf = umax * ih + vmax * ih;
+  const double gf = std::sqrt(9.8 * ih);
+  const double vf = std::max(prm->mu_plus / prm->rho_plus, prm->mu_minus / prm->rho_minus) * (2 * ih2 + 2 * ih2);
+  const double sf = std::sqrt(prm->sigma * kmax / (std::min(prm->rho_plus, prm->rho_minus) * (c->h * c->h)));
+  const double cfl = 0.5 * (cf + vf + std::sqrt((cf + vf) * (cf + vf) + 4 * (gf * gf) + 4 * (sf * sf)));
+  *dt = cfl == 0 ? 0.0 : cfl_number / cfl;
+  return P2D_OK;
+}
+
 void p2d_step_params_default(p2d_step_params* p) {
   if (!p) return;
   std::memset(p, 0, sizeof(*p));

@@ -284,6 +284,25 @@ __global__ void __launch_bounds__(BLOCK) k_diagnostics(int nx, int ny, double h,
   block_partial<3>(acc, 7u, partial);
 }
 
+// The maxima the time-step restriction needs, EULERIAN_FLUID_SOLVER_2D::CFLOneTimeStep
+// (EULERIAN_FLUID_SOLVER_2D.h:3517-3637): max|u| over every x-face, max|v| over every
+// y-face and, when phi is given, max|curvature| over the cells; block partials for
+// k_fin_store<3>.  A vanishing level-set gradient sets s->err, as in k_coeff.
+__global__ void __launch_bounds__(BLOCK) k_cfl_max(int nx, int ny, double h, const double* __restrict__ phi,
+                                                   const double* __restrict__ u, const double* __restrict__ v,
+                                                   double* partial, Scal* s) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  bool bad = false;
+  P2D_BAND(ny + 1);
+  for (int j = j0; j < j1 && i <= nx; ++j) {
+    if (j < ny) acc[0] = fmax(acc[0], fabs(u[fxi(i, j, nx)]));
+    if (i < nx) acc[1] = fmax(acc[1], fabs(v[fyi(i, j, nx)]));
+    if (phi && i < nx && j < ny) acc[2] = fmax(acc[2], fabs(curvature_at(phi, nx, ny, h, i, j, &bad)));
+  }
+  if (bad) s->err = 1;
+  block_partial<3>(acc, 7u, partial);
+}
+
 // Sum of p = hi + lo into s->red[3] (for the mean); 1-D grid-stride launch.
 __global__ void __launch_bounds__(BLOCK) k_psum(size_t N, const double* __restrict__ ph, const double* __restrict__ pl,
                                                 double* partial, Scal* s) {

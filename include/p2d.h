@@ -169,6 +169,17 @@ typedef struct p2d_step_params {
 
 void p2d_step_params_default(p2d_step_params* params);
 
+/* The time step the CFL rule allows for the velocity (u, v) and level set phi:
+ * EULERIAN_FLUID_SOLVER_2D::CFLOneTimeStep for the air-water solver
+ * (EULERIAN_FLUID_SOLVER_2D.h:3517-3637; SIMULATION_WORLD::DetermineTimeStep calls it before
+ * every step).  Advection max|u|/h + max|v|/h, gravity sqrt(9.8/h), viscosity
+ * max(mu/rho) (4/h^2), surface tension sqrt(sigma max|curvature| / (min rho h^2)), combined as
+ * cfl / (0.5 (c + w + sqrt((c + w)^2 + 4 g^2 + 4 s^2))).  *dt = 0 when that denominator is 0
+ * (the caller then keeps its own dt).  phi may be NULL when params->sigma is 0.
+ * P2D_NAN when the curvature is undefined (a vanishing gradient). */
+int p2d_cfl_dt(p2d_ctx* ctx, const double* phi, const double* u, const double* v, const p2d_step_params* params,
+               double cfl_number, double* dt, int mem);
+
 /* One time step of the air-water solver with third-order TVD Runge-Kutta, in place on phi
  * (nx*ny), u ((nx+1)*ny), v (nx*(ny+1)): EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread
  * with order_for_time_advancing = 3 (EULERIAN_FLUID_SOLVER_2D.h:1365-1797, no vortex sheet).

@@ -189,6 +189,20 @@ def apply_viscosity(phi, u, v, h, dt, rho_minus, rho_plus, mu_minus, mu_plus, ep
     return uo, vo
 
 
+def cfl_dt(phi, u, v, h, cfl_number=0.5, rho_minus=1.0, rho_plus=1000.0, sigma=0.0, mu_minus=0.0, mu_plus=0.0):
+    """EULERIAN_FLUID_SOLVER_2D::CFLOneTimeStep, air-water branch (EULERIAN_FLUID_SOLVER_2D.h:
+    3517-3637): advection c_f, gravity g_f = sqrt(9.8/dy), viscosity v_f, surface tension s_f
+    from the largest |curvature|, CFL = (c_f + v_f + sqrt((c_f + v_f)^2 + 4 g_f^2 + 4 s_f^2))/2,
+    dt = cfl_number / CFL (0 when CFL is 0)."""
+    c_f = np.abs(u).max() * (1.0 / h) + np.abs(v).max() * (1.0 / h)
+    g_f = np.sqrt(9.8 * (1.0 / h))
+    v_f = max(mu_plus / rho_plus, mu_minus / rho_minus) * (2 * (1.0 / (h * h)) + 2 * (1.0 / (h * h)))
+    kmax = np.abs(curvature(phi, h)).max() if sigma != 0.0 else 0.0
+    s_f = np.sqrt(sigma * kmax / (min(rho_plus, rho_minus) * (h * h)))
+    cfl = 0.5 * (c_f + v_f + np.sqrt((c_f + v_f) ** 2 + 4 * g_f ** 2 + 4 * s_f ** 2))
+    return 0.0 if cfl == 0 else float(cfl_number / cfl)
+
+
 def rk3(y, stage):
     """The third-order TVD Runge-Kutta combination the solver uses for the level set, its
     re-initialisation and the velocity (EULERIAN_FLUID_SOLVER_2D.h:1700-1795):

@@ -72,3 +72,32 @@ def test_moving_fluid_step_is_divergence_free_and_conserves_walls(O, W):
     assert np.all(a[:, 0] == 0) and np.all(a[:, -1] == 0) and np.all(b[0] == 0) and np.all(b[-1] == 0)
     # the bubble stays a bubble: its area (phi < 0 cells) changes by a few cells at most
     assert abs(int((p < 0).sum()) - int((w.phi < 0).sum())) <= 0.05 * int((w.phi < 0).sum())
+
+
+def test_cfl_closed_forms(O, W):
+    """CFLOneTimeStep: at rest only gravity restricts, dt = cfl / sqrt(9.8/h); a uniform
+    velocity adds c_f = (|u| + |v|)/h to the quadratic's root; the viscous and surface-tension
+    terms enter through the same root; a circle's curvature is 1/R."""
+    n = 64
+    h = 1.0 / n
+    phi = np.zeros((n, n)) + 1.0
+    u, v = np.zeros((n, n + 1)), np.zeros((n + 1, n))
+    assert abs(O.cfl_dt(phi, u, v, h, 0.5) - 0.5 / np.sqrt(9.8 / h)) <= 1e-15
+    u[:] = 0.3
+    v[:] = -0.2
+    c, g = 0.5 / h, np.sqrt(9.8 / h)
+    want = 0.5 / (0.5 * (c + np.sqrt(c * c + 4 * g * g)))
+    assert abs(O.cfl_dt(phi, u, v, h, 0.5) - want) <= 1e-15 * want
+    # viscosity: w = max(mu/rho) * 4/h^2
+    w = max(1e-3 / 1000.0, 2e-5 / 1.0) * 4 / h ** 2
+    want = 0.5 / (0.5 * (c + w + np.sqrt((c + w) ** 2 + 4 * g * g)))
+    got = O.cfl_dt(phi, u, v, h, 0.5, rho_minus=1.0, rho_plus=1000.0, mu_minus=2e-5, mu_plus=1e-3)
+    assert abs(got - want) <= 1e-14 * want
+    # surface tension on a circle of radius 0.25: max|kappa| is the clamp or close to 1/R
+    wc = W.circle(n)
+    kmax = np.abs(O.curvature(wc.phi, h)).max()
+    assert 1 / 0.3 <= kmax <= 1 / h
+    s = np.sqrt(0.0728 * kmax / (1.0 * h * h))
+    want = 0.5 / (0.5 * (c + np.sqrt(c * c + 4 * g * g + 4 * s * s)))
+    got = O.cfl_dt(wc.phi, u, v, h, 0.5, sigma=0.0728)
+    assert abs(got - want) <= 1e-14 * want
