@@ -10,6 +10,6 @@ try:  # torch's HIP runtime, when present, must be the first one loaded (see _na
 except ImportError:
     pass
 from . import workloads  # noqa: F401
-from ._native import (FP64, MIXED, PREC_F32, PREC_F64, TIME_REINIT_STEP, Context, P2DError, lib,  # noqa: F401
-                      status_string)
+from ._native import (FP64, MIXED, PREC_F32, PREC_F64, TIME_REINIT_STEP, VISC_DIRICHLET, VISC_REFERENCE,  # noqa: F401
+                      Context, P2DError, ViscositySolver, lib, status_string)
 from .projection import PROJECTION_2D  # noqa: F401

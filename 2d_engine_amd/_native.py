@@ -42,7 +42,8 @@ EXPORTS = [
     "p2d_apply_stencil", "p2d_vcycle", "p2d_mg_levels", "p2d_time_kernel", "p2d_kernel_timing",
     "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals", "p2d_advect_levelset",
     "p2d_advect_velocity", "p2d_apply_viscosity", "p2d_step_params_default", "p2d_advance",
-    "p2d_cfl_dt",
+    "p2d_cfl_dt", "p2d_visc_create", "p2d_visc_destroy", "p2d_visc_set_stream", "p2d_visc_solve",
+    "p2d_oil_water_params_default", "p2d_visc_oil_water",
 ]
 
 
@@ -74,6 +75,15 @@ class P2DStepParams(ctypes.Structure):
         ("rho_minus", ctypes.c_double), ("rho_plus", ctypes.c_double), ("sigma", ctypes.c_double),
         ("mu_minus", ctypes.c_double), ("mu_plus", ctypes.c_double), ("mollification", ctypes.c_double),
         ("weno_epsilon", ctypes.c_double), ("reinit_iterations", ctypes.c_int), ("reinit_scaling", ctypes.c_double),
+    ]
+
+
+class P2DOilWaterParams(ctypes.Structure):
+    _fields_ = [
+        ("x_min", ctypes.c_double), ("Re", ctypes.c_double), ("rho_oil", ctypes.c_double),
+        ("rho_water", ctypes.c_double), ("mu_oil", ctypes.c_double), ("mu_water", ctypes.c_double),
+        ("mollification", ctypes.c_double), ("tolerance", ctypes.c_double), ("max_iterations", ctypes.c_int),
+        ("boundary", ctypes.c_int),
     ]
 
 
@@ -115,6 +125,12 @@ def _load():
         "p2d_step_params_default": (None, [ctypes.POINTER(P2DStepParams)]),
         "p2d_advance": (i, [vp, vp, vp, vp, d, ctypes.POINTER(P2DStepParams), stp, i]),
         "p2d_cfl_dt": (i, [vp, vp, vp, vp, ctypes.POINTER(P2DStepParams), d, ctypes.POINTER(d), i]),
+        "p2d_visc_create": (i, [i, i, d, d, ctypes.POINTER(vp)]),
+        "p2d_visc_destroy": (i, [vp]),
+        "p2d_visc_set_stream": (i, [vp, vp]),
+        "p2d_visc_solve": (i, [vp, i, vp, vp, vp, vp, vp, vp, vp, i, d, i, stp, i]),
+        "p2d_oil_water_params_default": (None, [ctypes.POINTER(P2DOilWaterParams)]),
+        "p2d_visc_oil_water": (i, [vp, vp, vp, vp, d, ctypes.POINTER(P2DOilWaterParams), stp, stp, i]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -420,3 +436,63 @@ class Context:
         ms = ctypes.c_double()
         _check(lib.p2d_time_kernel(self._h, which, reps, ctypes.byref(ms)), "p2d_time_kernel")
         return ms.value
+
+
+VISC_REFERENCE, VISC_DIRICHLET = 0, 1
+
+
+class ViscositySolver:
+    """The semi-implicit viscosity solve (POISSON_SOLVER::SolveForViscosity,
+    POISSON_SOLVER.cpp:175-184, 620-712) and the oil-water driver around it
+    (VISCOSITY_2D.h:546-640) for an nx x ny cell grid with spacings dx, dy."""
+
+    def __init__(self, nx, ny, dx, dy):
+        self.nx, self.ny, self.dx, self.dy = nx, ny, dx, dy
+        h = ctypes.c_void_p()
+        _check(lib.p2d_visc_create(nx, ny, dx, dy, ctypes.byref(h)), "p2d_visc_create")
+        self._h = h
+        self.u_shape, self.v_shape, self.cell_shape = (ny, nx + 1), (ny + 1, nx), (ny, nx)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.p2d_visc_destroy(h)
+            self._h = None
+
+    def set_stream(self, stream):
+        _check(lib.p2d_visc_set_stream(self._h, stream), "p2d_visc_set_stream")
+
+    def solve(self, component, velocity, c1, c2, c3, c4, c5, explicit_term, boundary=VISC_REFERENCE,
+              tolerance=1e-6, max_iterations=10000, allow_not_converged=False):
+        """Returns (solution, stats); torch CUDA tensors are solved in place."""
+        shape = self.u_shape if component == 0 else self.v_shape
+        if not _is_torch(velocity):
+            velocity = np.array(velocity, dtype=np.float64, order="C", copy=True)
+        bx = _Buf(velocity, shape, writable=True)
+        bc = [_Buf(a, shape) for a in (c1, c2, c3, c4, c5, explicit_term)]
+        st = P2DStats()
+        allow = (P2D_OK, P2D_NOT_CONVERGED) if allow_not_converged else (P2D_OK,)
+        _check(lib.p2d_visc_solve(self._h, component, bx.ptr, *[b.ptr for b in bc], boundary, float(tolerance),
+                                  int(max_iterations), ctypes.byref(st), _mem(bx, *bc)), "p2d_visc_solve", allow)
+        return velocity, st.as_dict()
+
+    def oil_water(self, phi, u, v, dt, allow_not_converged=False, **params):
+        """VISCOSITY_2D::ApplyViscosity, oil-water, semi-implicit, dimensionless.  Keyword
+        parameters: the fields of p2d_oil_water_params.  Returns (u, v, stats_x, stats_y)."""
+        prm = P2DOilWaterParams()
+        lib.p2d_oil_water_params_default(ctypes.byref(prm))
+        names = {k for k, _ in P2DOilWaterParams._fields_}
+        for k, val in params.items():
+            if k not in names:
+                raise TypeError(f"unknown oil-water parameter {k}")
+            setattr(prm, k, val)
+        if not _is_torch(u):
+            u = np.array(u, dtype=np.float64, order="C", copy=True)
+            v = np.array(v, dtype=np.float64, order="C", copy=True)
+        bp = _Buf(phi, self.cell_shape)
+        bu, bv = _Buf(u, self.u_shape, writable=True), _Buf(v, self.v_shape, writable=True)
+        sx, sy = P2DStats(), P2DStats()
+        allow = (P2D_OK, P2D_NOT_CONVERGED) if allow_not_converged else (P2D_OK,)
+        _check(lib.p2d_visc_oil_water(self._h, bp.ptr, bu.ptr, bv.ptr, float(dt), ctypes.byref(prm), ctypes.byref(sx),
+                                      ctypes.byref(sy), _mem(bp, bu, bv)), "p2d_visc_oil_water", allow)
+        return u, v, sx.as_dict(), sy.as_dict()

@@ -28,6 +28,7 @@
 #include "p2d_field.hpp"
 #include "p2d_mg.hpp"
 #include "p2d_levelset.hpp"
+#include "p2d_visc.hpp"
 
 using namespace p2d;
 
@@ -1339,3 +1340,212 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
 }
 
 }  // extern "C"
+
+// --------------------------------------------------------------------------- semi-implicit viscosity
+// One grid's solver state for p2d_visc_*: coefficient, stencil and BiCGStab vectors sized for
+// the larger MAC component, staging for host arrays, the device scalars and their pinned mirror.
+struct p2d_visc {
+  int nx = 0, ny = 0;
+  double dx = 0, dy = 0;
+  size_t nmax = 0;
+  hipStream_t stream = nullptr, own = nullptr;
+  std::vector<void*> allocs;
+  double* c[5] = {};
+  double *ex = nullptr, *dg = nullptr, *cw = nullptr, *ce = nullptr, *cs = nullptr, *cn = nullptr;
+  double *x = nullptr, *r = nullptr, *rhat = nullptr, *p = nullptr, *v = nullptr, *y = nullptr, *sv = nullptr,
+         *z = nullptr, *t = nullptr;
+  double *uo = nullptr, *hphi = nullptr, *hu = nullptr, *hv = nullptr, *partial = nullptr;
+  VScal* s = nullptr;
+  VScal* hs = nullptr;
+};
+
+namespace {
+int valloc(p2d_visc* V, double** p, size_t n) {
+  void* q = nullptr;
+  HIPCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(double)));
+  HIPCHK(hipMemset(q, 0, std::max<size_t>(n, 1) * sizeof(double)));
+  V->allocs.push_back(q);
+  *p = (double*)q;
+  return P2D_OK;
+}
+int vcopy(p2d_visc* V, double* dst, const double* src, size_t n, hipMemcpyKind kind) {
+  HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), kind, V->stream));
+  return P2D_OK;
+}
+// Solves the component system for V->x (initial guess in V->x) with the coefficients in
+// V->c[0..4] and the right-hand side in V->ex.
+int visc_solve_device(p2d_visc* V, int comp, int mode, double tolerance, int max_iterations, p2d_stats* st) {
+  const int nxc = comp == 0 ? V->nx + 1 : V->nx, nyc = comp == 0 ? V->ny : V->ny + 1;
+  const int n = nxc * nyc;
+  const unsigned G = gs_blocks((size_t)n);
+  const double inv_dxdx = 1.0 / (V->dx * V->dx), inv_dydy = 1.0 / (V->dy * V->dy);
+  hipStream_t q = V->stream;
+  k_visc_rows<<<G, BLOCK, 0, q>>>(nxc, nyc, inv_dxdx, inv_dydy, mode, V->c[0], V->c[1], V->c[2], V->c[3], V->c[4],
+                                  V->dg, V->cw, V->ce, V->cs, V->cn);
+  std::memset(V->hs, 0, sizeof(VScal));
+  V->hs->tol2 = tolerance * tolerance;
+  V->hs->max_iter = max_iterations;
+  HIPCHK(hipMemcpyAsync(V->s, V->hs, sizeof(VScal), hipMemcpyHostToDevice, q));
+  const VStencil A{V->dg, V->cw, V->ce, V->cs, V->cn, nxc, nyc, mode};
+  k_bs_init<<<G, BLOCK, 0, q>>>(A, V->ex, V->x, V->r, V->rhat, V->p, V->v, V->partial);
+  k_bs_fin_init<<<1, FINISH_THREADS, 0, q>>>(V->partial, (int)G, V->s);
+  CHK(launched_ok());
+  int chunk = 4;
+  for (;;) {
+    HIPCHK(hipMemcpyAsync(V->hs, V->s, sizeof(VScal), hipMemcpyDeviceToHost, q));
+    HIPCHK(hipStreamSynchronize(q));
+    if (V->hs->done) break;
+    for (int k = 0; k < chunk; ++k) {
+      k_bs_dir<<<G, BLOCK, 0, q>>>(n, V->r, V->p, V->v, V->dg, V->y, V->s);
+      k_bs_apply<false><<<G, BLOCK, 0, q>>>(A, V->y, V->v, V->rhat, V->partial, V->s);
+      k_bs_fin_alpha<<<1, FINISH_THREADS, 0, q>>>(V->partial, (int)G, V->s);
+      k_bs_half<<<G, BLOCK, 0, q>>>(n, V->r, V->v, V->dg, V->sv, V->z, V->s);
+      k_bs_apply<true><<<G, BLOCK, 0, q>>>(A, V->z, V->t, V->sv, V->partial, V->s);
+      k_bs_fin_omega<<<1, FINISH_THREADS, 0, q>>>(V->partial, (int)G, V->s);
+      k_bs_update<<<G, BLOCK, 0, q>>>(n, V->x, V->y, V->z, V->sv, V->t, V->r, V->rhat, V->partial, V->s);
+      k_bs_fin_update<<<1, FINISH_THREADS, 0, q>>>(V->partial, (int)G, V->s);
+    }
+    CHK(launched_ok());
+    chunk = std::min(chunk * 2, 64);
+  }
+  const VScal& h = *V->hs;
+  int status = P2D_OK;
+  if (h.flag) status = P2D_NAN;
+  else if (!(h.r2 < h.tol2)) status = P2D_NOT_CONVERGED;
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->status = status;
+    st->iterations = h.iter;
+    st->outer = 1;
+    st->rel_residual = h.b2 > 0 ? std::sqrt(h.r2 / h.b2) : 0.0;
+    st->max_abs_residual = std::sqrt(h.r2);
+  }
+  return status;
+}
+}  // namespace
+
+int p2d_visc_create(int nx, int ny, double dx, double dy, p2d_visc** out) {
+  if (!out) return P2D_BAD_ARG;
+  *out = nullptr;
+  if (nx < 2 || ny < 2 || !(dx > 0) || !(dy > 0) || (long)(nx + 1) * (ny + 1) > (1L << 30)) return P2D_BAD_ARG;
+  p2d_visc* V = new p2d_visc();
+  V->nx = nx;
+  V->ny = ny;
+  V->dx = dx;
+  V->dy = dy;
+  V->nmax = std::max((size_t)(nx + 1) * ny, (size_t)nx * (ny + 1));
+  auto fail = [&](int rc) {
+    p2d_visc_destroy(V);
+    return rc;
+  };
+  if (hipStreamCreate(&V->own) != hipSuccess) return fail(P2D_HIP_ERROR);
+  V->stream = V->own;
+  const size_t M = V->nmax;
+  double** vecs[] = {&V->c[0], &V->c[1], &V->c[2], &V->c[3], &V->c[4], &V->ex, &V->dg, &V->cw, &V->ce, &V->cs,
+                     &V->cn, &V->x, &V->r, &V->rhat, &V->p, &V->v, &V->y, &V->sv, &V->z, &V->t, &V->uo, &V->hu,
+                     &V->hv};
+  for (double** a : vecs) {
+    const int rc = valloc(V, a, M);
+    if (rc != P2D_OK) return fail(rc);
+  }
+  int rc = valloc(V, &V->hphi, (size_t)nx * ny);
+  if (rc == P2D_OK) rc = valloc(V, &V->partial, 3 * (size_t)GRIDSTRIDE_BLOCKS);
+  if (rc != P2D_OK) return fail(rc);
+  if (hipMalloc((void**)&V->s, sizeof(VScal)) != hipSuccess) return fail(P2D_HIP_ERROR);
+  V->allocs.push_back(V->s);
+  if (hipHostMalloc((void**)&V->hs, sizeof(VScal), hipHostMallocDefault) != hipSuccess) return fail(P2D_HIP_ERROR);
+  *out = V;
+  return P2D_OK;
+}
+
+int p2d_visc_destroy(p2d_visc* V) {
+  if (!V) return P2D_BAD_ARG;
+  if (V->stream) (void)hipStreamSynchronize(V->stream);
+  for (void* p : V->allocs) (void)hipFree(p);
+  if (V->hs) (void)hipHostFree(V->hs);
+  if (V->own) (void)hipStreamDestroy(V->own);
+  delete V;
+  return P2D_OK;
+}
+
+int p2d_visc_set_stream(p2d_visc* V, void* stream) {
+  if (!V) return P2D_BAD_ARG;
+  HIPCHK(hipStreamSynchronize(V->stream));
+  V->stream = stream ? (hipStream_t)stream : V->own;
+  return P2D_OK;
+}
+
+int p2d_visc_solve(p2d_visc* V, int component, double* velocity, const double* c1, const double* c2,
+                   const double* c3, const double* c4, const double* c5, const double* explicit_term, int boundary,
+                   double tolerance, int max_iterations, p2d_stats* stats, int mem) {
+  if (!V || !velocity || !c1 || !c2 || !c3 || !c4 || !c5 || !explicit_term || (component != 0 && component != 1) ||
+      (boundary != P2D_VISC_REFERENCE && boundary != P2D_VISC_DIRICHLET) || !(tolerance >= 0) || max_iterations < 0)
+    return P2D_BAD_ARG;
+  const size_t n = component == 0 ? (size_t)(V->nx + 1) * V->ny : (size_t)V->nx * (V->ny + 1);
+  const hipMemcpyKind in = mem == P2D_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  const double* cs[5] = {c1, c2, c3, c4, c5};
+  for (int k = 0; k < 5; ++k) CHK(vcopy(V, V->c[k], cs[k], n, in));
+  CHK(vcopy(V, V->ex, explicit_term, n, in));
+  CHK(vcopy(V, V->x, velocity, n, in));
+  const int rc = visc_solve_device(V, component, boundary == P2D_VISC_REFERENCE ? VISC_LITERAL : VISC_DIRICHLET,
+                                   tolerance, max_iterations, stats);
+  if (rc != P2D_OK && rc != P2D_NOT_CONVERGED) return rc;
+  CHK(vcopy(V, velocity, V->x, n, mem == P2D_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice));
+  HIPCHK(hipStreamSynchronize(V->stream));
+  return rc;
+}
+
+void p2d_oil_water_params_default(p2d_oil_water_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->x_min = 0.0;
+  p->Re = 0.94983;
+  p->rho_oil = 0.905;
+  p->rho_water = 0.995;
+  p->mu_oil = 6.01;
+  p->mu_water = 0.01;
+  p->mollification = 0.0;
+  p->tolerance = 1e-6;
+  p->max_iterations = 10000;
+  p->boundary = P2D_VISC_REFERENCE;
+}
+
+// VISCOSITY_2D::ApplyViscosity for the oil-water solver with semi_implicit_approach and
+// dimensionless_form (VISCOSITY_2D.h:546-640): the x component's coefficients and explicit
+// term from the level set and the current (u, v), its solve, then the y component's from
+// the level set, v and u before and after the x solve, and its solve.
+int p2d_visc_oil_water(p2d_visc* V, const double* phi, double* u, double* v, double dt,
+                       const p2d_oil_water_params* prm, p2d_stats* stats_x, p2d_stats* stats_y, int mem) {
+  if (!V || !phi || !u || !v || !prm || !(dt > 0) || !(prm->Re > 0) || !(prm->rho_water > 0) ||
+      !(prm->mu_water > 0) || !(prm->tolerance >= 0) || prm->max_iterations < 0 ||
+      (prm->boundary != P2D_VISC_REFERENCE && prm->boundary != P2D_VISC_DIRICHLET))
+    return P2D_BAD_ARG;
+  const int nx = V->nx, ny = V->ny;
+  const size_t NU = (size_t)(nx + 1) * ny, NV = (size_t)nx * (ny + 1), N = (size_t)nx * ny;
+  const hipMemcpyKind in = mem == P2D_MEM_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+  const hipMemcpyKind back = mem == P2D_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  CHK(vcopy(V, V->hphi, phi, N, in));
+  CHK(vcopy(V, V->hu, u, NU, in));
+  CHK(vcopy(V, V->hv, v, NV, in));
+  CHK(vcopy(V, V->uo, V->hu, NU, hipMemcpyDeviceToDevice));
+  const OilWater o{V->dx, V->dy, prm->x_min, dt, prm->Re, prm->rho_oil, prm->rho_water, prm->mu_oil, prm->mu_water,
+                   prm->mollification > 0 ? prm->mollification : 1.5 * V->dx};
+  const int mode = prm->boundary == P2D_VISC_REFERENCE ? VISC_LITERAL : VISC_DIRICHLET;
+  k_ow_coef_x<<<gs_blocks(NU), BLOCK, 0, V->stream>>>(nx, ny, o, V->hphi, V->hu, V->hv, V->c[0], V->c[1], V->c[2],
+                                                       V->c[3], V->c[4], V->ex);
+  CHK(vcopy(V, V->x, V->hu, NU, hipMemcpyDeviceToDevice));
+  CHK(launched_ok());
+  int rx = visc_solve_device(V, 0, mode, prm->tolerance, prm->max_iterations, stats_x);
+  if (rx != P2D_OK && rx != P2D_NOT_CONVERGED) return rx;
+  CHK(vcopy(V, V->hu, V->x, NU, hipMemcpyDeviceToDevice));
+  k_ow_coef_y<<<gs_blocks(NV), BLOCK, 0, V->stream>>>(nx, ny, o, V->hphi, V->uo, V->hu, V->hv, V->c[0], V->c[1],
+                                                       V->c[2], V->c[3], V->c[4], V->ex);
+  CHK(vcopy(V, V->x, V->hv, NV, hipMemcpyDeviceToDevice));
+  CHK(launched_ok());
+  int ry = visc_solve_device(V, 1, mode, prm->tolerance, prm->max_iterations, stats_y);
+  if (ry != P2D_OK && ry != P2D_NOT_CONVERGED) return ry;
+  CHK(vcopy(V, u, V->hu, NU, back));
+  CHK(vcopy(V, v, V->x, NV, back));
+  HIPCHK(hipStreamSynchronize(V->stream));
+  return (rx == P2D_OK && ry == P2D_OK) ? P2D_OK : P2D_NOT_CONVERGED;
+}

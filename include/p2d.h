@@ -227,6 +227,60 @@ int p2d_kernel_time(p2d_ctx* ctx, int which, double* total_ms, int* launches);
 
 const char* p2d_status_string(int status);
 
+/* ---------------------------------------------------------------------------------------
+ * Semi-implicit viscosity: the linear solve of POISSON_SOLVER::SolveForViscosity and the
+ * oil-water driver around it.  A p2d_visc object serves one grid of nx*ny cells with spacings
+ * dx, dy (the oil-water grids need neither square cells nor multigrid-friendly sizes, so it
+ * is separate from p2d_ctx).  Component 0 is the x-face grid ((nx+1)*ny), 1 the y-face grid
+ * (nx*(ny+1)), row-major, i fastest. */
+typedef struct p2d_visc p2d_visc;
+
+enum {
+  P2D_VISC_REFERENCE = 0, /* out-of-grid neighbours couple to node 0, as the reference's CSR does */
+  P2D_VISC_DIRICHLET = 1  /* out-of-grid neighbours are zero velocity */
+};
+
+int p2d_visc_create(int nx, int ny, double dx, double dy, p2d_visc** out);
+int p2d_visc_destroy(p2d_visc* visc);
+int p2d_visc_set_stream(p2d_visc* visc, void* hip_stream);
+
+/* POISSON_SOLVER::SolveForViscosity(velocity, coef_1..coef_5, bc, explicit_term)
+ * (POISSON_SOLVER.cpp:175-184) with the system of
+ * BuildLinearSystemNodeForSemiImplicitViscosity (POISSON_SOLVER.cpp:620-712): at node (i, j)
+ *   (1 + (c1 + c2 + c3 + c4)/dx^2 + c5) x - c2/dx^2 x_W - c1/dx^2 x_E - c4/dy^2 x_S - c3/dy^2 x_N
+ *     = explicit_term.
+ * The reference's boundary-condition field keeps 0 = BC_FULL in its ghost ring (VISCOSITY_2D.h:
+ * 647-663 visits only the grid's own nodes), so an out-of-grid neighbour is unknown 0 there:
+ * boundary = P2D_VISC_REFERENCE reproduces that, P2D_VISC_DIRICHLET takes zero velocity.
+ * velocity: initial guess in, solution out.  The system is not symmetric; it is solved by
+ * Jacobi-preconditioned BiCGStab in FP64 to the reference's rule ||r||_2 < tolerance (the
+ * reference runs CG).  stats: iterations, rel_residual = ||r||/||b||, max_abs_residual = ||r||_2.
+ * P2D_NOT_CONVERGED after max_iterations; P2D_NAN on a breakdown. */
+int p2d_visc_solve(p2d_visc* visc, int component, double* velocity, const double* c1, const double* c2,
+                   const double* c3, const double* c4, const double* c5, const double* explicit_term,
+                   int boundary, double tolerance, int max_iterations, p2d_stats* stats, int mem);
+
+/* The OIL_WATER_SIMULATION values VISCOSITY_2D reads (2D_script.sim:345-433). */
+typedef struct p2d_oil_water_params {
+  double x_min;          /* x of the first cell centre: the radius r of the axisymmetric form */
+  double Re;             /* Reynolds number (0.94983) */
+  double rho_oil, rho_water, mu_oil, mu_water; /* 0.905, 0.995, 6.01, 0.01 */
+  double mollification;  /* epsilon_for_mollification; <= 0: 1.5 dx */
+  double tolerance;      /* PROJECTION tolerance (1e-6), absolute on ||r||_2 */
+  int max_iterations;    /* PROJECTION max_iteration (10000) */
+  int boundary;          /* P2D_VISC_REFERENCE (default) or P2D_VISC_DIRICHLET */
+} p2d_oil_water_params;
+
+void p2d_oil_water_params_default(p2d_oil_water_params* params);
+
+/* VISCOSITY_2D::ApplyViscosity(epsilon, dt) for oil_water_simulation with
+ * semi_implicit_approach and dimensionless_form (VISCOSITY_2D.h:546-640): mollified
+ * dimensionless density and viscosity, the five coefficient fields and the explicit term of
+ * the x component, its solve, then the y component's (whose explicit term reads u before and
+ * after the x solve) and its solve; u and v in place.  phi: nx*ny cells. */
+int p2d_visc_oil_water(p2d_visc* visc, const double* phi, double* u, double* v, double dt,
+                       const p2d_oil_water_params* params, p2d_stats* stats_x, p2d_stats* stats_y, int mem);
+
 #ifdef __cplusplus
 }
 #endif
