@@ -211,6 +211,46 @@ def time_step(P2D, torch, w, dev, reps=3):
                      "25 re-initialisation steps per call, mixed-precision projections to max|D u| <= 1e-10"}
 
 
+def time_semi_implicit(P2D, torch, n, dev, reps=3):
+    """The oil-water semi-implicit viscosity step (p2d_visc_oil_water: coefficients, then a
+    BiCGStab solve per MAC component) on an n x n vertical-pipe grid at the reference script's
+    properties (r in [0.02, 1.28], z in [0, 3.14], Re 0.94983, oil/water viscosity 601:1,
+    dt 1e-4, tolerance 1e-6 absolute), device-resident, timed with events."""
+    import numpy as np
+    dx, dy = 1.26 / n, 3.14 / n
+    x0 = 0.02 + 0.5 * dx
+    xs = x0 + np.arange(n) * dx
+    ys = (np.arange(n) + 0.5) * dy
+    phi = np.ascontiguousarray(xs[None, :] - (0.7 + 0.05 * np.sin(2 * np.pi * ys[:, None] / 3.14)))
+    rng = np.random.default_rng(5)
+    u, v = rng.uniform(-1, 1, (n, n + 1)), rng.uniform(-1, 1, (n + 1, n))
+    vs = P2D.ViscositySolver(n, n, dx, dy)
+    stream = torch.cuda.current_stream(dev)
+    vs.set_stream(stream.cuda_stream)
+    tp = torch.from_numpy(phi).to(dev)
+    src = [torch.from_numpy(a).to(dev) for a in (u, v)]
+    cur = [a.clone() for a in src]
+    kw = dict(x_min=x0, Re=0.94983, rho_oil=0.905, rho_water=0.995, mu_oil=6.01, mu_water=0.01)
+    times, sx, sy = [], None, None
+    for k in range(reps + 1):
+        for a, b in zip(cur, src):
+            a.copy_(b)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        _, _, sx, sy = vs.oil_water(tp, cur[0], cur[1], 1e-4, allow_not_converged=True, **kw)
+        e1.record(stream)
+        e1.synchronize()
+        if k:
+            times.append(e0.elapsed_time(e1))
+    ms = statistics.median(times)
+    its = sx["iterations"] + sy["iterations"]
+    return {"step_ms": round(ms, 3), "iterations_x": sx["iterations"], "iterations_y": sy["iterations"],
+            "residual_x": sx["max_abs_residual"], "residual_y": sy["max_abs_residual"],
+            "ms_per_iteration": round(ms / max(its, 1), 4),
+            "setup": f"vertical pipe {n}x{n}, oil-water semi-implicit, dimensionless, reference boundary coupling"}
+
+
 def read_traffic():
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, or None."""
     try:
@@ -297,6 +337,7 @@ def main():
                          "accounting": what}
     levelset = time_reinit(P2D, torch, ctx_m, w, dev)
     step = time_step(P2D, torch, w, dev)
+    semi = time_semi_implicit(P2D, torch, w.n, dev)
     dom = kernels[DOMINANT]
     traffic = read_traffic()
     roofline = {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -319,6 +360,7 @@ def main():
         "kernels": kernels,
         "levelset": levelset,
         "time_step": step,
+        "viscosity_semi_implicit": semi,
     }
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, serial)")
