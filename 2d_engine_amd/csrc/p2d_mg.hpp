@@ -162,23 +162,47 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_bottom_inverse(int nx, int
       if (j + 1 < ny) row[c + nx] -= C.N(i, j);
     }
     __syncthreads();
-    // in-place inversion: pivot row scaled, rank-1 update off row/column k,
-    // column k = -a_ik / a_kk, a_kk = 1 / a_kk
+    // Gauss-Jordan with the matrix in registers (thread t owns entries t + m SMALL_THREADS):
+    // step k needs the old pivot row and column, which their owners publish in LDS
+    // (double-buffered: one barrier per step).  New values: a_kk = 1/a_kk,
+    // a_kc = a_kc / a_kk, a_rk = -a_rk / a_kk, a_rc -= a_rk (a_kc / a_kk).
+    constexpr int OWN = BOTTOM_LDS_MAX * BOTTOM_LDS_MAX / SMALL_THREADS;
+    static_assert(OWN * SMALL_THREADS == BOTTOM_LDS_MAX * BOTTOM_LDS_MAX, "entries per thread");
+    __shared__ double prow[2][BOTTOM_LDS_MAX], pcol[2][BOTTOM_LDS_MAX];
+    double e[OWN];
+#pragma unroll
+    for (int m = 0; m < OWN; ++m) {
+      const int q = tid + m * SMALL_THREADS;
+      e[m] = q < n * n ? a[q] : 0.0;
+      if (q < n * n && q / n == 0) prow[0][q % n] = e[m];
+      if (q < n * n && q % n == 0) pcol[0][q / n] = e[m];
+    }
+    __syncthreads();
     for (int k = 0; k < n; ++k) {
-      const double d = 1.0 / a[k * n + k];
-      for (int j = tid; j < n; j += SMALL_THREADS)
-        if (j != k) a[k * n + j] *= d;
-      __syncthreads();
-      for (int q = tid; q < n * n; q += SMALL_THREADS) {
+      const int b = k & 1, nb = b ^ 1;
+      const double d = 1.0 / prow[b][k];
+#pragma unroll
+      for (int m = 0; m < OWN; ++m) {
+        const int q = tid + m * SMALL_THREADS;
+        if (q >= n * n) continue;
         const int r = q / n, c = q % n;
-        if (r != k && c != k) a[q] -= a[r * n + k] * a[k * n + c];
+        double v;
+        if (r == k && c == k) v = d;
+        else if (r == k) v = e[m] * d;
+        else if (c == k) v = e[m] * -d;
+        else v = e[m] - pcol[b][r] * (prow[b][c] * d);
+        e[m] = v;
+        if (r == k + 1) prow[nb][c] = v;
+        if (c == k + 1) pcol[nb][r] = v;
       }
       __syncthreads();
-      for (int r = tid; r < n; r += SMALL_THREADS)
-        if (r != k) a[r * n + k] *= -d;
-      if (tid == 0) a[k * n + k] = d;
-      __syncthreads();
     }
+#pragma unroll
+    for (int m = 0; m < OWN; ++m) {
+      const int q = tid + m * SMALL_THREADS;
+      if (q < n * n) a[q] = e[m];
+    }
+    __syncthreads();
     for (int q = tid; q < n * n; q += SMALL_THREADS) {
       const double v = a[q] - corr;
       inv64[q] = v;
