@@ -286,7 +286,51 @@ def cpu_baseline(w):
         "seconds_5_iterations": round(t1 - t0, 3), "seconds_20_iterations": round(t2 - t1, 3),
         "seconds_per_iteration": round((t2 - t1 - (t1 - t0)) / 15.0, 4),
         "residual_after_20": float(st20["residual"])}
+
+    # the reference's other solvers on the same system, bounded the same way: serial CG
+    # (CG_METHOD.cpp:13-73) and Jacobi-PCG on one and on all host threads
+    # (PCG_METHOD.cpp:266-383, the threaded path)
+    def per_iteration(solver, threads=0):
+        a = time.perf_counter()
+        O.project(w.phi, w.rho_minus, w.rho_plus, w.sigma, w.u, w.v, w.h, solver=solver, max_iteration=5,
+                  threads=threads)
+        b = time.perf_counter()
+        O.project(w.phi, w.rho_minus, w.rho_plus, w.sigma, w.u, w.v, w.h, solver=solver, max_iteration=25,
+                  threads=threads)
+        c = time.perf_counter()
+        return round((c - b - (b - a)) / 20.0, 4)
+
+    all_threads = host_threads()
+    base["other_reference_solvers"] = {
+        "cg_serial_seconds_per_iteration": per_iteration(O.SOLVER_CG, 1),
+        "jacobi_pcg_1_thread_seconds_per_iteration": per_iteration(O.SOLVER_JACOBI, 1),
+        f"jacobi_pcg_{all_threads}_threads_seconds_per_iteration": per_iteration(O.SOLVER_JACOBI, all_threads),
+        "how": "oracle restatements, 25 minus 5 bounded iterations (CSR assembly cancels), same workload"}
+    base["host"] = {"nproc": os.cpu_count(), "threads_used_for_all_core": all_threads, "cpu": cpu_model()}
     return base, (p, u, v)
+
+
+def host_threads():
+    """The host threads this process may use: the affinity mask, capped by OMP_NUM_THREADS."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
