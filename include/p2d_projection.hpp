@@ -11,12 +11,14 @@
  *   air_bubble_rising / water_drop :259-321         which density lies on the minus side (phi <= 0)
  *   water_density, air_density, surface_tension     the same
  *   Solve(dt) :235-242, Solve(dt, thread_id) :244-257  Solve(dt, phi, u, v)
- *   DeterminePressure() :480-509 + the update      inside Solve (one C ABI call, p2d_project)
+ *   DeterminePressure() :480-509 + the update      inside Solve (one C ABI call, p2d_project);
+ *                                                   DeterminePressure alone on a copy of u, v
  *   pressure_field, max_velocity_x/y                the same (pressure_field is mean-zero)
  *
  * Arrays are row-major, i fastest, without ghost cells: phi and pressure
- * nx*ny, u (nx+1)*ny, v nx*(ny+1).  INTEGRATION.md shows the copy from the
- * reference's ghosted FIELD_STRUCTURE_2D arrays.  Errors are thrown as
+ * nx*ny, u (nx+1)*ny, v nx*(ny+1); the reference's ghosted FIELD_STRUCTURE_2D
+ * arrays copy in and out row by row, skipping ghost_width cells on each side
+ * (ARRAY_2D::Index1D, ARRAY_2D.h:140-146).  Errors are thrown as
  * p2d::Error carrying the P2D_* status (the reference exits or stays silent).
  * Not thread-safe: call from one thread (inside BEGIN_HEAD_THREAD_WORK in the
  * reference's threaded Solve).
@@ -74,6 +76,14 @@ class PROJECTION_2D {
     max_velocity_y = last_stats.max_v;
     pressure_field.resize((size_t)nx_ * ny_);
     check(p2d_get_pressure(ctx_, pressure_field.data(), 1, P2D_MEM_HOST), "p2d_get_pressure");
+  }
+
+  // DeterminePressure() (PROJECTION_2D.h:480-509, also called directly by the solver,
+  // EULERIAN_FLUID_SOLVER_2D.h:1513, 1681): the pressure of the same solve, with u and v
+  // left untouched (the projection runs on a copy).
+  void DeterminePressure(double dt, const double* phi, const double* u, const double* v) {
+    std::vector<double> uc(u, u + (size_t)(nx_ + 1) * ny_), vc(v, v + (size_t)nx_ * (ny_ + 1));
+    Solve(dt, phi, uc.data(), vc.data(), P2D_MEM_HOST);
   }
 
   bool converged() const { return last_stats.status == P2D_OK; }

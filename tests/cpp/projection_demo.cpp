@@ -4,8 +4,9 @@
 //   projection_demo <dir> <n> <precision 0|1> <rho_minus> <rho_plus> <sigma>
 //
 // reads <dir>/phi.bin (n*n), u.bin ((n+1)*n), v.bin (n*(n+1)) as raw FP64,
-// runs Solve(dt = 1) and writes <dir>/p_out.bin, u_out.bin, v_out.bin and one
-// line of stats to stdout.
+// runs DeterminePressure(dt = 1) (checking that it leaves u, v alone), then
+// Solve(dt = 1) (checking the same pressure), and writes <dir>/p_out.bin,
+// u_out.bin, v_out.bin and one line of stats to stdout.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -52,7 +53,19 @@ int main(int argc, char** argv) {
     projection.water_density = std::atof(argv[5]);
     projection.surface_tension = std::atof(argv[6]);
     projection.tolerance = 1e-12;
+    // DeterminePressure first: it must leave u, v untouched and give Solve's pressure
+    const std::vector<double> u0 = u, v0 = v;
+    projection.DeterminePressure(1.0, phi.data(), u.data(), v.data());
+    const std::vector<double> p_only = projection.pressure_field;
+    if (u != u0 || v != v0) {
+      std::fprintf(stderr, "DeterminePressure changed the velocity\n");
+      return 4;
+    }
     projection.Solve(1.0, phi.data(), u.data(), v.data());
+    if (p_only != projection.pressure_field) {
+      std::fprintf(stderr, "DeterminePressure and Solve disagree on the pressure\n");
+      return 5;
+    }
     write_all(dir + "/p_out.bin", projection.pressure_field);
     write_all(dir + "/u_out.bin", u);
     write_all(dir + "/v_out.bin", v);
