@@ -1065,19 +1065,43 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     const int side = (int)threadIdx.x / (DTY + 2), r = (int)threadIdx.x % (DTY + 2);
     ps[r][side ? 65 : 0] = P(side ? i0 + 64 : i0 - 1, j0 - 1 + r);
   }
+  // Each wave owns RPW consecutive rows, so a row's north coefficient is the next row's
+  // south one, loaded once; a cell's east coefficient is the next lane's west one (DPP),
+  // only the last lane loads its own.  These loads do not depend on the LDS staging and
+  // are issued before the barrier.
+  constexpr int RPW = DTY / NW;
+  const int rb = w * RPW;
+  T bsr[RPW + 1], bwr[RPW], bwe[RPW];
+#pragma unroll
+  for (int m = 0; m <= RPW; ++m) {
+    const int j = j0 + rb + m;
+    bsr[m] = (i < nx && j < ny) ? bs[ci(i, j, nx)] : (T)0;
+  }
+#pragma unroll
+  for (int m = 0; m < RPW; ++m) {
+    const int j = j0 + rb + m;
+    bwr[m] = (i < nx && j < ny) ? bw[ci(i, j, nx)] : (T)0;
+  }
+#pragma unroll
+  for (int m = 0; m < RPW; ++m) {
+    const int j = j0 + rb + m;
+    T e = lane_next(bwr[m]);
+    if (c == 63 && i + 1 < nx && j < ny) e = bw[ci(i, j, nx) + 1];
+    bwe[m] = e;
+  }
   __syncthreads();
   double acc[1] = {0.0};
 #pragma unroll
-  for (int m = 0; m < DTY / NW; ++m) {
-    const int r = w + NW * m, j = j0 + r;
+  for (int m = 0; m < RPW; ++m) {
+    const int r = rb + m, j = j0 + r;
     if (i < nx && j < ny) {
       const size_t k = ci(i, j, nx);
       const T pc = ps[r + 1][c + 1];
       T a = 0;
-      if (i > 0) a += bw[k] * (pc - ps[r + 1][c]);
-      if (i + 1 < nx) a += bw[k + 1] * (pc - ps[r + 1][c + 2]);
-      if (j > 0) a += bs[k] * (pc - ps[r][c + 1]);
-      if (j + 1 < ny) a += bs[k + nx] * (pc - ps[r + 2][c + 1]);
+      if (i > 0) a += bwr[m] * (pc - ps[r + 1][c]);
+      if (i + 1 < nx) a += bwe[m] * (pc - ps[r + 1][c + 2]);
+      if (j > 0) a += bsr[m] * (pc - ps[r][c + 1]);
+      if (j + 1 < ny) a += bsr[m + 1] * (pc - ps[r + 2][c + 1]);
       pnew[k] = pc;
       q[k] = a;
       acc[0] += (double)pc * (double)a;
