@@ -111,6 +111,9 @@ struct p2d_ctx {
   double *phi = nullptr, *jc = nullptr, *bw = nullptr, *bs = nullptr;
   double *saw = nullptr, *sas = nullptr;  // ghost-fluid jump s a on the west / south face of each cell
   bool have_fluid = false, explicit_coeff = false, has_jc = true, coeff_ready = false;
+  // set by p2d_advance for its second and third projection of a step: the level set, and
+  // so the coefficients and the multigrid hierarchy, are those of the first
+  bool reuse_coeff = false;
   double rho_m = 1, rho_p = 1, sigma = 0;
 
   // right-hand side, solutions (double-double, double-buffered), staging
@@ -781,7 +784,7 @@ int p2d_project(p2d_ctx* c, double* u, double* v, double dt, p2d_stats* st, int 
     dv = c->sv;
   }
   c->hs->err = 0;
-  if (!c->explicit_coeff) {
+  if (!c->explicit_coeff && !(c->reuse_coeff && c->coeff_ready)) {
     HIPCHK(hipMemsetAsync(&c->s->err, 0, sizeof(int), c->stream));
     k_coeff<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, c->phi, 1.0 / c->rho_m,
                                                           1.0 / c->rho_p, c->sigma, c->bw, c->bs, c->jc, c->s);
@@ -1164,6 +1167,7 @@ int p2d_advance(p2d_ctx* c, double* phi, double* u, double* v, double dt, const 
     k_wall_zero<<<blocks_for((size_t)std::max(c->nx, c->ny)), BLOCK, 0, c->stream>>>(c->nx, c->ny, cu, cv);
     CHK(launched_ok());
     const int rc = p2d_project(c, cu, cv, dt, &st, P2D_MEM_DEVICE);
+    c->reuse_coeff = true;  // the next stages project with the same level set
     if (rc == P2D_NOT_CONVERGED) status = P2D_NOT_CONVERGED;
     else if (rc != P2D_OK) return rc;
     return P2D_OK;
@@ -1171,13 +1175,20 @@ int p2d_advance(p2d_ctx* c, double* phi, double* u, double* v, double dt, const 
   // velocity, TVD-RK3 (:1743-1797)
   HIPCHK(hipMemcpyAsync(Un, U, c->NU * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(Vn, V, c->NV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  CHK(stage());
-  CHK(stage());
+  c->reuse_coeff = false;
+  int src = stage();
+  if (src == P2D_OK) src = stage();
+  if (src != P2D_OK) {
+    c->reuse_coeff = false;
+    return src;
+  }
   lincomb(c, c->NU, 0.75, Un, 0.25, cu, U);
   lincomb(c, c->NV, 0.75, Vn, 0.25, cv, V);
   cu = U;
   cv = V;
-  CHK(stage());
+  src = stage();
+  c->reuse_coeff = false;
+  if (src != P2D_OK) return src;
   lincomb(c, c->NU, third, Un, two_thirds, cu, U);
   lincomb(c, c->NV, third, Vn, two_thirds, cv, V);
   CHK(launched_ok());
