@@ -114,6 +114,7 @@ struct p2d_ctx {
   // set by p2d_advance for its second and third projection of a step: the level set, and
   // so the coefficients and the multigrid hierarchy, are those of the first
   bool reuse_coeff = false;
+  int num_cu = 256;  // compute units of the device (tile kernels with more tiles run two blocks per CU)
   double rho_m = 1, rho_p = 1, sigma = 0;
 
   // right-hand side, solutions (double-double, double-buffered), staging
@@ -347,18 +348,32 @@ void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
                                                     v.p[0], v.p[1], v.q, c->partial, c->s);
   kt_end(c, kpre);
   k_fin_pre0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpre), (double)c->N, c->s);
-  for (int l = 1; l < mg.l0; ++l)
-    k_mg_pre<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpre), TBLOCK, 0, c->stream>>>(
-        lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-        c->s);
+  // FP32 coarse levels with more tiles than CUs run two blocks per CU (BUSY variants)
+  auto busy = [&](int l, int H) { return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H)) > c->num_cu; };
+  for (int l = 1; l < mg.l0; ++l) {
+    const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpre);
+    if (busy(l, Hpre))
+      k_mg_pre<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr,
+                                                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                            c->s);
+    else
+      k_mg_pre<T, false><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr,
+                                                      nullptr, nullptr, nullptr, nullptr, nullptr, c->s);
+  }
   SmallArgs<T> sa{};
   sa.nl = mg.L - mg.l0;
   for (int q = 0; q < sa.nl; ++q) sa.lv[q] = lv[mg.l0 + q];
   sa.binv = mg.binv;
   k_mg_small<T><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s);
-  for (int l = mg.l0 - 1; l >= 1; --l)
-    k_mg_post<T, false><<<tgrid(lv[l].nx, lv[l].ny, Hpost), TBLOCK, 0, c->stream>>>(
-        lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa, lv[l].xb, nu2, nullptr, c->s);
+  for (int l = mg.l0 - 1; l >= 1; --l) {
+    const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpost);
+    if (busy(l, Hpost))
+      k_mg_post<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa,
+                                                             lv[l].xb, nu2, nullptr, c->s);
+    else
+      k_mg_post<T, false><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa,
+                                                       lv[l].xb, nu2, nullptr, c->s);
+  }
   kt_begin(c, kpost);
   k_mg_post<T, true><<<gpost, TBLOCK, 0, c->stream>>>(lv[0], lv[1], nullptr, v.r[0], v.r[1], lv[0].xa, v.z, nu2,
                                                       c->partial, c->s);
@@ -655,6 +670,9 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
     if (hipGetDevice(&dev) == hipSuccess &&
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
       c->lds_limit = (size_t)lds - 1024;  // less the kernel's static LDS
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+      c->num_cu = cus;
   }
   c->stream = c->own;
   const size_t N = c->N;

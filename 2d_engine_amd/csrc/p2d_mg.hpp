@@ -495,8 +495,8 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
 // rbuf[1 - rpar] and x), and the block partials of ||r|| are stored.
 // Restriction r_H = P^T res: each cell splits its residual into the four
 // weight products; the x sums run across lanes, the y sums through LDS.
-template <class T, bool FUSE>
-__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE>())))
+template <class T, bool FUSE, bool BUSY = false>
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE || BUSY>())))
 k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB, T* xsol, const T* pA, const T* pB,
          const T* __restrict__ qv, double* partial, Scal* s) {
   if (s->done) return;
@@ -531,7 +531,8 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
   strip_stage<T>(S, F, giA, gy0, w, RY);
   strip_diag<T>(S);
   // interpolation weights of the restriction footprint: loaded with the rest on
-  // coarse levels, late in the sweeps on the finest (registers are short there)
+  // coarse levels, late in the sweeps where two blocks share a CU (the finest
+  // level, and BUSY coarse levels with more tiles than CUs): registers are short there
   T wxA[KR], wxB[KR], wyA[KR], wyB[KR];
   auto load_weights = [&]() {
 #pragma unroll
@@ -549,7 +550,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
       }
     }
   };
-  if (!FUSE) load_weights();
+  if (!(FUSE || BUSY)) load_weights();
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
     const int r = KR * w + m;
@@ -586,7 +587,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
   // on the finest level the weights' loads are issued before the last
   // half-sweep, whose work covers part of their latency
   for (int k = 0; k < 2 * nu - 1; ++k) strip_half<T>(S, lds, c, w, gy0, k & 1, k == 0, k);
-  if (FUSE) load_weights();
+  if (FUSE || BUSY) load_weights();
   strip_half<T>(S, lds, c, w, gy0, 1, false, 2 * nu - 1);
   // residual (reference order: r - (diag x - offsum)) with the last published
   // boundary rows (buffer (2 nu - 1) & 1), and the interior write of x
@@ -694,8 +695,8 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
 // stored.
 constexpr int PCW = 66;                           // staged coarse columns
 constexpr int PCH = (TY + 4 * MAX_NU) / 2 + 3;    // staged coarse rows
-template <class T, bool LEVEL0>
-__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, LEVEL0>())))
+template <class T, bool LEVEL0, bool BUSY = false>
+__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, LEVEL0 || BUSY>())))
 k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, const T* x_in, T* x_out, int nu,
           double* partial, Scal* s) {
   if (s->done) return;

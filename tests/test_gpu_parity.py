@@ -136,3 +136,22 @@ def test_projection_matches_golden(P2D, W, prec):
     assert np.linalg.norm(p - g["p"]) <= 1e-9 * np.linalg.norm(g["p"])
     scale = max(np.abs(g["u"]).max(), np.abs(g["v"]).max())
     assert max(np.abs(u - g["u"]).max(), np.abs(v - g["v"]).max()) <= 1e-8 * scale
+
+
+def test_vcycle_2048_with_two_block_coarse_level(P2D, O, W):
+    """At 2048^2 the first coarse level (1024^2, 320 tiles) has more tiles than the GPU has
+    CUs and runs the two-blocks-per-CU tile variants; the cycle still matches the oracle."""
+    w = W.rt(2048)
+    print("case vcycle 2048 prec=1", flush=True)
+    bx, by, jc = _coeffs(O, w)
+    ctx = P2D.Context(w.n, w.n, w.h)
+    ctx.set_coefficients(bx, by, w.phi, jc)
+    mg = O.MG(bx, by, nu_pre=3, nu_post=3)
+    assert mg.levels == ctx.mg_levels
+    r = np.random.default_rng(8).standard_normal((w.n, w.n))
+    r -= r.mean()
+    ref = mg.vcycle(r)
+    z = ctx.vcycle(r.astype(np.float32), prec=1)
+    assert np.linalg.norm(z - ref) <= 1e-4 * np.linalg.norm(ref)
+    z64 = ctx.vcycle(r, prec=0)
+    assert np.linalg.norm(z64 - ref) <= 1e-11 * np.linalg.norm(ref)
