@@ -66,3 +66,30 @@ def test_cpp_header_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-I", inc, str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_ctypes(tmp_path, P2D):
+    """p2d.h compiles as C99, and every struct the ctypes binding mirrors has the same size
+    and field offsets as the C compiler gives it (a mismatch would corrupt every call)."""
+    import ctypes
+    N = P2D._native
+    structs = {"p2d_config": N.P2DConfig, "p2d_stats": N.P2DStats, "p2d_step_params": N.P2DStepParams,
+               "p2d_oil_water_params": N.P2DOilWaterParams}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "p2d.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
