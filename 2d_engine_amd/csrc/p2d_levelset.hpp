@@ -23,6 +23,22 @@ __global__ void __launch_bounds__(BLOCK) k_reinit_sign(size_t N, double h, const
 // WENO5 derivative from the five one-sided differences v1..v5: smoothness
 // indicators, nonlinear weights with epsilon, the three candidate stencils
 // (ADVECTION_METHOD_2D.h:844-929, in its expression forms).
+// x / y for a constant divisor y with z = RN(1/y), correctly rounded without the general
+// division sequence: q = x z, then one FMA correction q + (x - q y) z (Markstein 1990: with z
+// the correctly rounded reciprocal and q within one ulp of x / y, the remainder x - q y is
+// exact and the corrected sum rounds to x / y).  An exact quotient keeps q (and with it the
+// sign of a zero), and so does an infinite one (its remainder is NaN).  Valid for
+// |x / y| >= 2^-1000, where the product cannot lose bits to underflow; the WENO inputs are
+// scaled differences of level-set values, zero or far above that.
+__device__ __forceinline__ double div_known(double x, double y, double z) {
+#pragma clang fp contract(off)
+  const double q = x * z;
+  const double r = fma(-q, y, x);
+  return (r == 0.0 || r != r) ? q : fma(r, z, q);
+}
+__device__ __forceinline__ double div3(double x) { return div_known(x, 3.0, 1.0 / 3.0); }
+__device__ __forceinline__ double div6(double x) { return div_known(x, 6.0, 1.0 / 6.0); }
+
 __device__ __forceinline__ double weno5_d(double v1, double v2, double v3, double v4, double v5, double eps) {
 #pragma clang fp contract(off)
   const double s1 = (double)13 / 12 * (v1 - 2 * v2 + v3) * (v1 - 2 * v2 + v3) +
@@ -35,9 +51,11 @@ __device__ __forceinline__ double weno5_d(double v1, double v2, double v3, doubl
   const double a3 = (double)3 / 10 * (double)1 / ((eps + s3) * (eps + s3));
   const double sum = a1 + a2 + a3;
   const double w1 = a1 / sum, w2 = a2 / sum, w3 = a3 / sum;
-  return w1 * (v1 * (double)1 / 3 - v2 * (double)7 / 6 + v3 * (double)11 / 6) +
-         w2 * (v2 * (-(double)1 / 6) + v3 * (double)5 / 6 + v4 * (double)1 / 3) +
-         w3 * (v3 * (double)1 / 3 + v4 * (double)5 / 6 - v5 * (double)1 / 6);
+  // the reference's v*(T)7/6 is (v * 7) / 6, a rounded division (div3, div6 give the same
+  // bits); -(T)1/6 is a constant
+  return w1 * (div3(v1 * (double)1) - div6(v2 * (double)7) + div6(v3 * (double)11)) +
+         w2 * (v2 * (-(double)1 / 6) + div6(v3 * (double)5) + div3(v4 * (double)1)) +
+         w3 * (div3(v3 * (double)1) + div6(v4 * (double)5) - div6(v5 * (double)1));
 }
 
 // |grad phi| by the reference's selection among the one-sided derivatives
