@@ -306,6 +306,27 @@ def cpu_baseline(w):
         "jacobi_pcg_1_thread_seconds_per_iteration": per_iteration(O.SOLVER_JACOBI, 1),
         f"jacobi_pcg_{all_threads}_threads_seconds_per_iteration": per_iteration(O.SOLVER_JACOBI, all_threads),
         "how": "oracle restatements, 25 minus 5 bounded iterations (CSR assembly cancels), same workload"}
+    # iterations to the stopping rule (relative 1e-10) at 256^2 and 512^2 of the same
+    # workload family, extrapolated to this size (CG iterations grow like n for Poisson)
+    W = importlib.import_module("2d_engine_amd.workloads")
+    conv = {}
+    for label, solver, threads, per_it in (("ic0_serial", O.SOLVER_IC0, 1,
+                                            base["reference_path_sample"]["seconds_per_iteration"]),
+                                           (f"jacobi_{all_threads}_threads", O.SOLVER_JACOBI, all_threads,
+                                            base["other_reference_solvers"][
+                                                f"jacobi_pcg_{all_threads}_threads_seconds_per_iteration"])):
+        its = {}
+        for m in (256, 512):
+            wm = W.rt(m)
+            a = time.perf_counter()
+            _, _, _, stm = O.project(wm.phi, wm.rho_minus, wm.rho_plus, wm.sigma, wm.u, wm.v, wm.h, solver=solver,
+                                     tol=1e-10, threads=threads)
+            its[m] = (stm["iterations"], round(time.perf_counter() - a, 3))
+        est_its = its[512][0] * (w.n / 512.0)
+        conv[label] = {"iterations_and_seconds_256": its[256], "iterations_and_seconds_512": its[512],
+                       f"estimated_iterations_{w.n}": int(round(est_its)),
+                       f"estimated_seconds_{w.n}": round(est_its * per_it, 1)}
+    base["reference_solvers_to_convergence"] = conv
     base["host"] = {"nproc": os.cpu_count(), "threads_used_for_all_core": all_threads, "cpu": cpu_model()}
     return base, (p, u, v)
 
