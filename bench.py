@@ -51,6 +51,24 @@ KERNELS = [
      "FP64 refinement: reads b p_hi p_lo (24 B) x (4 B) beta_x beta_y (16 B), writes p_hi p_lo (16 B) r (4 B)"),
 ]
 DOMINANT = "k_mg_pre<float,true>"
+
+# Algorithmic bytes per cell of one whole mixed projection, by the same rule as KERNELS:
+# a V-cycle is the finest pre + post + dir (45 + 29 + 24) plus, on each coarser level l
+# (N / 4^l cells), pre (reads r beta_x beta_y wx wy, writes x and coarse r: 25) and post
+# (reads r x_pre beta_x beta_y wx wy and coarse x, writes x: 29), summing to 54 N / 3; a
+# refinement pass is 64; the once-per-projection kernels are the coefficients (reads phi,
+# writes beta_x beta_y jc in FP64: 32), the right-hand side (reads u v beta_x beta_y, writes
+# b and the FP32 residual: 44), the multigrid setup (FP64 beta in, FP32 beta and weights
+# out, coarse beta, over the hierarchy: 48), the face update (72) and the divergence check (16).
+VCYCLE_B_PER_CELL = 45 + 29 + 24 + 54.0 / 3.0
+REFINE_B_PER_CELL = 64
+SETUP_B_PER_CELL = 32 + 44 + 48 + 72 + 16
+
+
+def projection_bytes(N, stats):
+    """Algorithmic bytes of one mixed projection: V-cycles = iterations + passes."""
+    vcycles = stats["iterations"] + stats["outer"]
+    return N * (vcycles * VCYCLE_B_PER_CELL + stats["outer"] * REFINE_B_PER_CELL + SETUP_B_PER_CELL)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_mg_pre_rt2048_mixed_pmc.md")
 
 
@@ -405,6 +423,12 @@ def main():
     semi = time_semi_implicit(P2D, torch, w.n, dev)
     dom = kernels[DOMINANT]
     traffic = read_traffic()
+    pbytes = projection_bytes(N, st_m)
+    per_projection = {"algorithmic_bytes": pbytes, "achieved_GBs": round(pbytes / (ms * 1e-3) / 1e9, 1),
+                      "frac": round(pbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "vcycles": st_m["iterations"] + st_m["outer"], "passes": st_m["outer"],
+                      "accounting": f"N x (V-cycles x {VCYCLE_B_PER_CELL:.1f} + passes x {REFINE_B_PER_CELL} + "
+                                    f"{SETUP_B_PER_CELL}) bytes, see bench.py VCYCLE_B_PER_CELL"}
     roofline = {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": DOMINANT + " (the largest share of the solve's kernel time)",
@@ -423,6 +447,7 @@ def main():
         "adjoint": adjoint,
         "roofline": roofline,
         "kernels": kernels,
+        "per_projection": per_projection,
         "levelset": levelset,
         "time_step": step,
         "viscosity_semi_implicit": semi,
