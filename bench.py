@@ -69,7 +69,7 @@ def projection_bytes(N, stats):
     """Algorithmic bytes of one mixed projection: V-cycles = iterations + passes."""
     vcycles = stats["iterations"] + stats["outer"]
     return N * (vcycles * VCYCLE_B_PER_CELL + stats["outer"] * REFINE_B_PER_CELL + SETUP_B_PER_CELL)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_mg_pre_rt2048_mixed_pmc.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_mg_pre_rt{n}_mixed_pmc.md")  # per workload size
 
 
 def log(*a):
@@ -269,10 +269,11 @@ def time_semi_implicit(P2D, torch, n, dev, reps=3):
             "setup": f"vertical pipe {n}x{n}, oil-water semi-implicit, dimensionless, reference boundary coupling"}
 
 
-def read_traffic():
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary, or None."""
+def read_traffic(n):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary of
+    this workload size, or None when there is none."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILE.format(n=n)) as f:
             text = f.read()
     except OSError:
         return None
@@ -422,7 +423,7 @@ def main():
     step = time_step(P2D, torch, w, dev)
     semi = time_semi_implicit(P2D, torch, w.n, dev)
     dom = kernels[DOMINANT]
-    traffic = read_traffic()
+    traffic = read_traffic(w.n)
     pbytes = projection_bytes(N, st_m)
     per_projection = {"algorithmic_bytes": pbytes, "achieved_GBs": round(pbytes / (ms * 1e-3) / 1e9, 1),
                       "frac": round(pbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
