@@ -23,10 +23,10 @@ __global__ void __launch_bounds__(BLOCK) k_reinit_sign(size_t N, double h, const
 // WENO5 derivative from the five one-sided differences v1..v5: smoothness
 // indicators, nonlinear weights with epsilon, the three candidate stencils
 // (ADVECTION_METHOD_2D.h:844-929, in its expression forms).
-// x / y for a constant divisor y with z = RN(1/y), correctly rounded without the general
-// division sequence: q = x z, then one FMA correction q + (x - q y) z (Markstein 1990: with z
-// the correctly rounded reciprocal and q within one ulp of x / y, the remainder x - q y is
-// exact and the corrected sum rounds to x / y).  An exact quotient keeps q (and with it the
+// x / y given z = RN(1/y), correctly rounded without the general division sequence:
+// q = x z, then one FMA correction q + (x - q y) z (Markstein 1990: with z the correctly
+// rounded reciprocal and q within one ulp of x / y, the remainder x - q y is exact and the
+// corrected sum rounds to x / y).  An exact quotient keeps q (and with it the
 // sign of a zero), and so does an infinite one (its remainder is NaN).  Valid for
 // |x / y| >= 2^-1000, where the product cannot lose bits to underflow; the WENO inputs are
 // scaled differences of level-set values, zero or far above that.
@@ -50,7 +50,10 @@ __device__ __forceinline__ double weno5_d(double v1, double v2, double v3, doubl
   const double a2 = (double)6 / 10 * (double)1 / ((eps + s2) * (eps + s2));
   const double a3 = (double)3 / 10 * (double)1 / ((eps + s3) * (eps + s3));
   const double sum = a1 + a2 + a3;
-  const double w1 = a1 / sum, w2 = a2 / sum, w3 = a3 / sum;
+  // the three quotients by sum share one correctly rounded reciprocal (Markstein's
+  // correction holds for any divisor whose reciprocal is correctly rounded)
+  const double zs = 1.0 / sum;
+  const double w1 = div_known(a1, sum, zs), w2 = div_known(a2, sum, zs), w3 = div_known(a3, sum, zs);
   // the reference's v*(T)7/6 is (v * 7) / 6, a rounded division (div3, div6 give the same
   // bits); -(T)1/6 is a constant
   return w1 * (div3(v1 * (double)1) - div6(v2 * (double)7) + div6(v3 * (double)11)) +
