@@ -93,3 +93,23 @@ def test_struct_layouts_match_ctypes(tmp_path, P2D):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for fname, _ in py._fields_:
             assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_parameter_defaults_and_visc_argument_checks(P2D):
+    """The step and oil-water defaults are the reference script's values, and the viscosity
+    solver rejects bad grids before touching the GPU."""
+    import ctypes
+    N = P2D._native
+    prm = N.P2DStepParams()
+    N.lib.p2d_step_params_default(ctypes.byref(prm))
+    assert (prm.gravity_x, prm.gravity_y) == (0.0, -9.8)
+    assert prm.reinit_iterations == 25 and prm.reinit_scaling == 5.0 and prm.weno_epsilon == 1e-6
+    ow = N.P2DOilWaterParams()
+    N.lib.p2d_oil_water_params_default(ctypes.byref(ow))
+    assert (ow.Re, ow.rho_oil, ow.rho_water, ow.mu_oil, ow.mu_water) == (0.94983, 0.905, 0.995, 6.01, 0.01)
+    assert ow.tolerance == 1e-6 and ow.max_iterations == 10000 and ow.boundary == N.VISC_REFERENCE
+    h = ctypes.c_void_p()
+    for nx, ny, dx, dy in ((1, 8, 0.1, 0.1), (8, 1, 0.1, 0.1), (8, 8, 0.0, 0.1), (8, 8, 0.1, -1.0)):
+        assert N.lib.p2d_visc_create(nx, ny, dx, dy, ctypes.byref(h)) == N.P2D_BAD_ARG
+    assert N.lib.p2d_visc_solve(None, 0, None, None, None, None, None, None, None, 0, 1e-6, 10, None, 0) == \
+        N.P2D_BAD_ARG
