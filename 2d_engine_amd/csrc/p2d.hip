@@ -36,6 +36,7 @@ namespace {
 
 constexpr int BOTTOM_MAX = 64;      // coarsen until the grid has <= 64 cells (or a side is odd)
 constexpr int BOTTOM_LIMIT = 1024;  // largest bottom grid accepted (dense pseudo-inverse)
+static_assert(BOTTOM_LIMIT <= SMALL_THREADS, "k_mg_bottom_inverse keeps one diagonal entry per thread");
 constexpr int GRIDSTRIDE_BLOCKS = 2048;
 
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
@@ -335,19 +336,30 @@ void invalidate(p2d_ctx* c) {
 }
 
 // One V-cycle on the finest level's residual rbuf[s->rpar] (after the pending
-// Krylov update, if any), result in v.z.
+// Krylov update, if any), result in v.z, in two parts: the head (finest
+// pre-smoothing fused with the update, and its finish, which applies the
+// convergence test) and the tail (the rest; every tail kernel returns at once
+// once the test has set s->done).
 template <class T>
-void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
-  const int nu1 = c->cfg.mg_pre_sweeps, nu2 = c->cfg.mg_post_sweeps;
-  const int Hpre = 2 * nu1 + 1, Hpost = 2 * nu2;
+void vcycle_head(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
+  const int nu1 = c->cfg.mg_pre_sweeps;
   const Level<T>* lv = mg.lv.data();
-  const dim3 gpre = tgrid(lv[0].nx, lv[0].ny, Hpre), gpost = tgrid(lv[0].nx, lv[0].ny, Hpost);
-  const int kpre = sizeof(T) == 4 ? KT_PRE0_F32 : KT_PRE0_F64, kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
+  const dim3 gpre = tgrid(lv[0].nx, lv[0].ny, 2 * nu1 + 1);
+  const int kpre = sizeof(T) == 4 ? KT_PRE0_F32 : KT_PRE0_F64;
   kt_begin(c, kpre);
   k_mg_pre<T, true><<<gpre, TBLOCK, 0, c->stream>>>(lv[0], lv[1], nullptr, lv[0].xa, nu1, v.r[0], v.r[1], v.x,
                                                     v.p[0], v.p[1], v.q, c->partial, c->s);
   kt_end(c, kpre);
   k_fin_pre0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpre), (double)c->N, c->s);
+}
+
+template <class T>
+void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
+  const int nu1 = c->cfg.mg_pre_sweeps, nu2 = c->cfg.mg_post_sweeps;
+  const int Hpre = 2 * nu1 + 1, Hpost = 2 * nu2;
+  const Level<T>* lv = mg.lv.data();
+  const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost);
+  const int kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
   // FP32 coarse levels with more tiles than CUs run two blocks per CU (BUSY variants)
   auto busy = [&](int l, int H) { return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H)) > c->num_cu; };
   for (int l = 1; l < mg.l0; ++l) {
@@ -382,8 +394,16 @@ void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
 }
 
 template <class T>
-void cg_iteration(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
-  vcycle<T>(c, mg, v);
+void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
+  vcycle_head<T>(c, mg, v);
+  vcycle_tail<T>(c, mg, v);
+}
+
+// The part of a CG iteration after the V-cycle's head: the rest of the V-cycle,
+// the new direction and alpha.
+template <class T>
+void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
+  vcycle_tail<T>(c, mg, v);
   const Level<T>& l0 = mg.lv[0];
   const dim3 g = dgrid(c->nx, c->ny);
   const int kd = sizeof(T) == 4 ? KT_DIR_F32 : KT_DIR_F64;
@@ -453,9 +473,17 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
   int chunk = predict(0);
   bool report = false;  // a refinement's norms wait in red (the iterations do not touch red)
   bool first = true;
+  // A chunk ends after the head of its last iteration (the convergence test):
+  // a pass that ends there skips the tail's launches, and one that goes on
+  // enqueues the tail first.
+  bool tail_pending = false;
   while (true) {
     const auto t_launch0 = std::chrono::steady_clock::now();
-    for (int k = 0; k < chunk; ++k) cg_iteration<T>(c, mg, v);
+    for (int k = 0; k < chunk; ++k) {
+      if (tail_pending) cg_tail<T>(c, mg, v);
+      vcycle_head<T>(c, mg, v);
+      tail_pending = true;
+    }
     CHK(launched_ok());
     const auto t_launch1 = std::chrono::steady_clock::now();
     CHK(sync_scal(c));
@@ -504,7 +532,8 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
       continue;
     }
     // end of a pass: refinement p += x (double-double), the true residual into the
-    // buffer the next pass reads
+    // buffer the next pass reads; the last head's tail is not needed
+    tail_pending = false;
     if (outer < MAX_PASSES) last[outer] = iters - seg_start;
     T* rdst = c->hs->rpar ? v.r[1] : v.r[0];
     const double* oh = outer > 0 ? solh[*cur] : nullptr;  // the first pass starts from p = 0

@@ -137,13 +137,17 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_bottom_inverse(int nx, int
   const int n = nx * ny, tid = threadIdx.x;
   const Coef<double> C{bw, bs, nx, ny};
   __shared__ double alpha_sh;
+  __shared__ double diag[SMALL_THREADS];  // n <= SMALL_THREADS (the host's bottom limit)
   __shared__ double a[BOTTOM_LDS_MAX * BOTTOM_LDS_MAX];
+  // diagonal entries loaded in parallel, summed in cell order by one thread
+  for (int c = tid; c < n; c += SMALL_THREADS) {
+    const int i = c % nx, j = c / nx;
+    diag[c] = C.W(i, j) + C.E(i, j) + C.S(i, j) + C.N(i, j);
+  }
+  __syncthreads();
   if (tid == 0) {
     double dsum = 0;
-    for (int c = 0; c < n; ++c) {
-      const int i = c % nx, j = c / nx;
-      dsum += C.W(i, j) + C.E(i, j) + C.S(i, j) + C.N(i, j);
-    }
+    for (int c = 0; c < n; ++c) dsum += diag[c];
     alpha_sh = dsum / n / n;
   }
   __syncthreads();
@@ -155,7 +159,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_bottom_inverse(int nx, int
     for (int c = tid; c < n; c += SMALL_THREADS) {
       const int i = c % nx, j = c / nx;
       double* row = a + c * n;
-      row[c] += C.W(i, j) + C.E(i, j) + C.S(i, j) + C.N(i, j);
+      row[c] += diag[c];
       if (i > 0) row[c - 1] -= C.W(i, j);
       if (i + 1 < nx) row[c + 1] -= C.E(i, j);
       if (j > 0) row[c - nx] -= C.S(i, j);
@@ -216,7 +220,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_bottom_inverse(int nx, int
   for (int c = tid; c < n; c += SMALL_THREADS) {
     const int i = c % nx, j = c / nx;
     double* row = work + (size_t)c * w;
-    row[c] += C.W(i, j) + C.E(i, j) + C.S(i, j) + C.N(i, j);
+    row[c] += diag[c];
     if (i > 0) row[c - 1] -= C.W(i, j);
     if (i + 1 < nx) row[c + 1] -= C.E(i, j);
     if (j > 0) row[c - nx] -= C.S(i, j);
