@@ -96,6 +96,12 @@ struct p2d_ctx {
   size_t N = 0, NU = 0, NV = 0;
   double h = 0;
   hipStream_t own = nullptr, stream = nullptr;
+  // The bottom pseudo-inverse runs on a side stream, forked from the solve
+  // stream after the multigrid setup and joined before its first use (the
+  // single-workgroup cycle), so that it overlaps the first V-cycle's descent.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool join_pending = false;
   std::vector<void*> allocs;
   size_t lds_limit = 65536;  // dynamic LDS available to the single-workgroup V-cycle
   // bench hook: HIP-event timing of every launch of selected finest-level kernels
@@ -234,6 +240,14 @@ int sync_scal(p2d_ctx* c) {
   return P2D_OK;
 }
 
+// Orders the solve stream after the pending bottom pseudo-inverse, if any.
+int join_side(p2d_ctx* c) {
+  if (!c->join_pending) return P2D_OK;
+  c->join_pending = false;
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  return P2D_OK;
+}
+
 // Levels: l = 0 uses the context's coefficients (bw0/bs0) when given; every
 // level owns xa, xb, r; levels above l0 own their interpolation weights.
 template <class T>
@@ -300,6 +314,7 @@ template <class T>
 int mg_setup(p2d_ctx* c, MG<T>& mg, bool cast32) {
   MG<double>& m = c->mg64;
   const bool common = c->mg_common;
+  CHK(join_side(c));  // the setup may rewrite what a pending inverse reads
   for (int l = 0; l < m.L; ++l) {
     const Level<double>& f = m.lv[l];
     T* wx = const_cast<T*>(mg.lv[l].wx);  // null on the bottom level
@@ -314,8 +329,13 @@ int mg_setup(p2d_ctx* c, MG<T>& mg, bool cast32) {
   }
   if (!common) {
     const Level<double>& B = m.lv[m.L - 1];
-    k_mg_bottom_inverse<<<1, SMALL_THREADS, 0, c->stream>>>(B.nx, B.ny, B.bw, B.bs, c->bottom_work, c->binv64,
-                                                            c->binv32);
+    HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    k_mg_bottom_inverse<<<1, SMALL_THREADS, 0, c->side>>>(B.nx, B.ny, B.bw, B.bs, c->bottom_work, c->binv64,
+                                                          c->binv32);
+    CHK(launched_ok());
+    HIPCHK(hipEventRecord(c->ev_join, c->side));
+    c->join_pending = true;
     c->mg_common = true;
   }
   mg.valid = true;
@@ -372,6 +392,7 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
       k_mg_pre<T, false><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr,
                                                       nullptr, nullptr, nullptr, nullptr, nullptr, c->s);
   }
+  (void)join_side(c);  // enqueue-only; a failure shows at the next launch check
   SmallArgs<T> sa{};
   sa.nl = mg.L - mg.l0;
   for (int q = 0; q < sa.nl; ++q) sa.lv[q] = lv[mg.l0 + q];
@@ -561,6 +582,7 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
     // before enqueueing more iterations
     chunk = outer >= last_passes ? 0 : predict(outer);
   }
+  CHK(join_side(c));  // a solve that ended before any single-workgroup cycle
   c->last_passes[slot] = outer;
   if (outer == 0) {  // no refinement ran: p = 0
     k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, solh[*cur], 0.0);
@@ -693,7 +715,10 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
     p2d_destroy(c);
     return rc;
   };
-  if (hipStreamCreate(&c->own) != hipSuccess) return fail(P2D_HIP_ERROR);
+  if (hipStreamCreate(&c->own) != hipSuccess || hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+    return fail(P2D_HIP_ERROR);
   {
     int dev = 0, lds = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
@@ -770,11 +795,15 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
 int p2d_destroy(p2d_ctx* c) {
   if (!c) return P2D_BAD_ARG;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   for (void* p : c->allocs) (void)hipFree(p);
   for (auto& t : c->ktimer)
     for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
   if (c->hs) (void)hipHostFree(c->hs);
   if (c->own) (void)hipStreamDestroy(c->own);
+  if (c->side) (void)hipStreamDestroy(c->side);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   delete c;
   return P2D_OK;
 }

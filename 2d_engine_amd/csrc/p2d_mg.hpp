@@ -166,46 +166,52 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_bottom_inverse(int nx, int
       if (j + 1 < ny) row[c + nx] -= C.N(i, j);
     }
     __syncthreads();
-    // Gauss-Jordan with the matrix in registers (thread t owns entries t + m SMALL_THREADS):
-    // step k needs the old pivot row and column, which their owners publish in LDS
-    // (double-buffered: one barrier per step).  New values: a_kk = 1/a_kk,
-    // a_kc = a_kc / a_kk, a_rk = -a_rk / a_kk, a_rc -= a_rk (a_kc / a_kk).
-    constexpr int OWN = BOTTOM_LDS_MAX * BOTTOM_LDS_MAX / SMALL_THREADS;
-    static_assert(OWN * SMALL_THREADS == BOTTOM_LDS_MAX * BOTTOM_LDS_MAX, "entries per thread");
+    // Gauss-Jordan with the matrix in the registers of the first GJ_THREADS threads
+    // (thread t owns entries t + m GJ_THREADS; four wavefronts, one per SIMD, each
+    // with OWN independent entries): step k needs the old pivot row and column,
+    // which their owners publish in LDS (double-buffered: one barrier per step).
+    // New values: a_kk = 1/a_kk, a_kc = a_kc / a_kk, a_rk = -a_rk / a_kk,
+    // a_rc -= a_rk (a_kc / a_kk), computed branch-free with the pivot row, column
+    // and element selected.  The other threads only pass the barriers.
+    constexpr int GJ_THREADS = SMALL_THREADS;
+    constexpr int OWN = BOTTOM_LDS_MAX * BOTTOM_LDS_MAX / GJ_THREADS;
+    static_assert(OWN * GJ_THREADS == BOTTOM_LDS_MAX * BOTTOM_LDS_MAX && GJ_THREADS <= SMALL_THREADS,
+                  "entries per thread");
     __shared__ double prow[2][BOTTOM_LDS_MAX], pcol[2][BOTTOM_LDS_MAX];
+    const bool gj = tid < GJ_THREADS;
     double e[OWN];
+    int er[OWN], ec[OWN];  // entry coordinates; -2: no entry (never a pivot row or the next one)
 #pragma unroll
     for (int m = 0; m < OWN; ++m) {
-      const int q = tid + m * SMALL_THREADS;
-      e[m] = q < n * n ? a[q] : 0.0;
-      if (q < n * n && q / n == 0) prow[0][q % n] = e[m];
-      if (q < n * n && q % n == 0) pcol[0][q / n] = e[m];
+      const int q = tid + m * GJ_THREADS;
+      const bool valid = gj && q < n * n;
+      er[m] = valid ? q / n : -2;
+      ec[m] = valid ? q % n : -2;
+      e[m] = valid ? a[q] : 0.0;
+      if (er[m] == 0) prow[0][ec[m]] = e[m];
+      if (ec[m] == 0) pcol[0][er[m]] = e[m];
     }
     __syncthreads();
     for (int k = 0; k < n; ++k) {
       const int b = k & 1, nb = b ^ 1;
-      const double d = 1.0 / prow[b][k];
+      if (gj) {
+        const double d = 1.0 / prow[b][k];
 #pragma unroll
-      for (int m = 0; m < OWN; ++m) {
-        const int q = tid + m * SMALL_THREADS;
-        if (q >= n * n) continue;
-        const int r = q / n, c = q % n;
-        double v;
-        if (r == k && c == k) v = d;
-        else if (r == k) v = e[m] * d;
-        else if (c == k) v = e[m] * -d;
-        else v = e[m] - pcol[b][r] * (prow[b][c] * d);
-        e[m] = v;
-        if (r == k + 1) prow[nb][c] = v;
-        if (c == k + 1) pcol[nb][r] = v;
+        for (int m = 0; m < OWN; ++m) {
+          const int r = er[m], c = ec[m];
+          const double pc = pcol[b][r < 0 ? 0 : r], pr = prow[b][c < 0 ? 0 : c];
+          const double gen = e[m] - pc * (pr * d), rowv = e[m] * d, colv = e[m] * -d;
+          const double v = r == k ? (c == k ? d : rowv) : (c == k ? colv : gen);
+          e[m] = v;
+          if (r == k + 1) prow[nb][c] = v;
+          if (c == k + 1) pcol[nb][r] = v;
+        }
       }
       __syncthreads();
     }
 #pragma unroll
-    for (int m = 0; m < OWN; ++m) {
-      const int q = tid + m * SMALL_THREADS;
-      if (q < n * n) a[q] = e[m];
-    }
+    for (int m = 0; m < OWN; ++m)
+      if (er[m] >= 0) a[er[m] * n + ec[m]] = e[m];
     __syncthreads();
     for (int q = tid; q < n * n; q += SMALL_THREADS) {
       const double v = a[q] - corr;
