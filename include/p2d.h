@@ -26,7 +26,9 @@
  * Pointers are host or device memory as selected by `mem`.  A context is not
  * thread-safe: call it from one thread (the reference runs the solve on thread 0
  * only, PROJECTION_2D.h:250-254).  Every call is ordered on the context's stream
- * and returns after its results are available.
+ * and returns after its results are available.  Inside a call, the multigrid's
+ * bottom pseudo-inverse runs on a second stream that the context owns, forked
+ * from and joined back into the context's stream by events.
  */
 #ifndef P2D_H
 #define P2D_H
