@@ -60,15 +60,20 @@ DOMINANT = "k_mg_pre<float,true>"
 # writes beta_x beta_y jc in FP64: 32), the right-hand side (reads u v beta_x beta_y, writes
 # b and the FP32 residual: 44), the multigrid setup (FP64 beta in, FP32 beta and weights
 # out, coarse beta, over the hierarchy: 48), the face update (72) and the divergence check (16).
+# An iteration is one full V-cycle with its direction; each pass ends with one more finest
+# pre-smoothing (its update and convergence test, 45), after which the rest of that
+# V-cycle is not launched.
 VCYCLE_B_PER_CELL = 45 + 29 + 24 + 54.0 / 3.0
+HEAD_B_PER_CELL = 45
 REFINE_B_PER_CELL = 64
 SETUP_B_PER_CELL = 32 + 44 + 48 + 72 + 16
 
 
 def projection_bytes(N, stats):
-    """Algorithmic bytes of one mixed projection: V-cycles = iterations + passes."""
-    vcycles = stats["iterations"] + stats["outer"]
-    return N * (vcycles * VCYCLE_B_PER_CELL + stats["outer"] * REFINE_B_PER_CELL + SETUP_B_PER_CELL)
+    """Algorithmic bytes of one mixed projection: a full V-cycle per iteration, a
+    pass-ending finest pre-smoothing and a refinement per pass, and the setup."""
+    return N * (stats["iterations"] * VCYCLE_B_PER_CELL + stats["outer"] * (HEAD_B_PER_CELL + REFINE_B_PER_CELL)
+                + SETUP_B_PER_CELL)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_mg_pre_rt{n}_mixed_pmc.md")  # per workload size
 
 
@@ -427,9 +432,10 @@ def main():
     pbytes = projection_bytes(N, st_m)
     per_projection = {"algorithmic_bytes": pbytes, "achieved_GBs": round(pbytes / (ms * 1e-3) / 1e9, 1),
                       "frac": round(pbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                      "vcycles": st_m["iterations"] + st_m["outer"], "passes": st_m["outer"],
-                      "accounting": f"N x (V-cycles x {VCYCLE_B_PER_CELL:.1f} + passes x {REFINE_B_PER_CELL} + "
-                                    f"{SETUP_B_PER_CELL}) bytes, see bench.py VCYCLE_B_PER_CELL"}
+                      "vcycles": st_m["iterations"], "passes": st_m["outer"],
+                      "accounting": f"N x (iterations x {VCYCLE_B_PER_CELL:.1f} + passes x "
+                                    f"({HEAD_B_PER_CELL} + {REFINE_B_PER_CELL}) + {SETUP_B_PER_CELL}) bytes, "
+                                    "see bench.py VCYCLE_B_PER_CELL"}
     roofline = {"bound": "hbm", "achieved": dom["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(dom["achieved_GBs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": DOMINANT + " (the largest share of the solve's kernel time)",
