@@ -381,7 +381,7 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost);
   const int kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
   // FP32 coarse levels with more tiles than CUs run two blocks per CU (BUSY variants)
-  auto busy = [&](int l, int H) { return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H)) > c->num_cu; };
+  auto busy = [&](int l, int H) { return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H)) > (size_t)c->num_cu; };
   for (int l = 1; l < mg.l0; ++l) {
     const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpre);
     if (busy(l, Hpre))
