@@ -50,9 +50,13 @@ inline dim3 tgrid(int nx, int ny, int H) {
   const int TX = tile_tx(H);
   return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + TY - 1) / TY));
 }
-// Direction kernel: 64 x DTY cells per block.
+// Direction kernel: DTX x DTY cells per block.
+inline dim3 dirgrid(int nx, int ny) {
+  return dim3((unsigned)((nx + DTX - 1) / DTX), (unsigned)((ny + DTY - 1) / DTY));
+}
+// Level-set tile kernels (re-initialisation, advection): RI_W x RI_H cells per block.
 inline dim3 dgrid(int nx, int ny) {
-  return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + DTY - 1) / DTY));
+  return dim3((unsigned)((nx + RI_W - 1) / RI_W), (unsigned)((ny + RI_H - 1) / RI_H));
 }
 // Refinement tile kernel: 64 x FTY cells per block.
 inline dim3 fgrid(int nx, int ny) {
@@ -426,7 +430,7 @@ template <class T>
 void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   vcycle_tail<T>(c, mg, v);
   const Level<T>& l0 = mg.lv[0];
-  const dim3 g = dgrid(c->nx, c->ny);
+  const dim3 g = dirgrid(c->nx, c->ny);
   const int kd = sizeof(T) == 4 ? KT_DIR_F32 : KT_DIR_F64;
   kt_begin(c, kd);
   k_dir<T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, v.p[0], v.p[1], v.q, c->partial, c->s);
@@ -769,7 +773,7 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
     // partial sums: the largest block count of any reducing launch
     size_t mb = std::max<size_t>(count(bgrid(c->nx + 1, c->ny + 1)), GRIDSTRIDE_BLOCKS);
     for (int H = 2; H <= 2 * MAX_NU + 1; ++H) mb = std::max(mb, count(tgrid(c->nx, c->ny, H)));
-    mb = std::max(mb, count(dgrid(c->nx, c->ny)));
+    mb = std::max(mb, count(dirgrid(c->nx, c->ny)));
     mb = std::max(mb, count(fgrid(c->nx, c->ny)));
     A(c->partial, (size_t)RED_K * mb);
   }
@@ -1382,11 +1386,11 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
   auto launch = [&]() {
     switch (which) {
       case 0:
-        k_dir<float><<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, L32[0].bw, L32[0].bs, c->v32.z,
+        k_dir<float><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, L32[0].bw, L32[0].bs, c->v32.z,
                                                                    c->v32.p[0], c->v32.p[1], c->v32.q, c->partial, c->s);
         break;
       case 1:
-        k_dir<double><<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, c->v64.z,
+        k_dir<double><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, c->v64.z,
                                                                     c->v64.p[0], c->v64.p[1], c->v64.q, c->partial, c->s);
         break;
       case 2:

@@ -1043,15 +1043,21 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 // --------------------------------------------------------------------------- Krylov direction kernel
 // p = (z - mean z) + beta p_old (FP64 arithmetic, stored as T; p_old/p_new =
 // pbuf[iter & 1] / pbuf[(iter+1) & 1]), q = A0' p, alpha = rz / p.q; marks the
-// update pending.  Tile kernel: p of a 64 x DTY tile plus a one-cell ring is
-// formed in LDS (all loads issued at once), then the stencil reads LDS.
-// Grid dim3(ceil(nx/64), ceil(ny/DTY)), BLOCK threads.
+// update pending.  Tile kernel over 128 x DTY cells: lane c of a wavefront holds
+// the column pair (2c, 2c+1) of a row, so every array moves with one two-element
+// load or store per lane.  p of the tile plus a one-cell ring is formed in LDS
+// (all loads issued at once), then the stencil reads LDS.  The finest level has
+// an even nx (it coarsens at least once), so a pair is inside the grid or
+// outside it as a whole.  Grid dim3(ceil(nx/128), ceil(ny/DTY)), BLOCK threads.
+constexpr int DTX = 128;  // columns per block of the direction kernel
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restrict__ bw, const T* __restrict__ bs,
                                                const T* __restrict__ z, T* pA, T* pB, T* __restrict__ q,
                                                double* partial, Scal* s) {
   if (s->done) return;
-  __shared__ T ps[DTY + 2][66];
+  // row layout: [1] west ring, [2 + 2c, 3 + 2c] pair c, [DTX + 2] east ring (pairs 2-element aligned)
+  __shared__ Pair<T> ps2[DTY + 2][DTX / 2 + 2];
+  T(*ps)[DTX + 4] = reinterpret_cast<T(*)[DTX + 4]>(&ps2[0][0]);
   const double beta = s->beta, zm = s->zmean;
   const bool first = s->iter == s->seg0;
   const T* pold = (s->iter & 1) ? pB : pA;
@@ -1059,64 +1065,90 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
   int bx, by;
   xcd_tile(bx, by);
-  const int i0 = bx * 64, j0 = by * DTY;
-  const int i = i0 + c;
-  auto P = [&](int ii, int jj) -> T {
-    if (ii < 0 || ii >= nx || jj < 0 || jj >= ny) return (T)0;
-    const size_t k = ci(ii, jj, nx);
-    return first ? (T)((double)z[k] - zm) : (T)(((double)z[k] - zm) + beta * (double)pold[k]);
+  const int i0 = bx * DTX, j0 = by * DTY;
+  const int iA = i0 + 2 * c;
+  const bool col_in = iA < nx;
+  auto pform = [&](T zz, T po) -> T {
+    return first ? (T)((double)zz - zm) : (T)(((double)zz - zm) + beta * (double)po);
   };
   constexpr int NW = BLOCK / 64, RR = (DTY + 2 + NW - 1) / NW;
 #pragma unroll
   for (int m = 0; m < RR; ++m) {
-    const int r = w + NW * m;
-    if (r < DTY + 2) ps[r][c + 1] = P(i, j0 - 1 + r);
+    const int r = w + NW * m, jj = j0 - 1 + r;
+    if (r >= DTY + 2) continue;
+    Pair<T> v{(T)0, (T)0};
+    if (col_in && jj >= 0 && jj < ny) {
+      const unsigned k = cell_u(iA, jj, nx);
+      const Pair<T> z2 = ld_at<Pair<T>>(z, k);
+      const Pair<T> o2 = first ? Pair<T>{(T)0, (T)0} : ld_at<Pair<T>>(pold, k);
+      v.a = pform(z2.a, o2.a);
+      v.b = pform(z2.b, o2.b);
+    }
+    ps2[r][1 + c] = v;
   }
   if (threadIdx.x < 2 * (DTY + 2)) {
     const int side = (int)threadIdx.x / (DTY + 2), r = (int)threadIdx.x % (DTY + 2);
-    ps[r][side ? 65 : 0] = P(side ? i0 + 64 : i0 - 1, j0 - 1 + r);
+    const int ii = side ? i0 + DTX : i0 - 1, jj = j0 - 1 + r;
+    T v = 0;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+      const unsigned k = cell_u(ii, jj, nx);
+      v = pform(z[k], first ? (T)0 : pold[k]);
+    }
+    ps[r][side ? DTX + 2 : 1] = v;
   }
-  // Each wave owns RPW consecutive rows, so a row's north coefficient is the next row's
-  // south one, loaded once; a cell's east coefficient is the next lane's west one (DPP),
-  // only the last lane loads its own.  These loads do not depend on the LDS staging and
-  // are issued before the barrier.
+  // Each wave owns RPW consecutive rows, so a row's north faces are the next row's
+  // south ones, loaded once; the east face of B is the next lane's west face of A
+  // (DPP), only the last lane loads its own.  These loads do not depend on the LDS
+  // staging and are issued before the barrier.
   constexpr int RPW = DTY / NW;
   const int rb = w * RPW;
-  T bsr[RPW + 1], bwr[RPW], bwe[RPW];
+  T sA[RPW + 1], sB[RPW + 1], wA[RPW], wB[RPW], eB[RPW];
 #pragma unroll
   for (int m = 0; m <= RPW; ++m) {
     const int j = j0 + rb + m;
-    bsr[m] = (i < nx && j < ny) ? bs[ci(i, j, nx)] : (T)0;
+    const Pair<T> s2 = (col_in && j < ny) ? ld_at<Pair<T>>(bs, cell_u(iA, j, nx)) : Pair<T>{(T)0, (T)0};
+    sA[m] = s2.a;
+    sB[m] = s2.b;
   }
 #pragma unroll
   for (int m = 0; m < RPW; ++m) {
     const int j = j0 + rb + m;
-    bwr[m] = (i < nx && j < ny) ? bw[ci(i, j, nx)] : (T)0;
+    const Pair<T> w2 = (col_in && j < ny) ? ld_at<Pair<T>>(bw, cell_u(iA, j, nx)) : Pair<T>{(T)0, (T)0};
+    wA[m] = w2.a;
+    wB[m] = w2.b;
   }
 #pragma unroll
   for (int m = 0; m < RPW; ++m) {
     const int j = j0 + rb + m;
-    T e = lane_next(bwr[m]);
-    if (c == 63 && i + 1 < nx && j < ny) e = bw[ci(i, j, nx) + 1];
-    bwe[m] = e;
+    T e = lane_next(wA[m]);
+    if (c == 63 && iA + 2 < nx && j < ny) e = ld_at<T>(bw, cell_u(iA + 2, j, nx));
+    eB[m] = e;
   }
   __syncthreads();
   double acc[1] = {0.0};
 #pragma unroll
   for (int m = 0; m < RPW; ++m) {
     const int r = rb + m, j = j0 + r;
-    if (i < nx && j < ny) {
-      const size_t k = ci(i, j, nx);
-      const T pc = ps[r + 1][c + 1];
-      T a = 0;
-      if (i > 0) a += bwr[m] * (pc - ps[r + 1][c]);
-      if (i + 1 < nx) a += bwe[m] * (pc - ps[r + 1][c + 2]);
-      if (j > 0) a += bsr[m] * (pc - ps[r][c + 1]);
-      if (j + 1 < ny) a += bsr[m + 1] * (pc - ps[r + 2][c + 1]);
-      pnew[k] = pc;
-      q[k] = a;
-      acc[0] += (double)pc * (double)a;
-    }
+    if (!col_in || j >= ny) continue;
+    const unsigned k = cell_u(iA, j, nx);
+    const Pair<T> pc = ps2[r + 1][1 + c], ps_ = ps2[r][1 + c], pn = ps2[r + 2][1 + c];
+    const T pa = pc.a, pb = pc.b;
+    // cell A (column iA): west, east (= B), south, north
+    T qa = 0;
+    if (iA > 0) qa += wA[m] * (pa - ps[r + 1][1 + 2 * c]);
+    qa += wB[m] * (pa - pb);
+    if (j > 0) qa += sA[m] * (pa - ps_.a);
+    if (j + 1 < ny) qa += sA[m + 1] * (pa - pn.a);
+    // cell B (column iA + 1): west (= A), east, south, north
+    T qb = 0;
+    qb += wB[m] * (pb - pa);
+    if (iA + 2 < nx) qb += eB[m] * (pb - ps[r + 1][4 + 2 * c]);
+    if (j > 0) qb += sB[m] * (pb - ps_.b);
+    if (j + 1 < ny) qb += sB[m + 1] * (pb - pn.b);
+    st_at(pnew, k, Pair<T>{pa, pb});
+    st_at(q, k, Pair<T>{qa, qb});
+    acc[0] += (double)pa * (double)qa;
+    acc[0] += (double)pb * (double)qb;
   }
   block_partial<1>(acc, 0u, partial);
 }
