@@ -58,9 +58,9 @@ inline dim3 dirgrid(int nx, int ny) {
 inline dim3 dgrid(int nx, int ny) {
   return dim3((unsigned)((nx + RI_W - 1) / RI_W), (unsigned)((ny + RI_H - 1) / RI_H));
 }
-// Refinement tile kernel: 64 x FTY cells per block.
+// Refinement tile kernel: FTX x FTY cells per block.
 inline dim3 fgrid(int nx, int ny) {
-  return dim3((unsigned)((nx + 63) / 64), (unsigned)((ny + FTY - 1) / FTY));
+  return dim3((unsigned)((nx + FTX - 1) / FTX), (unsigned)((ny + FTY - 1) / FTY));
 }
 constexpr int MAX_PASSES = 32;
 // Kernels p2d_time_kernel / p2d_kernel_timing name (finest level).

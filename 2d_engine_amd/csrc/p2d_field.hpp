@@ -166,8 +166,15 @@ __global__ void __launch_bounds__(BLOCK) k_rhs(int nx, int ny, double h, double 
 // On the double-double pressure: p_new = p_old + e (e may be NULL; ph/pl NULL: p_old = 0),
 // r = (b' - mean b') - sum_f +-G_f with G_f = beta_f fdiff(p, s a) (west and
 // south faces +, east and north -), stored as R; block partials sum r^2,
-// max |r|, sum r.  saw/sas may be NULL (no jump).  Grid dim3(ceil(nx/64),
-// ceil(ny/FTY)), BLOCK threads; p_new of the tile interior into (qh, ql).
+// max |r|, sum r.  saw/sas may be NULL (no jump).  Tile kernel over FTX x FTY
+// cells: lane c of a wavefront holds the column pair (2c, 2c+1) of a row and
+// each wavefront FTY / 4 consecutive rows, so every array moves with one
+// two-element load or store per lane, a row's north faces are the next row's
+// south ones and the east face of a pair's second cell comes from the next lane.
+// p_new of the tile plus a one-cell ring is staged in LDS.  The finest level has
+// an even nx.  Grid dim3(ceil(nx/FTX), ceil(ny/FTY)), BLOCK threads; p_new of the
+// tile interior into (qh, ql).
+constexpr int FTX = 128;
 template <class TE, class R>
 __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const double* __restrict__ bw,
                                                          const double* __restrict__ bs,
@@ -178,62 +185,135 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
                                                          double* __restrict__ ql, R* __restrict__ r,
                                                          double* partial, const Scal* s) {
 #pragma clang fp contract(off)
-  __shared__ double sh[FTY + 2][66], sl[FTY + 2][66];
+  // row layout: [1] west ring, [2 + 2c, 3 + 2c] pair c, [FTX + 2] east ring
+  __shared__ Pair<double> sh2[FTY + 2][FTX / 2 + 2], sl2[FTY + 2][FTX / 2 + 2];
+  double(*sh)[FTX + 4] = reinterpret_cast<double(*)[FTX + 4]>(&sh2[0][0]);
+  double(*sl)[FTX + 4] = reinterpret_cast<double(*)[FTX + 4]>(&sl2[0][0]);
   const double bm = s->bmean;
   const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
   int bx, by;
   xcd_tile(bx, by);
-  const int i0 = bx * 64, j0 = by * FTY;
-  const int i = i0 + c;
-  auto stage = [&](int ii, int jj, int rr, int cc) {
-    double hh = 0.0, ll = 0.0;
-    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
-      const size_t q = ci(ii, jj, nx);
-      if (!ph) {
-        hh = e ? (double)e[q] : 0.0;
-      } else if (e) {
-        dd_add(ph[q], pl[q], (double)e[q], hh, ll);
-      } else {
-        hh = ph[q];
-        ll = pl[q];
-      }
+  const int i0 = bx * FTX, j0 = by * FTY;
+  const int iA = i0 + 2 * c;
+  const bool col_in = iA < nx;
+  // p_new of one cell from p_old (hi, lo) and the correction
+  auto pnew = [&](double oh, double ol, double ev, double& hh, double& ll) {
+    if (!ph) {
+      hh = ev;
+      ll = 0.0;
+    } else if (e) {
+      dd_add(oh, ol, ev, hh, ll);
+    } else {
+      hh = oh;
+      ll = ol;
     }
-    sh[rr][cc] = hh;
-    sl[rr][cc] = ll;
   };
   constexpr int NW = BLOCK / 64, RR = (FTY + 2 + NW - 1) / NW;
 #pragma unroll
   for (int m = 0; m < RR; ++m) {
-    const int rr = w + NW * m;
-    if (rr < FTY + 2) stage(i, j0 - 1 + rr, rr, c + 1);
+    const int rr = w + NW * m, jj = j0 - 1 + rr;
+    if (rr >= FTY + 2) continue;
+    Pair<double> h2{0.0, 0.0}, l2{0.0, 0.0};
+    if (col_in && jj >= 0 && jj < ny) {
+      const unsigned k = cell_u(iA, jj, nx);
+      const Pair<double> oh = ph ? ld_at<Pair<double>>(ph, k) : Pair<double>{0.0, 0.0};
+      const Pair<double> ol = ph ? ld_at<Pair<double>>(pl, k) : Pair<double>{0.0, 0.0};
+      const Pair<TE> e2 = e ? ld_at<Pair<TE>>(e, k) : Pair<TE>{(TE)0, (TE)0};
+      pnew(oh.a, ol.a, (double)e2.a, h2.a, l2.a);
+      pnew(oh.b, ol.b, (double)e2.b, h2.b, l2.b);
+    }
+    sh2[rr][1 + c] = h2;
+    sl2[rr][1 + c] = l2;
   }
   if (threadIdx.x < 2 * (FTY + 2)) {
     const int side = (int)threadIdx.x / (FTY + 2), rr = (int)threadIdx.x % (FTY + 2);
-    stage(side ? i0 + 64 : i0 - 1, j0 - 1 + rr, rr, side ? 65 : 0);
+    const int ii = side ? i0 + FTX : i0 - 1, jj = j0 - 1 + rr;
+    double hh = 0.0, ll = 0.0;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+      const size_t q = ci(ii, jj, nx);
+      pnew(ph ? ph[q] : 0.0, ph ? pl[q] : 0.0, e ? (double)e[q] : 0.0, hh, ll);
+    }
+    sh[rr][side ? FTX + 2 : 1] = hh;
+    sl[rr][side ? FTX + 2 : 1] = ll;
+  }
+  // faces and jumps of the wave's rows (issued before the barrier): south of rows
+  // rb .. rb + RPW (the last is the north of the wave's top row), west of A and B
+  // (B's west = A's east), east of B from the next lane
+  constexpr int RPW = FTY / NW;
+  const int rb = w * RPW;
+  double sA[RPW + 1], sB[RPW + 1], jsA[RPW + 1], jsB[RPW + 1], wA[RPW], wB[RPW], eB[RPW], jwA[RPW], jwB[RPW],
+      jeB[RPW];
+#pragma unroll
+  for (int m = 0; m <= RPW; ++m) {
+    const int j = j0 + rb + m;
+    const bool in = col_in && j < ny;
+    const unsigned k = cell_u(iA, j, nx);
+    const Pair<double> s2 = in ? ld_at<Pair<double>>(bs, k) : Pair<double>{0.0, 0.0};
+    const Pair<double> j2 = in && sas ? ld_at<Pair<double>>(sas, k) : Pair<double>{0.0, 0.0};
+    sA[m] = s2.a;
+    sB[m] = s2.b;
+    jsA[m] = j2.a;
+    jsB[m] = j2.b;
+  }
+#pragma unroll
+  for (int m = 0; m < RPW; ++m) {
+    const int j = j0 + rb + m;
+    const bool in = col_in && j < ny;
+    const unsigned k = cell_u(iA, j, nx);
+    const Pair<double> w2 = in ? ld_at<Pair<double>>(bw, k) : Pair<double>{0.0, 0.0};
+    const Pair<double> j2 = in && saw ? ld_at<Pair<double>>(saw, k) : Pair<double>{0.0, 0.0};
+    wA[m] = w2.a;
+    wB[m] = w2.b;
+    jwA[m] = j2.a;
+    jwB[m] = j2.b;
+  }
+#pragma unroll
+  for (int m = 0; m < RPW; ++m) {
+    const int j = j0 + rb + m;
+    double ev = lane_next(wA[m]), jv = lane_next(jwA[m]);
+    if (c == 63 && iA + 2 < nx && j < ny) {
+      const unsigned k = cell_u(iA + 2, j, nx);
+      ev = ld_at<double>(bw, k);
+      jv = saw ? ld_at<double>(saw, k) : 0.0;
+    }
+    eB[m] = ev;
+    jeB[m] = jv;
   }
   __syncthreads();
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-  for (int m = 0; m < FTY / NW; ++m) {
-    const int rr = w + NW * m, j = j0 + rr;
-    if (i < nx && j < ny) {
-      const size_t k = ci(i, j, nx);
-      const double ch = sh[rr + 1][c + 1], cl = sl[rr + 1][c + 1];
-      double g = 0.0;
-      if (i > 0) g += bw[k] * fdiff(ch, cl, sh[rr + 1][c], sl[rr + 1][c], saw ? saw[k] : 0.0);
-      if (i + 1 < nx) g -= bw[k + 1] * fdiff(sh[rr + 1][c + 2], sl[rr + 1][c + 2], ch, cl, saw ? saw[k + 1] : 0.0);
-      if (j > 0) g += bs[k] * fdiff(ch, cl, sh[rr][c + 1], sl[rr][c + 1], sas ? sas[k] : 0.0);
-      if (j + 1 < ny) g -= bs[k + nx] * fdiff(sh[rr + 2][c + 1], sl[rr + 2][c + 1], ch, cl, sas ? sas[k + nx] : 0.0);
-      const double rv = (b[k] - bm) - g;
-      if (qh) {
-        qh[k] = ch;
-        ql[k] = cl;
-      }
-      r[k] = (R)rv;
-      acc[0] += rv * rv;
-      acc[1] = fmax(acc[1], fabs(rv));
-      acc[2] += rv;
+  for (int m = 0; m < RPW; ++m) {
+    const int rr = rb + m, j = j0 + rr;
+    if (!col_in || j >= ny) continue;
+    const unsigned k = cell_u(iA, j, nx);
+    const Pair<double> ch2 = sh2[rr + 1][1 + c], cl2 = sl2[rr + 1][1 + c];
+    const Pair<double> dh2 = sh2[rr][1 + c], dl2 = sl2[rr][1 + c];
+    const Pair<double> uh2 = sh2[rr + 2][1 + c], ul2 = sl2[rr + 2][1 + c];
+    const Pair<double> b2 = ld_at<Pair<double>>(b, k);
+    // cell A (column iA): west, east (= B), south, north
+    double gA = 0.0;
+    if (iA > 0) gA += wA[m] * fdiff(ch2.a, cl2.a, sh[rr + 1][1 + 2 * c], sl[rr + 1][1 + 2 * c], jwA[m]);
+    gA -= wB[m] * fdiff(ch2.b, cl2.b, ch2.a, cl2.a, jwB[m]);
+    if (j > 0) gA += sA[m] * fdiff(ch2.a, cl2.a, dh2.a, dl2.a, jsA[m]);
+    if (j + 1 < ny) gA -= sA[m + 1] * fdiff(uh2.a, ul2.a, ch2.a, cl2.a, jsA[m + 1]);
+    // cell B (column iA + 1): west (= A), east, south, north
+    double gB = 0.0;
+    gB += wB[m] * fdiff(ch2.b, cl2.b, ch2.a, cl2.a, jwB[m]);
+    if (iA + 2 < nx) gB -= eB[m] * fdiff(sh[rr + 1][4 + 2 * c], sl[rr + 1][4 + 2 * c], ch2.b, cl2.b, jeB[m]);
+    if (j > 0) gB += sB[m] * fdiff(ch2.b, cl2.b, dh2.b, dl2.b, jsB[m]);
+    if (j + 1 < ny) gB -= sB[m + 1] * fdiff(uh2.b, ul2.b, ch2.b, cl2.b, jsB[m + 1]);
+    const double ra = (b2.a - bm) - gA, rbv = (b2.b - bm) - gB;
+    if (qh) {
+      st_at(qh, k, ch2);
+      st_at(ql, k, cl2);
     }
+    st_at(r, k, Pair<R>{(R)ra, (R)rbv});
+    acc[0] += ra * ra;
+    acc[1] = fmax(acc[1], fabs(ra));
+    acc[2] += ra;
+    acc[0] += rbv * rbv;
+    acc[1] = fmax(acc[1], fabs(rbv));
+    acc[2] += rbv;
   }
   block_partial<3>(acc, 2u, partial);
 }

@@ -353,40 +353,6 @@ static_assert(KR * TWAVES >= RYMAX, "a tile region must fit the strips");
 static_assert(2 * RYMAX * NCXMAX <= 2 * TILE_LDS_PAIRS, "restriction row sums must fit the boundary buffers");
 static_assert(3 * TBLOCK * sizeof(double) <= TILE_LDS_PAIRS * 2 * sizeof(float), "block reduction must fit them");
 
-template <class T>
-struct alignas(2 * sizeof(T)) Pair {
-  T a, b;
-};
-
-// The value held by the previous / next lane of the wavefront (0 for lane 0 /
-// lane 63): DPP wave_shr:1 / wave_shl:1 with bound_ctrl.
-__device__ __forceinline__ int dpp_prev(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
-__device__ __forceinline__ int dpp_next(int v) { return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true); }
-__device__ __forceinline__ float lane_prev(float v) { return __int_as_float(dpp_prev(__float_as_int(v))); }
-__device__ __forceinline__ float lane_next(float v) { return __int_as_float(dpp_next(__float_as_int(v))); }
-__device__ __forceinline__ double lane_prev(double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)dpp_prev((int)(unsigned)b), hi = (unsigned)dpp_prev((int)(unsigned)(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-__device__ __forceinline__ double lane_next(double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned lo = (unsigned)dpp_next((int)(unsigned)b), hi = (unsigned)dpp_next((int)(unsigned)(b >> 32));
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// Tile kernels address their level's arrays with 32-bit byte offsets (a level
-// holds fewer than 2^32 bytes per array; p2d_create checks it), so a load or
-// store takes the array's base in scalar registers and one VGPR offset.
-__device__ __forceinline__ unsigned cell_u(int i, int j, int nx) { return (unsigned)i + (unsigned)j * (unsigned)nx; }
-template <class V, class T>
-__device__ __forceinline__ V ld_at(const T* base, unsigned cell) {
-  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) + cell * (unsigned)sizeof(T));
-}
-template <class V, class T>
-__device__ __forceinline__ void st_at(T* base, unsigned cell, V v) {
-  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + cell * (unsigned)sizeof(T)) = v;
-}
 
 // Per row m of the strip: right-hand side, faces (wA west of A, wB west of B
 // = east of A, eB east of B, s south; the north faces are the next row's
