@@ -328,8 +328,7 @@ int mg_setup(p2d_ctx* c, MG<T>& mg, bool cast32) {
     double* cbw = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bw) : nullptr;
     double* cbs = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bs) : nullptr;
     if (!wx && !f32bw && !cbw) continue;
-    k_mg_setup<T><<<dim3((unsigned)((f.nx + STX - 1) / STX), (unsigned)((f.ny + STY - 1) / STY)), BLOCK, 0,
-                    c->stream>>>(
+    k_mg_setup<T><<<dim3(blocks_for((size_t)f.nx), (unsigned)f.ny), BLOCK, 0, c->stream>>>(
         f.nx, f.ny, f.bw, f.bs, wx, wy, f32bw, f32bs, cbw, cbs);
   }
   if (!common) {
