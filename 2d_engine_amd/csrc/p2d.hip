@@ -38,6 +38,7 @@ constexpr int BOTTOM_MAX = 64;      // coarsen until the grid has <= 64 cells (o
 constexpr int BOTTOM_LIMIT = 1024;  // largest bottom grid accepted (dense pseudo-inverse)
 static_assert(BOTTOM_LIMIT <= SMALL_THREADS, "k_mg_bottom_inverse keeps one diagonal entry per thread");
 constexpr int GRIDSTRIDE_BLOCKS = 2048;
+constexpr long SETUP_TILED_MIN_CELLS = 512L * 512;  // levels set up by k_mg_setup_tiled
 
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 inline unsigned gs_blocks(size_t n) { return (unsigned)std::min<size_t>(blocks_for(n), GRIDSTRIDE_BLOCKS); }
@@ -328,8 +329,14 @@ int mg_setup(p2d_ctx* c, MG<T>& mg, bool cast32) {
     double* cbw = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bw) : nullptr;
     double* cbs = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bs) : nullptr;
     if (!wx && !f32bw && !cbw) continue;
-    k_mg_setup<T><<<dim3(blocks_for((size_t)f.nx), (unsigned)f.ny), BLOCK, 0, c->stream>>>(
-        f.nx, f.ny, f.bw, f.bs, wx, wy, f32bw, f32bs, cbw, cbs);
+    // large levels: the tiled sweep (each face reciprocal formed once); small ones
+    // are latency-bound, where the per-cell sweep's shorter chain wins
+    if ((long)f.nx * f.ny >= SETUP_TILED_MIN_CELLS)
+      k_mg_setup_tiled<T><<<dim3((unsigned)((f.nx + STX - 1) / STX), (unsigned)((f.ny + STY - 1) / STY)), BLOCK, 0,
+                            c->stream>>>(f.nx, f.ny, f.bw, f.bs, wx, wy, f32bw, f32bs, cbw, cbs);
+    else
+      k_mg_setup<T><<<dim3(blocks_for((size_t)f.nx), (unsigned)f.ny), BLOCK, 0, c->stream>>>(
+          f.nx, f.ny, f.bw, f.bs, wx, wy, f32bw, f32bs, cbw, cbs);
   }
   if (!common) {
     const Level<double>& B = m.lv[m.L - 1];

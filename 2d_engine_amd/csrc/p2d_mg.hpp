@@ -126,6 +126,101 @@ __global__ void __launch_bounds__(BLOCK) k_mg_setup(int nx, int ny, const double
   }
 }
 
+// od_weight from the faces' reciprocals (k_mg_setup_tiled): the parent weight of
+// fine index i along a row/column whose face coefficients are face(k), k = 0..nf
+// (walls 0), given their reciprocals
+// rface(k) = 1 / face(k) (inf at a wall): the resistance ratio R_far / (R_near +
+// R_far) between the parent and the neighbouring coarse centre; 3/4 for
+// constant coefficients, 1 at a wall.  0.5 / f is formed as 0.5 (1 / f), which
+// is the same number.
+template <class T, class F>
+__device__ __forceinline__ T od_weight_rcp(F rface, int i, int nf) {
+  const int P = i >> 1;
+  const T near_ = (T)0.5 * rface(2 * P + 1);
+  T far_;
+  if ((i & 1) == 0) {
+    if (P == 0) return (T)1;
+    far_ = rface(2 * P) + (T)0.5 * rface(2 * P - 1);
+  } else {
+    if (2 * P + 2 >= nf) return (T)1;
+    far_ = rface(2 * P + 2) + (T)0.5 * rface(2 * P + 3);
+  }
+  const T big = (T)1e30;
+  if (!(near_ < big) || !(far_ < big)) return far_ >= near_ ? (T)1 : (T)0;
+  return far_ / (near_ + far_);
+}
+
+// Coarse face = average over the two fine rows of the series conductance
+// 2 / (0.5/b_{k-1} + 1/b_k + 0.5/b_{k+1}) between the two coarse centres
+// (reciprocals rface as above).
+template <class F>
+__device__ __forceinline__ double coarse_face_rcp(F rface, int k, int nf) {
+  if (k <= 0 || k >= nf) return 0.0;
+  const double r = 0.5 * rface(k - 1) + rface(k) + 0.5 * rface(k + 1);
+  return 2.0 / r;
+}
+
+// k_mg_setup for the large levels, with the same outputs bit for bit.  Tile
+// kernel over STX x STY cells: the reciprocals of the faces the tile's weights
+// and coarse faces read (x faces of rows j0 .. j0 + STY, columns i0 - 1 ..
+// i0 + STX + 1; y faces of columns i0 .. i0 + STX, rows j0 - 1 .. j0 + STY + 1)
+// are formed once into LDS.  Faces beyond the grid are 0 (reciprocal inf), as at
+// the walls.  Grid dim3(ceil(nx/STX), ceil(ny/STY)), BLOCK threads.
+constexpr int STX = 64, STY = 16;
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_mg_setup_tiled(int nx, int ny, const double* __restrict__ bw,
+                                                    const double* __restrict__ bs, T* __restrict__ wx,
+                                                    T* __restrict__ wy, float* __restrict__ f32bw,
+                                                    float* __restrict__ f32bs, double* __restrict__ cbw,
+                                                    double* __restrict__ cbs) {
+  __shared__ double rx[STY + 1][STX + 3], ry[STY + 3][STX + 1];
+  int bx, by;
+  xcd_tile(bx, by);
+  const int i0 = bx * STX, j0 = by * STY;
+  const bool need = wx || cbw;
+  if (need) {
+    for (int t = threadIdx.x; t < (STY + 1) * (STX + 3); t += BLOCK) {
+      const int a = t % (STX + 3), r = t / (STX + 3), q = i0 - 1 + a, jj = j0 + r;
+      const double f = (q >= 0 && q < nx && jj < ny) ? bw[ci(q, jj, nx)] : 0.0;
+      rx[r][a] = 1.0 / f;
+    }
+    for (int t = threadIdx.x; t < (STY + 3) * (STX + 1); t += BLOCK) {
+      const int a = t % (STX + 1), r = t / (STX + 1), ii = i0 + a, q = j0 - 1 + r;
+      const double f = (ii < nx && q >= 0 && q < ny) ? bs[ci(ii, q, nx)] : 0.0;
+      ry[r][a] = 1.0 / f;
+    }
+    __syncthreads();
+  }
+  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  constexpr int NW = BLOCK / 64;
+#pragma unroll
+  for (int m = 0; m < STY / NW; ++m) {
+    const int i = i0 + c, j = j0 + w + NW * m;
+    if (i >= nx || j >= ny) continue;
+    const size_t k = ci(i, j, nx);
+    auto row = [&](int jj) { return [=](int q) { return rx[jj - j0][q - (i0 - 1)]; }; };
+    auto col = [&](int ii) { return [=](int q) { return ry[q - (j0 - 1)][ii - i0]; }; };
+    if (wx) {
+      double a = od_weight_rcp<double>(row(j), i, nx);
+      double b = od_weight_rcp<double>(col(i), j, ny);
+      const int Q = (i >> 1) + ((i & 1) ? 1 : -1), Qj = (j >> 1) + ((j & 1) ? 1 : -1);
+      if (Q < 0 || Q > (nx >> 1) - 1) a = 1.0;
+      if (Qj < 0 || Qj > (ny >> 1) - 1) b = 1.0;
+      wx[k] = (T)a;
+      wy[k] = (T)b;
+    }
+    if (f32bw) {
+      f32bw[k] = (float)bw[k];
+      f32bs[k] = (float)bs[k];
+    }
+    if (cbw && (i & 1) == 0 && (j & 1) == 0) {
+      const int I = i >> 1, J = j >> 1, cnx = nx >> 1;
+      cbw[ci(I, J, cnx)] = 0.5 * (coarse_face_rcp(row(2 * J), 2 * I, nx) + coarse_face_rcp(row(2 * J + 1), 2 * I, nx));
+      cbs[ci(I, J, cnx)] = 0.5 * (coarse_face_rcp(col(2 * I), 2 * J, ny) + coarse_face_rcp(col(2 * I + 1), 2 * J, ny));
+    }
+  }
+}
+
 // Pseudo-inverse of the singular bottom operator A (null space = constants):
 // A+ = (A + a 1 1^T)^-1 - 1 1^T / (a n^2) with a n = mean diagonal; the shifted
 // matrix is SPD and is inverted by Gauss-Jordan without pivoting in one
