@@ -45,7 +45,9 @@ def build():
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # P2D_ORACLE_LIB: another build of the same source (e.g. the sanitizer build of
+        # `make -C oracle asan`)
+        path = os.environ.get("P2D_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
@@ -61,6 +63,8 @@ def lib():
             "orc_cg": (_i, [_i, _pi, _pi, _pd, _pd, _pd, _d, _i, _i, _pd]),
             "orc_pcg_ic0": (_i, [_i, _i, _pi, _pi, _pd, _pd, _pd, _d, _i, _i, _pd]),
             "orc_pcg_jacobi": (_i, [_i, _pi, _pi, _pd, _pd, _pd, _d, _i, _i, _i, _pd]),
+            "orc_ic0_factor": (None, [_i, _i, _pi, _pi, _pd, _pd, _pd, _pd]),
+            "orc_ic0_solve": (None, [_i, _i, _pi, _pi, _pd, _pd, _pd]),
             "orc_apply_A0": (None, [_i, _i, _d, _pd, _pd, _pd, _pd]),
             "orc_rhs_consistent": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
             "orc_update_consistent": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
@@ -322,6 +326,21 @@ def pcg_ic0(A, b, nx, ny, x0=None, tol=1e-10, max_iteration=100000, relative=Tru
     it = lib().orc_pcg_ic0(nx, ny, _p(A.row_ptr), _p(A.col), _p(A.val), _p(x), _p(_f64(b)), tol, max_iteration,
                            1 if relative else 0, ctypes.byref(res))
     return x, it, res.value
+
+
+def ic0_factor(A, nx, ny):
+    """IC(0) factor of a Neumann-box CSR matrix (CSR_MATRIX.h:535-628): per cell the
+    bottom, left and diagonal entries of its row of L, each (ny, nx)."""
+    Lb, Ll, Ld = (np.empty((ny, nx)) for _ in range(3))
+    lib().orc_ic0_factor(nx, ny, _p(A.row_ptr), _p(A.col), _p(A.val), _p(Lb), _p(Ll), _p(Ld))
+    return Lb, Ll, Ld
+
+
+def ic0_solve(A, r, nx, ny):
+    """z = (L L^T)^-1 r by the reference's two triangular solves (PCG_METHOD.cpp:70-118)."""
+    z = np.empty(A.n)
+    lib().orc_ic0_solve(nx, ny, _p(A.row_ptr), _p(A.col), _p(A.val), _p(_f64(r)), _p(z))
+    return z
 
 
 def pcg_jacobi(A, b, x0=None, tol=1e-10, max_iteration=100000, relative=True, threads=0):

@@ -23,16 +23,24 @@
 //   face beta, jump terms, assembly  closed forms, matrix invariants, the
 //                                    Liu-Fedkiw-Kang known answer (test 3) under
 //                                    refinement, balanced-force drop
-//   curvature                        circle and RT closed forms at O(h^2), clamp
+//   curvature                        circle and RT closed forms at O(h^2), clamp,
+//                                    a linear level set (corner cells included)
+//   IC(0) factor and solve           L L^T = A on A's pattern; equal to a dense
+//                                    textbook IC(0) and its triangular solves
+//   CG / IC(0)-PCG iterates          iterate by iterate against dense textbook
+//                                    restatements; iteration counts under the
+//                                    absolute rule and the off-by-one cap
 //   CG / IC(0)-PCG / Jacobi-PCG /    agreement on the mean-zero pressure (the
-//   MG-PCG                           system fixes it), absolute stopping rule
+//   MG-PCG                           system fixes it)
 //   multigrid V-cycle                symmetry, positivity, linearity, spectrum
 //   consistent update, adjoint       D u_proj = -dt r identity, central differences,
 //                                    idempotence and self-adjointness
-// Parity unpinned (only a build of the reference could pin them): IC(0)-PCG
-// iterates and iteration counts (orc_pcg_ic0), the reference-literal smeared
-// density update (orc_update_reference_literal) and the curvature of the four
-// corner cells (orc_curvature).
+//   reference-literal update         closed forms: linear pressure (wall faces against
+//                                    the ghost pressure 0), face-aligned interface,
+//                                    exactly zero level set, mollification band
+// Not reproducible by design: the curvature of the four corner cells, where the
+// reference reads corner ghosts that it never fills (LEVELSET_2D.h:935); here they
+// are extrapolated (DEVIATION in levelset_with_ghosts) and pinned by the linear case.
 
 #include <algorithm>
 #include <cmath>
@@ -394,7 +402,9 @@ struct SolveStats {
   double residual;  // final ||r||_2 (recursive)
 };
 
-// A12: CG_METHOD::CGMethod, CG_METHOD.cpp:13-73.
+// A12: CG_METHOD::CGMethod, CG_METHOD.cpp:13-73.  Returns the number of iterations
+// run; the reference's own counter (num_iteration) stops one short of it when the
+// loop ends by convergence, because the break comes before its increment (:52-56).
 int orc_cg(int N, const int* row_ptr, const int* col, const double* val, double* x, const double* b, double tol,
            int max_iteration, int mode, double* out_res) {
   std::vector<double> r(N), p(N), Ap(N);
@@ -404,7 +414,8 @@ int orc_cg(int N, const int* row_ptr, const int* col, const double* val, double*
   const double thr = mode == 0 ? tol * tol : tol * tol * dot(N, b, b);
   double res_old = dot(N, r.data(), p.data()), res_new = res_old;
   int it = 0;
-  if (res_old < thr) { if (out_res) *out_res = std::sqrt(res_old); return 0; }
+  // As for PCG: the reference tests only after an iteration (CG_METHOD.cpp:38-69).
+  if (mode == 1 && res_old < thr) { if (out_res) *out_res = std::sqrt(res_old); return 0; }
   while (it < max_iteration) {
     orc_csr_spmv(N, row_ptr, col, val, p.data(), Ap.data());
     const double alpha = res_old / dot(N, p.data(), Ap.data());
@@ -424,10 +435,12 @@ int orc_cg(int N, const int* row_ptr, const int* col, const double* val, double*
 }
 
 // A13: IncompleteCholeskyDecomposition (CSR_MATRIX.h:535-628) for a grid whose
-// cells are all full rows in i-fastest order (the Neumann box): textbook IC(0)
-// of the 5-point matrix (Meijerink-van der Vorst) -- L_cb = A_bc / L_bb,
-// L_cl = A_lc / L_ll, L_cc = sqrt(A_cc - L_cb^2 - L_cl^2).  Stored per cell as
-// (Lb, Ll, Ld).
+// cells are all full rows in i-fastest order (the Neumann box): IC(0) of the
+// 5-point matrix (Meijerink-van der Vorst).  Row c of L holds, in this order,
+// the bottom entry L_cb = (1/L_bb) A_bc (CSR_MATRIX.h:567-583), the left entry
+// L_cl = (1/L_ll) A_lc (:585-601) and the diagonal L_cc = sqrt(A_cc - (L_cb^2 +
+// L_cl^2)) summed in row order (:603-620); the reciprocal-then-multiply order is
+// the reference's.  Stored per cell as (Lb, Ll, Ld).
 static void ic0_factor(int nx, int ny, const int* row_ptr, const int* col, const double* val,
                        std::vector<double>& Lb, std::vector<double>& Ll, std::vector<double>& Ld) {
   const int N = nx * ny;
@@ -441,13 +454,16 @@ static void ic0_factor(int nx, int ny, const int* row_ptr, const int* col, const
     for (int i = 0; i < nx; ++i) {
       const int c = i + j * nx;
       double sum = 0;
-      if (j > 0) { Lb[c] = A(c - nx, c) / Ld[c - nx]; sum += Lb[c] * Lb[c]; }
-      if (i > 0) { Ll[c] = A(c - 1, c) / Ld[c - 1]; sum += Ll[c] * Ll[c]; }
+      if (j > 0) { Lb[c] = 1.0 / Ld[c - nx] * A(c - nx, c); sum += Lb[c] * Lb[c]; }
+      if (i > 0) { Ll[c] = 1.0 / Ld[c - 1] * A(c - 1, c); sum += Ll[c] * Ll[c]; }
       Ld[c] = std::sqrt(A(c, c) - sum);
     }
 }
 
-// MultiplicationByMinverse, PCG_METHOD.cpp:70-118: solve L y = b, then L^T x = y.
+// MultiplicationByMinverse, PCG_METHOD.cpp:70-118: solve L y = b row by row, then
+// L^T x = y column by column; each row's sum runs in the CSR order of L (bottom,
+// left), the column sums collect row c + nx before row c + 1, and both divide by
+// multiplying with 1/L_cc, as the reference does.
 static void ic0_apply(int nx, int ny, const std::vector<double>& Lb, const std::vector<double>& Ll,
                       const std::vector<double>& Ld, const double* b, double* x, std::vector<double>& y) {
   const int N = nx * ny;
@@ -456,15 +472,32 @@ static void ic0_apply(int nx, int ny, const std::vector<double>& Lb, const std::
     double s = 0;
     if (c >= nx) s += Lb[c] * y[c - nx];
     if (i > 0) s += Ll[c] * y[c - 1];
-    y[c] = (b[c] - s) / Ld[c];
+    y[c] = (b[c] - s) * (1.0 / Ld[c]);
   }
   for (int c = N - 1; c >= 0; --c) {
     const int i = c % nx;
     double s = 0;
     if (c + nx < N) s += Lb[c + nx] * x[c + nx];
     if (i + 1 < nx) s += Ll[c + 1] * x[c + 1];
-    x[c] = (y[c] - s) / Ld[c];
+    x[c] = (y[c] - s) * (1.0 / Ld[c]);
   }
+}
+
+// The IC(0) factor of a Neumann-box CSR matrix, per cell (test hook of ic0_factor).
+void orc_ic0_factor(int nx, int ny, const int* row_ptr, const int* col, const double* val, double* Lb, double* Ll,
+                    double* Ld) {
+  std::vector<double> b, l, d;
+  ic0_factor(nx, ny, row_ptr, col, val, b, l, d);
+  std::copy(b.begin(), b.end(), Lb);
+  std::copy(l.begin(), l.end(), Ll);
+  std::copy(d.begin(), d.end(), Ld);
+}
+
+// z = (L L^T)^{-1} r with the IC(0) factor of a Neumann-box CSR matrix (test hook of ic0_apply).
+void orc_ic0_solve(int nx, int ny, const int* row_ptr, const int* col, const double* val, const double* r, double* z) {
+  std::vector<double> b, l, d, y((size_t)nx * ny);
+  ic0_factor(nx, ny, row_ptr, col, val, b, l, d);
+  ic0_apply(nx, ny, b, l, d, r, z, y);
 }
 
 // A13: PCG_METHOD::PCGMethod with IC(0), PCG_METHOD.cpp:176-264 (the solver the
@@ -485,7 +518,9 @@ int orc_pcg_ic0(int nx, int ny, const int* row_ptr, const int* col, const double
     if (mode == 0) return res_new < tol * tol;
     return dot(N, r.data(), r.data()) <= tol * tol * bb;
   };
-  if (done()) { if (out_res) *out_res = std::sqrt(dot(N, r.data(), r.data())); return 0; }
+  // The reference's loop has no test before its first iteration (PCG_METHOD.cpp:206-263);
+  // the configs' relative rule (mode 1) accepts a converged initial guess.
+  if (mode == 1 && done()) { if (out_res) *out_res = std::sqrt(dot(N, r.data(), r.data())); return 0; }
   while (true) {
     orc_csr_spmv(N, row_ptr, col, val, p.data(), Ap.data());
     const double alpha = res_new / dot(N, p.data(), Ap.data());
@@ -715,8 +750,15 @@ void orc_update_reference_literal(int nx, int ny, double h, double dt, const dou
 //
 // Level operator (h^2-scaled): (A' x)_c = sum_f beta_f (x_c - x_nb), singular on constants
 // (no-flux walls, no pin on any level).
-// Coarse faces: beta_H = (beta_a + beta_b)/2 over the two fine faces on the coarse face.
-// P = cell-centred bilinear (9,3,3,1)/16 with constant extension at walls; R = P^T.
+// The default mode (3, the one the GPU runs) has
+//   coarse faces: per fine row or column, the series conductance between the two coarse
+//                 centres, 2 / (0.5/b_{k-1} + 1/b_k + 0.5/b_{k+1}), averaged over the two
+//                 fine rows that meet the coarse face (coarse_face);
+//   P: tensor product of 1-D operator-dependent weights, w on the parent coarse cell and
+//      1 - w on its neighbour, w = R_far / (R_near + R_far) from the fine faces' resistances
+//      (od_weight; 3/4 for constant coefficients, 1 at a wall); R = P^T.
+// Mode 0 (tests only) keeps the plain forms: beta_H = (beta_a + beta_b)/2 and cell-centred
+// bilinear P with weights (9,3,3,1)/16.
 // Smoother: red-black Gauss-Seidel, pre (red, black) x nu, post (black, red) x nu.
 // Bottom: pseudo-inverse of the coarsest operator (constants in its null space), so the
 // V-cycle treats the constant mode exactly as the singular fine operator does.
