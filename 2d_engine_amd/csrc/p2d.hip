@@ -706,6 +706,10 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
       cfg->max_iterations < 1 || cfg->max_outer < 1 || (cfg->precision != P2D_FP64 && cfg->precision != P2D_MIXED) ||
       (!(cfg->rtol > 0) && !(cfg->div_tol > 0)) || !(cfg->inner_rtol > 0 && cfg->inner_rtol < 1))
     return P2D_BAD_ARG;
+  // The tile kernels address a level's arrays with 32-bit byte offsets (ld_at / st_at):
+  // the largest array, nx*ny doubles, must stay below 2^32 bytes.
+  if ((unsigned long long)cfg->nx * (unsigned long long)cfg->ny * sizeof(double) > (unsigned long long)P2D_MAX_ARRAY_BYTES)
+    return P2D_BAD_ARG;
   {
     // the V-cycle needs at least one coarsening and a bottom that fits the dense solve
     const int L = num_levels(cfg->nx, cfg->ny, cfg->mg_max_levels);

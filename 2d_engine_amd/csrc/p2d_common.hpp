@@ -238,9 +238,10 @@ __device__ __forceinline__ double lane_next(double v) {
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// Tile kernels address their level's arrays with 32-bit byte offsets (a level
-// holds fewer than 2^32 bytes per array; p2d_create checks it), so a load or
-// store takes the array's base in scalar registers and one VGPR offset.
+// Tile kernels address their level's arrays with 32-bit byte offsets, so a load
+// or store takes the array's base in scalar registers and one VGPR offset.  This
+// holds because p2d_create rejects grids whose FP64 cell array (the largest array
+// of any level) reaches 2^32 bytes (P2D_MAX_ARRAY_BYTES).
 __device__ __forceinline__ unsigned cell_u(int i, int j, int nx) { return (unsigned)i + (unsigned)j * (unsigned)nx; }
 template <class V, class T>
 __device__ __forceinline__ V ld_at(const T* base, unsigned cell) {

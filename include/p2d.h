@@ -49,6 +49,11 @@ enum {
 };
 
 enum { P2D_MEM_HOST = 0, P2D_MEM_DEVICE = 1 };
+
+/* Largest array the kernels address, in bytes: p2d_create rejects grids with
+ * nx * ny * sizeof(double) above it (at most 536,870,911 cells; 23170^2 fits). */
+#define P2D_MAX_ARRAY_BYTES 0xFFFFFFFFull
+
 enum { P2D_FP64 = 0, P2D_MIXED = 1 }; /* solver precision */
 enum { P2D_PREC_F64 = 0, P2D_PREC_F32 = 1 }; /* element type of test-hook buffers */
 
@@ -82,7 +87,9 @@ typedef struct p2d_ctx p2d_ctx;
 void p2d_config_default(p2d_config* cfg, int nx, int ny, double h);
 
 /* PROJECTION_2D constructor + InitializeFromBlock (PROJECTION_2D.h:107-232):
- * allocates every device buffer for the grid once. */
+ * allocates every device buffer for the grid once.  P2D_BAD_ARG, before any device
+ * call, for invalid options, nx * ny * 8 > P2D_MAX_ARRAY_BYTES, or sizes that do not
+ * halve down to a bottom grid of at most 1024 cells. */
 int p2d_create(const p2d_config* cfg, p2d_ctx** out);
 int p2d_destroy(p2d_ctx* ctx);
 

@@ -48,6 +48,24 @@ def test_bad_arguments_rejected_without_compute(P2D):
     assert N.status_string(N.P2D_NAN) == "NaN or breakdown"
 
 
+def test_array_size_limit_checked_before_any_device_call(P2D):
+    """The tile kernels use 32-bit byte offsets: a grid whose FP64 cell array reaches 2^32
+    bytes is refused with P2D_BAD_ARG before the library touches a device, while one just
+    below the limit gets past every argument check (here, with no GPU, to P2D_NO_DEVICE)."""
+    N = P2D._native
+    h = ctypes.c_void_p()
+    for n in (23552, 24576):                    # 23*1024 and 24*1024: legal bottoms, > 4 GiB arrays
+        cfg = N.default_config(n, n, 1.0 / n)
+        assert n * n * 8 > 0xFFFFFFFF
+        assert N.lib.p2d_create(ctypes.byref(cfg), ctypes.byref(h)) == N.P2D_BAD_ARG
+    import torch
+    if torch.cuda.device_count() > 0:
+        return   # the grid below the limit would allocate ~230 GB on a device
+    cfg = N.default_config(32768, 15360, 1.0 / 32768)   # 503 M cells, bottom 32 x 15
+    assert 32768 * 15360 * 8 <= 0xFFFFFFFF
+    assert N.lib.p2d_create(ctypes.byref(cfg), ctypes.byref(h)) == N.P2D_NO_DEVICE
+
+
 def test_status_codes_match_header():
     src = open(os.path.join(ROOT, "include", "p2d.h")).read()
     codes = dict(re.findall(r"(P2D_[A-Z_]+)\s*=\s*(\d+)", src))
