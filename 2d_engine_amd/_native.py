@@ -43,7 +43,7 @@ EXPORTS = [
     "p2d_kernel_time", "p2d_status_string", "p2d_reinitialize", "p2d_levelset_normals", "p2d_advect_levelset",
     "p2d_advect_velocity", "p2d_apply_viscosity", "p2d_step_params_default", "p2d_advance",
     "p2d_cfl_dt", "p2d_visc_create", "p2d_visc_destroy", "p2d_visc_set_stream", "p2d_visc_solve",
-    "p2d_oil_water_params_default", "p2d_visc_oil_water",
+    "p2d_oil_water_params_default", "p2d_visc_oil_water", "p2d_divergence", "p2d_solve_pressure",
 ]
 
 
@@ -54,7 +54,7 @@ class P2DConfig(ctypes.Structure):
         ("mg_pre_sweeps", ctypes.c_int), ("mg_post_sweeps", ctypes.c_int),
         ("rtol", ctypes.c_double), ("div_tol", ctypes.c_double),
         ("max_iterations", ctypes.c_int), ("max_outer", ctypes.c_int),
-        ("inner_rtol", ctypes.c_double),
+        ("inner_rtol", ctypes.c_double), ("abs_tol", ctypes.c_double),
     ]
 
 
@@ -108,6 +108,8 @@ def _load():
         "p2d_set_coefficients": (i, [vp, vp, vp, vp, vp, i]),
         "p2d_project": (i, [vp, vp, vp, d, stp, i]),
         "p2d_get_pressure": (i, [vp, vp, i, i]),
+        "p2d_divergence": (i, [vp, vp, vp, d, vp, i]),
+        "p2d_solve_pressure": (i, [vp, vp, vp, stp, i]),
         "p2d_get_coefficients": (i, [vp, vp, vp, vp, i]),
         "p2d_adjoint": (i, [vp, vp, vp, vp, vp, vp, vp, stp, i]),
         "p2d_apply_stencil": (i, [vp, vp, vp, i, i]),
@@ -264,6 +266,26 @@ class Context:
         b = _Buf(out, self.cell_shape, writable=True)
         _check(lib.p2d_get_pressure(self._h, b.ptr, 1 if mean_zero else 0, b.mem), "p2d_get_pressure")
         return out
+
+    def divergence(self, u, v, dt=1.0):
+        """DetermineDivergence: (D u) / dt per cell, numpy or a torch tensor on the device."""
+        bu, bv = _Buf(u, self.u_shape), _Buf(v, self.v_shape)
+        out = np.empty(self.cell_shape) if bu.mem == MEM_HOST else u.new_empty(self.cell_shape)
+        bo = _Buf(out, self.cell_shape, writable=True)
+        _check(lib.p2d_divergence(self._h, bu.ptr, bv.ptr, dt, bo.ptr, _mem(bu, bv, bo)), "p2d_divergence")
+        return out
+
+    def solve_pressure(self, div, allow_not_converged=False):
+        """DeterminePressure for a given divergence field: returns (p in the reference's gauge
+        p(0,0) = 0, stats); the pressure is also kept for pressure()."""
+        bd = _Buf(div, self.cell_shape)
+        out = np.empty(self.cell_shape) if bd.mem == MEM_HOST else div.new_empty(self.cell_shape)
+        bo = _Buf(out, self.cell_shape, writable=True)
+        st = P2DStats()
+        allow = (P2D_OK, P2D_NOT_CONVERGED) if allow_not_converged else (P2D_OK,)
+        _check(lib.p2d_solve_pressure(self._h, bd.ptr, bo.ptr, ctypes.byref(st), _mem(bd, bo)),
+               "p2d_solve_pressure", allow)
+        return out, st.as_dict()
 
     def coefficients(self):
         bx, by, jc = np.empty(self.u_shape), np.empty(self.v_shape), np.empty(self.cell_shape)

@@ -141,6 +141,7 @@ struct p2d_ctx {
   double *st_phi = nullptr, *st_phin = nullptr, *st_u = nullptr, *st_v = nullptr, *st_un = nullptr, *st_vn = nullptr;
   double* sa[4] = {nullptr, nullptr, nullptr, nullptr};
   bool have_pressure = false;
+  bool pressure_of_projection = false;  // the kept pressure is a p2d_project's (the adjoint needs one)
   double last_dt = 1.0;
 
   KVec<double> v64;
@@ -474,7 +475,7 @@ struct Rhs {
 
 // Solves A0' x = b' - jumps (h^2-scaled, mean removed) for x = sol[*cur], the
 // pressure in double-double.  rtol: ||r||_2 <= rtol ||r_0||_2; atol:
-// ||r||_inf <= atol (scaled); <= 0 disables.
+// ||r||_inf <= atol (scaled); abs2: ||r||_2^2 <= abs2 (scaled); <= 0 disables.
 //
 // The host enqueues inner iterations in chunks sized from the previous solve's
 // pass lengths; the kernels skip work once s->done is set.  When a pass (inner
@@ -484,7 +485,7 @@ struct Rhs {
 // the device, so a pass costs one host synchronisation.
 template <class T>
 int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double* solh[2], double* soll[2],
-            int* cur, double rtol, double atol, SolveResult* out) {
+            int* cur, double rtol, double atol, double abs2, SolveResult* out) {
   const size_t N = c->N;
   const bool mixed = sizeof(T) == 4;
   *cur = 0;
@@ -494,7 +495,8 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
   const double rel = mixed ? c->cfg.inner_rtol : 1e-14;
   const bool verbose = std::getenv("P2D_VERBOSE") != nullptr;
   const auto t_solve = std::chrono::steady_clock::now();
-  k_inner_init<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? 0 : 1, rtol, atol, c->cfg.max_iterations);
+  k_inner_init<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? 0 : 1, rtol, atol, abs2,
+                                       c->cfg.max_iterations);
   k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
   CHK(launched_ok());
   int status = P2D_OK, outer = 0, stalled = 0, iters = 0, seg_start = 0;
@@ -603,7 +605,7 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
     rinf = c->hs->red[1];
   }
   auto converged = [&](double a2, double ai) {
-    return (rtol <= 0 || a2 <= rtol * rtol * b2) && (atol <= 0 || ai <= atol);
+    return (rtol <= 0 || a2 <= rtol * rtol * b2) && (atol <= 0 || ai <= atol) && (abs2 <= 0 || a2 <= abs2);
   };
   if (status == P2D_OK && !converged(r2, rinf)) status = P2D_NOT_CONVERGED;
   out->status = status;
@@ -616,13 +618,13 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
 }
 
 int solve(p2d_ctx* c, const Rhs& rhs, double* solh[2], double* soll[2], int* cur, double rtol, double atol,
-          SolveResult* out) {
+          double abs2, SolveResult* out) {
   if (c->cfg.precision == P2D_MIXED) {
     CHK(ensure_mg(c, P2D_PREC_F32));
-    return solve_t<float>(c, c->mg32, c->v32, 1, rhs, solh, soll, cur, rtol, atol, out);
+    return solve_t<float>(c, c->mg32, c->v32, 1, rhs, solh, soll, cur, rtol, atol, abs2, out);
   }
   CHK(ensure_mg(c, P2D_PREC_F64));
-  return solve_t<double>(c, c->mg64, c->v64, 0, rhs, solh, soll, cur, rtol, atol, out);
+  return solve_t<double>(c, c->mg64, c->v64, 0, rhs, solh, soll, cur, rtol, atol, abs2, out);
 }
 
 int stage_in(p2d_ctx* c, double* dst, const double* src, size_t n, int mem) {
@@ -664,6 +666,45 @@ __global__ void k_pattern(size_t n, T* a) {
   if (k < n) a[k] = (T)(((k * 2654435761u) % 1000u) * 1e-3 - 0.5);
 }
 
+// Coefficients of the current fluid: face beta and sigma kappa (k_coeff) unless
+// they are explicit or reused, then the per-face jump terms s a (k_jump).
+int assemble(p2d_ctx* c) {
+  if (!c->explicit_coeff && !(c->reuse_coeff && c->coeff_ready)) {
+    HIPCHK(hipMemsetAsync(&c->s->err, 0, sizeof(int), c->stream));
+    k_coeff<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, c->phi, 1.0 / c->rho_m,
+                                                          1.0 / c->rho_p, c->sigma, c->bw, c->bs, c->jc, c->s);
+    CHK(launched_ok());
+    c->coeff_ready = true;
+    invalidate(c);
+  }
+  if (c->has_jc)
+    k_jump<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->phi, c->jc, c->saw, c->sas);
+  return launched_ok();
+}
+
+// b' and the first residual in the solve precision, from the velocity (du, dv, dt)
+// or from a given divergence field (div); norms into s->red, mean into s->bmean.
+int build_rhs(p2d_ctx* c, const double* du, const double* dv, const double* div, double dt) {
+  const double* saw = c->has_jc ? c->saw : nullptr;
+  const double* sas = c->has_jc ? c->sas : nullptr;
+  if (c->cfg.precision == P2D_MIXED)
+    k_rhs<float><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, div, c->bw, c->bs,
+                                                               saw, sas, c->b, c->v32.r[0], c->partial);
+  else
+    k_rhs<double><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, div, c->bw, c->bs,
+                                                                saw, sas, c->b, c->v64.r[0], c->partial);
+  k_fin_rhs<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx, c->ny)), (double)c->N, c->s);
+  return launched_ok();
+}
+
+// The configuration's absolute rule ||r||_2 <= abs_tol on the unscaled system, as a
+// target on the sum of squares of the h^2-scaled residual (0: off).
+double abs2_target(const p2d_ctx* c) {
+  if (!(c->cfg.abs_tol > 0)) return 0.0;
+  const double t = c->cfg.abs_tol * c->h * c->h;
+  return t * t;
+}
+
 }  // namespace
 
 // =========================================================================== C ABI
@@ -684,6 +725,7 @@ void p2d_config_default(p2d_config* cfg, int nx, int ny, double h) {
   cfg->max_iterations = 1000;
   cfg->max_outer = 12;
   cfg->inner_rtol = 1e-4;
+  cfg->abs_tol = 0.0;
 }
 
 const char* p2d_status_string(int st) {
@@ -704,7 +746,8 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
   if (cfg->nx < 2 || cfg->ny < 2 || cfg->nx > 65535 || cfg->ny > 65535 || !(cfg->h > 0) || cfg->mg_pre_sweeps < 1 ||
       cfg->mg_pre_sweeps > MAX_NU || cfg->mg_post_sweeps < 1 || cfg->mg_post_sweeps > MAX_NU ||
       cfg->max_iterations < 1 || cfg->max_outer < 1 || (cfg->precision != P2D_FP64 && cfg->precision != P2D_MIXED) ||
-      (!(cfg->rtol > 0) && !(cfg->div_tol > 0)) || !(cfg->inner_rtol > 0 && cfg->inner_rtol < 1))
+      (!(cfg->rtol > 0) && !(cfg->div_tol > 0) && !(cfg->abs_tol > 0)) || !(cfg->abs_tol >= 0) ||
+      !(cfg->inner_rtol > 0 && cfg->inner_rtol < 1))
     return P2D_BAD_ARG;
   // The tile kernels address a level's arrays with 32-bit byte offsets (ld_at / st_at):
   // the largest array, nx*ny doubles, must stay below 2^32 bytes.
@@ -875,29 +918,14 @@ int p2d_project(p2d_ctx* c, double* u, double* v, double dt, p2d_stats* st, int 
     dv = c->sv;
   }
   c->hs->err = 0;
-  if (!c->explicit_coeff && !(c->reuse_coeff && c->coeff_ready)) {
-    HIPCHK(hipMemsetAsync(&c->s->err, 0, sizeof(int), c->stream));
-    k_coeff<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, c->phi, 1.0 / c->rho_m,
-                                                          1.0 / c->rho_p, c->sigma, c->bw, c->bs, c->jc, c->s);
-    CHK(launched_ok());
-    c->coeff_ready = true;
-    invalidate(c);
-  }
+  CHK(assemble(c));
   const double* saw = c->has_jc ? c->saw : nullptr;
   const double* sas = c->has_jc ? c->sas : nullptr;
-  if (c->has_jc)
-    k_jump<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->phi, c->jc, c->saw, c->sas);
-  if (c->cfg.precision == P2D_MIXED)
-    k_rhs<float><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, c->bw, c->bs, saw, sas,
-                                                               c->b, c->v32.r[0], c->partial);
-  else
-    k_rhs<double><<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, c->bw, c->bs, saw, sas,
-                                                                c->b, c->v64.r[0], c->partial);
-  k_fin_rhs<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(bgrid(c->nx, c->ny)), (double)c->N, c->s);
-  CHK(launched_ok());
+  CHK(build_rhs(c, du, dv, nullptr, dt));
   const double atol = c->cfg.div_tol > 0 ? 0.5 * c->cfg.div_tol * c->h * c->h / dt : 0.0;
   SolveResult sr;
-  CHK(solve(c, Rhs{c->b, saw, sas}, c->ph, c->pl, &c->pcur, c->cfg.rtol, atol, &sr));
+  CHK(solve(c, Rhs{c->b, saw, sas}, c->ph, c->pl, &c->pcur, c->cfg.rtol, atol, abs2_target(c), &sr));
+  c->pressure_of_projection = true;
   const bool bad_geom = c->hs->err != 0;
   const double b2 = sr.b2;
   k_face_update<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, c->ph[c->pcur],
@@ -942,6 +970,57 @@ int p2d_get_pressure(p2d_ctx* c, double* p, int mean_zero, int mem) {
   return P2D_OK;
 }
 
+int p2d_divergence(p2d_ctx* c, const double* u, const double* v, double dt, double* div, int mem) {
+  if (!c || !u || !v || !div || !(dt > 0)) return P2D_BAD_ARG;
+  const double* du = u;
+  const double* dv = v;
+  double* out = div;
+  if (mem == P2D_MEM_HOST) {
+    CHK(stage_in(c, c->su, u, c->NU, mem));
+    CHK(stage_in(c, c->sv, v, c->NV, mem));
+    du = c->su;
+    dv = c->sv;
+    out = c->sp;
+  }
+  k_divergence<<<bgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, dt, du, dv, out);
+  CHK(launched_ok());
+  if (mem == P2D_MEM_HOST) CHK(stage_out(c, div, c->sp, c->N, mem));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return P2D_OK;
+}
+
+int p2d_solve_pressure(p2d_ctx* c, const double* div, double* p, p2d_stats* st, int mem) {
+  if (!c || !div || !c->have_fluid) return P2D_BAD_ARG;
+  const double* dd = div;
+  if (mem == P2D_MEM_HOST) {
+    CHK(stage_in(c, c->sg1, div, c->N, mem));
+    dd = c->sg1;
+  }
+  c->hs->err = 0;
+  CHK(assemble(c));
+  const double* saw = c->has_jc ? c->saw : nullptr;
+  const double* sas = c->has_jc ? c->sas : nullptr;
+  CHK(build_rhs(c, nullptr, nullptr, dd, 1.0));
+  const double atol = c->cfg.div_tol > 0 ? 0.5 * c->cfg.div_tol * c->h * c->h : 0.0;
+  SolveResult sr;
+  CHK(solve(c, Rhs{c->b, saw, sas}, c->ph, c->pl, &c->pcur, c->cfg.rtol, atol, abs2_target(c), &sr));
+  const bool bad_geom = c->hs->err != 0;
+  c->have_pressure = true;
+  c->pressure_of_projection = false;
+  if (p) CHK(p2d_get_pressure(c, p, 0, mem));
+  if (st) {
+    st->status = bad_geom ? P2D_NAN : sr.status;
+    st->iterations = sr.iterations;
+    st->outer = sr.outer;
+    st->rel_residual = sr.b2 > 0 ? std::sqrt(sr.r2 / sr.b2) : 0.0;
+    st->max_abs_residual = sr.rinf / (c->h * c->h);
+    st->max_div = 0;
+    st->max_u = st->max_v = 0;
+  }
+  if (bad_geom) return P2D_NAN;
+  return sr.status;
+}
+
 int p2d_get_coefficients(p2d_ctx* c, double* bx, double* by, double* jc, int mem) {
   if (!c || !c->coeff_ready) return P2D_BAD_ARG;
   k_cells_to_faces<<<bgrid(c->nx + 1, c->ny + 1), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, c->sa[2],
@@ -956,7 +1035,7 @@ int p2d_get_coefficients(p2d_ctx* c, double* bx, double* by, double* jc, int mem
 
 int p2d_adjoint(p2d_ctx* c, const double* gu, const double* gv, double* du, double* dv, double* dbx, double* dby,
                 p2d_stats* st, int mem) {
-  if (!c || !gu || !gv || !c->have_pressure) return P2D_BAD_ARG;
+  if (!c || !gu || !gv || !c->have_pressure || !c->pressure_of_projection) return P2D_BAD_ARG;
   const double* g1 = gu;
   const double* g2 = gv;
   if (mem == P2D_MEM_HOST) {
@@ -976,7 +1055,7 @@ int p2d_adjoint(p2d_ctx* c, const double* gu, const double* gv, double* du, doub
   SolveResult sr;
   int mcur = 0;
   const double rt = c->cfg.rtol > 0 ? c->cfg.rtol : 1e-10;
-  CHK(solve(c, Rhs{c->b, nullptr, nullptr}, c->muh, c->mul, &mcur, rt, 0.0, &sr));
+  CHK(solve(c, Rhs{c->b, nullptr, nullptr}, c->muh, c->mul, &mcur, rt, 0.0, 0.0, &sr));
   const double b2 = sr.b2;
   const double* saw = c->has_jc ? c->saw : nullptr;
   const double* sas = c->has_jc ? c->sas : nullptr;

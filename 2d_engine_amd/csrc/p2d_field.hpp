@@ -129,20 +129,24 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_rhs(const double* __rest
 // part enters the residual face by face (fdiff).  Also the initial residual
 // (p = 0) r0 = b - sum_f +-beta_f s_f a_f, stored as R for the first solve
 // pass (its mean is removed there).  u: (nx+1) x ny x-faces, v: nx x (ny+1)
-// y-faces; saw/sas may be NULL (no jump).
+// y-faces; saw/sas may be NULL (no jump).  With div given (nx x ny, the
+// reference's divergence_field, already divided by dt) u, v and dt are unused
+// and b = -h^2 div.
 template <class R>
 __global__ void __launch_bounds__(BLOCK) k_rhs(int nx, int ny, double h, double dt, const double* __restrict__ u,
-                                               const double* __restrict__ v, const double* __restrict__ bw,
-                                               const double* __restrict__ bs, const double* __restrict__ saw,
-                                               const double* __restrict__ sas, double* __restrict__ b,
-                                               R* __restrict__ r0, double* partial) {
+                                               const double* __restrict__ v, const double* __restrict__ div,
+                                               const double* __restrict__ bw, const double* __restrict__ bs,
+                                               const double* __restrict__ saw, const double* __restrict__ sas,
+                                               double* __restrict__ b, R* __restrict__ r0, double* partial) {
 #pragma clang fp contract(off)
   const double ih = 1.0 / h, idt = 1.0 / dt, h2 = h * h;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   P2D_BAND(ny);
   for (int j = j0; j < j1 && i < nx; ++j) {
     const size_t k = ci(i, j, nx);
-    const double d = (ih * (u[fxi(i + 1, j, nx)] - u[fxi(i, j, nx)]) + ih * (v[fyi(i, j + 1, nx)] - v[fyi(i, j, nx)])) * idt;
+    const double d =
+        div ? div[k]
+            : (ih * (u[fxi(i + 1, j, nx)] - u[fxi(i, j, nx)]) + ih * (v[fyi(i, j + 1, nx)] - v[fyi(i, j, nx)])) * idt;
     const double bv = h2 * (-d);
     double g = 0.0;
     if (saw) {
@@ -160,6 +164,22 @@ __global__ void __launch_bounds__(BLOCK) k_rhs(int nx, int ny, double h, double 
     acc[3] += bv;
   }
   block_partial<4>(acc, 2u, partial);
+}
+
+// PROJECTION_2D::DetermineDivergence (air-water, PROJECTION_2D.h:395-414):
+// div = (1/dx (u_{i+1} - u_i) + 1/dy (v_{j+1} - v_j)) then div *= 1/dt, in that
+// operation order (dx = dy = h).
+__global__ void __launch_bounds__(BLOCK) k_divergence(int nx, int ny, double h, double dt,
+                                                      const double* __restrict__ u, const double* __restrict__ v,
+                                                      double* __restrict__ div) {
+#pragma clang fp contract(off)
+  const double ih = 1.0 / h, idt = 1.0 / dt;
+  P2D_BAND(ny);
+  for (int j = j0; j < j1 && i < nx; ++j) {
+    double d = ih * (u[fxi(i + 1, j, nx)] - u[fxi(i, j, nx)]) + ih * (v[fyi(i, j + 1, nx)] - v[fyi(i, j, nx)]);
+    d *= idt;
+    div[ci(i, j, nx)] = d;
+  }
 }
 
 // --------------------------------------------------------------------------- FP64 refinement step

@@ -364,18 +364,20 @@ __device__ __forceinline__ void segment_start(Scal* s, double ncells) {
   s->rinf = s->red[1];
 }
 // Start of a solve from the initial residual's norms in red: the reference
-// norm b2 (mean-free), the outer targets (rtol on ||r||_2 relative to b2, atol
-// on max |r|; <= 0 disables) and the first segment's inner targets: relative
+// norm b2 (mean-free), the outer targets (rtol on ||r||_2 relative to b2, abs2
+// on ||r||_2^2, atol on max |r|; <= 0 disables; the sum-of-squares targets
+// combine into the smaller one) and the first segment's inner targets: relative
 // rel2 and, when inner_abs, absolute ones with a margin for the recursive
 // residual (0.25 out2, 0.5 outi).  Sets outer_done when r0 already meets the
 // outer targets.
 __global__ void k_inner_init(Scal* s, double ncells, double rel2, int inner_abs, double rtol, double atol,
-                             int max_iter) {
+                             double abs2, int max_iter) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   const double m = s->red[2] / ncells;
   const double b2 = fmax(s->red[0] - ncells * m * m, 0.0);
   s->b2 = b2;
   s->out2 = rtol > 0.0 ? rtol * rtol * b2 : -1.0;
+  if (abs2 > 0.0) s->out2 = s->out2 < 0.0 ? abs2 : fmin(s->out2, abs2);
   s->outi = atol > 0.0 ? atol : -1.0;
   segment_start(s, ncells);
   s->rel2 = rel2;

@@ -11,8 +11,14 @@ meaning follow it so a caller of the reference finds the same surface:
          CSF_model, air_water_simulation          written by the caller,
                                                  EULERIAN_FLUID_SOLVER_2D.h:506-515
   Solve(dt)                                      :235-242
-  DeterminePressure(dt)                          :480-509 (pressure only)
-  pressure_field, max_velocity_x, max_velocity_y outputs (:29, :71-72)
+  DetermineDivergence(dt)                        :395-478 -> divergence_field
+  DeterminePressure()                            :480-509 (the solve for divergence_field)
+  pressure_field, max_velocity_x, max_velocity_y outputs (:29, :71-72); pressure_field in
+                                                 the reference's gauge p(0,0) = 0
+
+`tolerance` keeps the reference's absolute rule ||b - A p||_2 < tolerance
+(CG_METHOD.cpp:52).  Two further rules, off (0) by default, may be added:
+relative_tolerance (||r||_2 <= it * ||b||_2) and div_tolerance (max|D u_proj|).
 
 Error behaviour: like the reference, a solve that hits max_iteration returns
 silently (its stats say so); invalid input and device failures raise P2DError.
@@ -49,7 +55,9 @@ class PROJECTION_2D:
         self.water_density = 1000.0
         self.air_density = 1.226
         self.surface_tension = 0.0728
-        self.div_tolerance = 1e-10
+        self.relative_tolerance = 0.0
+        self.div_tolerance = 0.0
+        self.divergence_field = None
         self.pressure_field = None
         self.max_velocity_x = 0.0
         self.max_velocity_y = 0.0
@@ -66,6 +74,7 @@ class PROJECTION_2D:
         self.Dirichlet_Boundary_Condition = bool(block.get("Dirichlet_Boundary_Condition", False))
         self.Neumann_Boundary_Condition = bool(block.get("Neumann_Boundary_Condition", False))
         self.poisson_solver_type = str(block.get("poisson_solver_type", "PCG")).upper()
+        self.relative_tolerance = float(block.get("relative_tolerance", self.relative_tolerance))
         self.div_tolerance = float(block.get("div_tolerance", self.div_tolerance))
 
     # ------------------------------------------------------------------ internals
@@ -77,10 +86,11 @@ class PROJECTION_2D:
         prec = _SOLVER_TYPES.get(self.poisson_solver_type)
         if prec is None:
             raise ValueError(f"unknown poisson_solver_type {self.poisson_solver_type}")
-        key = (prec, self.tolerance, self.max_iteration, self.div_tolerance)
+        key = (prec, self.tolerance, self.max_iteration, self.relative_tolerance, self.div_tolerance)
         if self._ctx is None or self._ctx_key != key:
-            self._ctx = N.Context(self.nx, self.ny, self.dx, precision=prec, rtol=self.tolerance,
-                                  div_tol=self.div_tolerance, max_iterations=max(1, self.max_iteration))
+            self._ctx = N.Context(self.nx, self.ny, self.dx, precision=prec, abs_tol=self.tolerance,
+                                  rtol=self.relative_tolerance, div_tol=self.div_tolerance,
+                                  max_iterations=max(1, self.max_iteration))
             self._ctx_key = key
         return self._ctx
 
@@ -111,16 +121,19 @@ class PROJECTION_2D:
         self.pressure_field = ctx.pressure(mean_zero=False)
         return st
 
-    def DeterminePressure(self, dt=1.0):
-        """Pressure only: the velocity fields are left untouched."""
+    def DetermineDivergence(self, dt):
+        """divergence_field = (D u) / dt of the current face velocities (on the GPU)."""
+        ctx = self._context()
+        self.divergence_field = ctx.divergence(self.water_velocity_field_mac_x, self.water_velocity_field_mac_y, dt)
+        return self.divergence_field
+
+    def DeterminePressure(self):
+        """The ghost-fluid pressure solve for whatever divergence_field holds (set by
+        DetermineDivergence or by the caller); the velocity fields are not touched."""
+        if self.divergence_field is None:
+            raise ValueError("divergence_field is not set (call DetermineDivergence or assign it)")
         ctx = self._context()
         rho_m, rho_p = self._densities()
         ctx.set_fluid(self.water_levelset, rho_m, rho_p, self._sigma())
-        u = self.water_velocity_field_mac_x
-        v = self.water_velocity_field_mac_y
-        if hasattr(u, "data_ptr"):
-            u, v = u.clone(), v.clone()
-        _, _, st = ctx.project(u, v, dt, allow_not_converged=True)
-        self.stats = st
-        self.pressure_field = ctx.pressure(mean_zero=False)
+        self.pressure_field, self.stats = ctx.solve_pressure(self.divergence_field, allow_not_converged=True)
         return self.pressure_field

@@ -64,11 +64,18 @@ typedef struct p2d_config {
   int mg_max_levels;   /* 0: coarsen until <= 64 cells or an odd size */
   int mg_pre_sweeps;   /* red-black Gauss-Seidel sweeps before restriction, 1..3 (default 3) */
   int mg_post_sweeps;  /* ... after prolongation, 1..3 (default 3; equal counts keep M symmetric) */
-  double rtol;         /* stop when ||r||_2 <= rtol ||b||_2 (0 disables); PROJECTION_2D's `tolerance` */
-  double div_tol;      /* stop when the predicted max|D u_proj| <= div_tol (0 disables) */
+  /* Stopping rules: every enabled one must hold for the true (FP64) residual r = b - A p of
+   * the unscaled system (entries beta/h^2, b = -div - F as POISSON_SOLVER.cpp:951 builds it).
+   * At least one of rtol, div_tol, abs_tol must be > 0. */
+  double rtol;         /* ||r||_2 <= rtol ||b||_2 (mean-free b; 0 disables).  The configurations' rule */
+  double div_tol;      /* the predicted max|D u_proj| = dt max|r| <= div_tol (0 disables); for
+                        * p2d_solve_pressure, max|r| <= div_tol */
   int max_iterations;  /* cap on inner Krylov iterations per solve (PROJECTION_2D's `max_iteration`) */
   int max_outer;       /* cap on FP64 refinement passes */
   double inner_rtol;   /* mixed: relative residual reduction of each FP32 inner solve (default 1e-4) */
+  double abs_tol;      /* ||r||_2 <= abs_tol (0 disables, the default): PROJECTION_2D's `tolerance`
+                        * with the reference's absolute rule, res < tolerance^2 on r.r
+                        * (CG_METHOD.cpp:52; its IC(0)-PCG tests r.z, PCG_METHOD.cpp:253) */
 } p2d_config;
 
 typedef struct p2d_stats {
@@ -116,6 +123,22 @@ int p2d_project(p2d_ctx* ctx, double* u, double* v, double dt, p2d_stats* stats,
  * 0: shifted so p(0,0) = 0, the reference's pin (POISSON_SOLVER.cpp:789-807). */
 int p2d_get_pressure(p2d_ctx* ctx, double* p, int mean_zero, int mem);
 
+/* PROJECTION_2D::DetermineDivergence(dt) (air-water, PROJECTION_2D.h:395-414): the cell field
+ * div = (D u) / dt that the projection's right-hand side is built from (nx*ny; the
+ * reference's divergence_field, in its operation order). */
+int p2d_divergence(p2d_ctx* ctx, const double* u, const double* v, double dt, double* div, int mem);
+
+/* PROJECTION_2D::DeterminePressure() (PROJECTION_2D.h:480-509, called directly by the solver at
+ * EULERIAN_FLUID_SOLVER_2D.h:1513, 1681): the ghost-fluid Poisson solve A p = -div - F
+ * (POISSON_SOLVER::Solve, POISSON_SOLVER.cpp:127-145; A and F of the current fluid, as in
+ * p2d_project) for a caller-written divergence_field div (nx*ny, already divided by dt).  The
+ * pressure is kept in the context (p2d_get_pressure) and, when p is not NULL, written to p in
+ * the reference's gauge p(0,0) = 0.  The no-flux operator is singular on constants, so the mean
+ * of -div - F is removed first; for a compatible right-hand side (zero sum: any div = D u / dt
+ * of wall-tight faces) the result is the reference's pinned solution.  Stops on the
+ * configuration's rules, with div_tol bounding max|r|.  Velocities are not touched. */
+int p2d_solve_pressure(p2d_ctx* ctx, const double* div, double* p, p2d_stats* stats, int mem);
+
 /* Coefficients built by the last p2d_project / p2d_set_coefficients (any may be NULL):
  * beta_x ((nx+1)*ny), beta_y (nx*(ny+1)), jc (nx*ny) = sigma*kappa. */
 int p2d_get_coefficients(p2d_ctx* ctx, double* beta_x, double* beta_y, double* jc, int mem);
@@ -123,7 +146,8 @@ int p2d_get_coefficients(p2d_ctx* ctx, double* beta_x, double* beta_y, double* j
 /* Adjoint of the last projection (no reference counterpart).  Given g = dL/du_proj,
  * dL/dv_proj on interior faces, returns dL/du_in, dL/dv_in and dL/dbeta_x, dL/dbeta_y
  * (wall entries 0).  One extra solve with the same symmetric operator; only the forward
- * pressure is kept from the forward pass.  Outputs may be NULL. */
+ * pressure is kept from the forward pass.  Outputs may be NULL.  P2D_BAD_ARG when the
+ * context's last solve was not a p2d_project. */
 int p2d_adjoint(p2d_ctx* ctx, const double* gu, const double* gv, double* du, double* dv, double* dbeta_x,
                 double* dbeta_y, p2d_stats* stats, int mem);
 

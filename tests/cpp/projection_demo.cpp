@@ -4,9 +4,10 @@
 //   projection_demo <dir> <n> <precision 0|1> <rho_minus> <rho_plus> <sigma>
 //
 // reads <dir>/phi.bin (n*n), u.bin ((n+1)*n), v.bin (n*(n+1)) as raw FP64,
-// runs DeterminePressure(dt = 1) (checking that it leaves u, v alone), then
-// Solve(dt = 1) (checking the same pressure), and writes <dir>/p_out.bin,
-// u_out.bin, v_out.bin and one line of stats to stdout.
+// runs DetermineDivergence(dt = 1) and DeterminePressure() (checking that they
+// leave u, v alone), then Solve(dt = 1) (checking the same pressure), and writes
+// <dir>/p_out.bin (the reference's gauge, p(0,0) = 0), u_out.bin, v_out.bin and
+// one line of stats to stdout.
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -52,10 +53,14 @@ int main(int argc, char** argv) {
     projection.air_density = std::atof(argv[4]);
     projection.water_density = std::atof(argv[5]);
     projection.surface_tension = std::atof(argv[6]);
-    projection.tolerance = 1e-12;
+    projection.tolerance = 0;  // the reference's absolute rule off: the fixture's relative one instead
+    projection.relative_tolerance = 1e-12;
+    projection.div_tolerance = 1e-10;
+    projection.max_iteration = 1000;
     // DeterminePressure first: it must leave u, v untouched and give Solve's pressure
     const std::vector<double> u0 = u, v0 = v;
-    projection.DeterminePressure(1.0, phi.data(), u.data(), v.data());
+    projection.DetermineDivergence(1.0, u.data(), v.data());
+    projection.DeterminePressure(phi.data());
     const std::vector<double> p_only = projection.pressure_field;
     if (u != u0 || v != v0) {
       std::fprintf(stderr, "DeterminePressure changed the velocity\n");
