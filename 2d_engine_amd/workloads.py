@@ -20,7 +20,15 @@ Workloads (density ratio 1000:1 everywhere, rho = 1 on the phi <= 0 side):
   rt          Rayleigh-Taylor phi = y - 0.5 - 0.01 cos(8 pi x), sigma=0,
               u,v ~ U[-1,1] seed 2
   drops512    512 circles c~U[.05,.95]^2, r~U[.005,.03] (seed 4), sigma=0.0728,
-              u,v ~ FP32 U[-1,1] (values representable in FP32)
+              u,v ~ FP32 U[-1,1] (values representable in FP32); at 4096^2 its dt is
+              the CFL step of that state (DROPS512_CFL_DT), elsewhere 1
+
+The reference projects only at the step its CFL rule allows (CFLOneTimeStep,
+EULERIAN_FLUID_SOLVER_2D.h:3517-3637, before every step: SIMULATION_WORLD.h:1194-1207).
+The other workloads use dt = 1: for them dt only scales the pressure (sigma = 0 or
+small jumps).  For drops512 at dt = 1 the surface tension of r = 0.005 drops drives
+|u_proj| to ~1.5e3, where the rounding of u alone leaves max|D u| ~ 1.5e-9; at its CFL
+step |u_proj| stays O(1).
 """
 import math
 
@@ -181,11 +189,21 @@ def rt(n=2048):
     return Workload("rt", n, phi, 1.0, 1000.0, 0.0, u, v)
 
 
+# dt of drops512 at 4096^2: CFLOneTimeStep of the workload's own phi, u, v at CFL number 0.5,
+# rho 1 / 1000, sigma 0.0728, gravity 9.8 and no viscosity.  Its parts: advection
+# c = (max|u| + max|v|) / h = 8192.0, gravity g = sqrt(9.8 / h) = 200.4 and surface tension
+# s = sqrt(sigma max|kappa| / (min rho h^2)) = 70730 with max|kappa| at the clamp 1/h (the
+# kinks of the union of circles); dt = 0.5 / (0.5 (c + sqrt(c^2 + 4 g^2 + 4 s^2))).  The
+# oracle's restatement (CPU test) and p2d_cfl_dt (GPU test) both reproduce it.
+DROPS512_CFL_DT = 6.671550069261913e-06
+
+
 def drops512(n=4096):
     c, r = random_circles(512, 4, 0.05, 0.95, 0.005, 0.03)
     phi = levelset_circles(n, c, r)
     u, v = uniform_velocity(n, 4, fp32=True)
-    return Workload("drops512", n, phi, 1.0, 1000.0, 0.0728, u, v)
+    dt = DROPS512_CFL_DT if n == 4096 else 1.0
+    return Workload("drops512", n, phi, 1.0, 1000.0, 0.0728, u, v, dt=dt, extra={"cfl_number": 0.5})
 
 
 def target_velocity(n, seed=3):

@@ -35,3 +35,14 @@ def test_band_limited_field_statistics(W):
     u, v = W.band_limited_velocity(128, 1)
     core = u[:, 1:-1]
     assert 0.8 < np.sqrt(np.mean(core ** 2)) < 1.2
+
+
+def test_drops512_dt_is_its_cfl_step(W, O):
+    """drops512 at 4096^2 projects at the step of the reference's CFL rule for its own state
+    (CFLOneTimeStep, EULERIAN_FLUID_SOLVER_2D.h:3517-3637) at CFL 0.5, as the oracle restates it."""
+    w = W.drops512(4096)
+    dt = O.cfl_dt(w.phi, w.u, w.v, w.h, cfl_number=0.5, rho_minus=w.rho_minus, rho_plus=w.rho_plus,
+                  sigma=w.sigma)
+    assert w.dt == W.DROPS512_CFL_DT
+    assert abs(dt - w.dt) <= 1e-14 * dt
+    assert W.drops512(64).dt == 1.0
