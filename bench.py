@@ -69,9 +69,20 @@ REFINE_B_PER_CELL = 64
 SETUP_B_PER_CELL = 32 + 44 + 48 + 72 + 16
 
 
-def projection_bytes(N, stats):
-    """Algorithmic bytes of one mixed projection: a full V-cycle per iteration, a
-    pass-ending finest pre-smoothing and a refinement per pass, and the setup."""
+# The FP64 path moves every Krylov vector, coefficient and weight in FP64: twice the
+# V-cycle and head bytes; its refinement reads b p_hi p_lo x beta_x beta_y and writes
+# p_hi p_lo r, all FP64 (72).
+FP64_VCYCLE_B_PER_CELL = 2 * VCYCLE_B_PER_CELL
+FP64_HEAD_B_PER_CELL = 2 * HEAD_B_PER_CELL
+FP64_REFINE_B_PER_CELL = 72
+
+
+def projection_bytes(N, stats, fp64=False):
+    """Algorithmic bytes of one projection: a full V-cycle per iteration, a pass-ending
+    finest pre-smoothing and a refinement per pass, and the setup."""
+    if fp64:
+        return N * (stats["iterations"] * FP64_VCYCLE_B_PER_CELL
+                    + stats["outer"] * (FP64_HEAD_B_PER_CELL + FP64_REFINE_B_PER_CELL) + SETUP_B_PER_CELL)
     return N * (stats["iterations"] * VCYCLE_B_PER_CELL + stats["outer"] * (HEAD_B_PER_CELL + REFINE_B_PER_CELL)
                 + SETUP_B_PER_CELL)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_mg_pre_rt{n}_mixed_pmc.md")  # per workload size
@@ -274,6 +285,82 @@ def time_semi_implicit(P2D, torch, n, dev, reps=3):
             "setup": f"vertical pipe {n}x{n}, oil-water semi-implicit, dimensionless, reference boundary coupling"}
 
 
+def _timed(torch, stream, fn, reps, warmup=1):
+    """Median ms of fn() over reps runs after warmup runs, HIP events on the stream."""
+    times, out = [], None
+    for k in range(warmup + reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        out = fn()
+        e1.record(stream)
+        e1.synchronize()
+        if k >= warmup:
+            times.append(e0.elapsed_time(e1))
+    return statistics.median(times), out
+
+
+def time_configs(P2D, torch, W, dev, steps=5):
+    """The other configurations, each on its own workload, device-resident, timed like the
+    headline: C1 drops64 1024^2 FP64 MG-PCG; C3 its forward + adjoint; C4 drops512 4096^2
+    mixed forward at its CFL step then the adjoint (only the forward pressure kept), with the
+    device memory the run holds (hipMemGetInfo before the context and after each phase)."""
+    stream = torch.cuda.current_stream(dev)
+    out = {}
+
+    def run(w, precision, adjoint):
+        T = lambda a: torch.from_numpy(a).to(dev)
+        torch.cuda.synchronize()
+        free = [torch.cuda.mem_get_info(dev)[0]]
+        ctx = P2D.Context(w.n, w.n, w.h, precision=precision, rtol=1e-10, div_tol=1e-10, max_iterations=2000)
+        ctx.set_stream(stream.cuda_stream)
+        free.append(torch.cuda.mem_get_info(dev)[0])
+        phi, u0, v0 = T(w.phi), T(w.u), T(w.v)
+        u, v = u0.clone(), v0.clone()
+        ctx.set_fluid(phi, w.rho_minus, w.rho_plus, w.sigma)
+
+        def fwd():
+            u.copy_(u0)
+            v.copy_(v0)
+            return ctx.project(u, v, w.dt)[2]
+        ms, st = _timed(torch, stream, fwd, steps, warmup=2)
+        free.append(torch.cuda.mem_get_info(dev)[0])
+        N = w.n * w.n
+        pb = projection_bytes(N, st, fp64=precision == P2D.FP64)
+        r = {"ms_median": round(ms, 4), "projections_per_s": round(1000.0 / ms, 2),
+             "iterations": st["iterations"], "outer": st["outer"], "max_div": st["max_div"], "dt": w.dt,
+             "algorithmic_GBs": round(pb / (ms * 1e-3) / 1e9, 1)}
+        if adjoint:
+            ut, vt = W.target_velocity(w.n)
+            fwd()
+            gu = (u - T(ut)).contiguous()
+            gv = (v - T(vt)).contiguous()
+            ams, sta = _timed(torch, stream, lambda: ctx.adjoint(gu, gv)[4], max(3, steps // 2))
+            free.append(torch.cuda.mem_get_info(dev)[0])
+            r["adjoint"] = {"ms_median": round(ams, 4), "iterations": sta["iterations"], "status": sta["status"],
+                            "forward_plus_adjoint_over_forward": round((ms + ams) / ms, 3)}
+        r["device_memory"] = {"context_bytes": free[0] - free[1], "peak_bytes": free[0] - min(free),
+                              "peak_bytes_per_cell": round((free[0] - min(free)) / N, 1),
+                              "how": "hipMemGetInfo (torch.cuda.mem_get_info) before the context, after it, "
+                                     "after the forward runs and after the adjoint runs; peak = the largest drop "
+                                     "in free memory, inputs and gradients included"}
+        del ctx
+        return r
+
+    w1 = W.drops64(1024)
+    out["c1_drops64_1024_fp64"] = run(w1, P2D.FP64, True)
+    out["c1_drops64_1024_fp64"]["solver"] = "FP64 MG-PCG (9-level V-cycle, 3+3 red-black sweeps)"
+    out["c1_drops64_1024_mixed"] = run(w1, P2D.MIXED, False)
+    w4 = W.drops512(4096)
+    out["c4_drops512_4096_mixed"] = run(w4, P2D.MIXED, True)
+    out["c4_drops512_4096_fp64"] = run(w4, P2D.FP64, False)
+    out["c4_drops512_4096_mixed"]["speedup_vs_fp64"] = round(
+        out["c4_drops512_4096_fp64"]["ms_median"] / out["c4_drops512_4096_mixed"]["ms_median"], 3)
+    out["note"] = ("C3 (adjoint at 1024^2) is c1_drops64_1024_fp64.adjoint; C4's dt is the CFL step of its "
+                   "state (workloads.DROPS512_CFL_DT)")
+    return out
+
+
 def read_traffic(n):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary of
     this workload size, or None when there is none."""
@@ -386,6 +473,7 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (1024^2, 4096^2)")
     ap.add_argument("--pmc-calibration", action="store_true",
                     help="end with three 256 MiB device copies (reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -407,8 +495,11 @@ def main():
     fp64 = None
     if not args.no_fp64:
         ctx_d, t_d, st_d, _ = time_path(P2D, torch, w, P2D.FP64, args.steps, args.warmup, dev)
-        fp64 = {"projections_per_s": 1000.0 / statistics.median(t_d), "ms_median": statistics.median(t_d),
-                "iterations": st_d["iterations"], "outer": st_d["outer"], "max_div": st_d["max_div"]}
+        ms_d = statistics.median(t_d)
+        pb_d = projection_bytes(N, st_d, fp64=True)
+        fp64 = {"projections_per_s": 1000.0 / ms_d, "ms_median": ms_d,
+                "iterations": st_d["iterations"], "outer": st_d["outer"], "max_div": st_d["max_div"],
+                "algorithmic_bytes": pb_d, "achieved_GBs": round(pb_d / (ms_d * 1e-3) / 1e9, 1)}
     ms = statistics.median(t_m)
     value = 1000.0 / ms
     adj_ms, st_a = time_adjoint(ctx_m, torch, w, uv_m, max(3, args.steps // 4), dev)
@@ -427,6 +518,7 @@ def main():
     levelset = time_reinit(P2D, torch, ctx_m, w, dev)
     step = time_step(P2D, torch, w, dev)
     semi = time_semi_implicit(P2D, torch, w.n, dev)
+    configs = time_configs(P2D, torch, W, dev) if not args.no_configs else None
     dom = kernels[DOMINANT]
     traffic = read_traffic(w.n)
     pbytes = projection_bytes(N, st_m)
@@ -458,6 +550,7 @@ def main():
         "levelset": levelset,
         "time_step": step,
         "viscosity_semi_implicit": semi,
+        "configs": configs,
     }
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, serial)")
