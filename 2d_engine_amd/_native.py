@@ -44,6 +44,7 @@ EXPORTS = [
     "p2d_advect_velocity", "p2d_apply_viscosity", "p2d_step_params_default", "p2d_advance",
     "p2d_cfl_dt", "p2d_visc_create", "p2d_visc_destroy", "p2d_visc_set_stream", "p2d_visc_solve",
     "p2d_oil_water_params_default", "p2d_visc_oil_water", "p2d_divergence", "p2d_solve_pressure",
+    "p2d_cfl_dt_noalloc",
 ]
 
 
@@ -127,6 +128,7 @@ def _load():
         "p2d_step_params_default": (None, [ctypes.POINTER(P2DStepParams)]),
         "p2d_advance": (i, [vp, vp, vp, vp, d, ctypes.POINTER(P2DStepParams), stp, i]),
         "p2d_cfl_dt": (i, [vp, vp, vp, vp, ctypes.POINTER(P2DStepParams), d, ctypes.POINTER(d), i]),
+        "p2d_cfl_dt_noalloc": (i, [i, i, d, vp, vp, vp, ctypes.POINTER(P2DStepParams), d, vp, vp]),
         "p2d_visc_create": (i, [i, i, d, d, ctypes.POINTER(vp)]),
         "p2d_visc_destroy": (i, [vp]),
         "p2d_visc_set_stream": (i, [vp, vp]),

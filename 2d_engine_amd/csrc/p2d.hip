@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "p2d.h"
@@ -707,6 +708,39 @@ double abs2_target(const p2d_ctx* c) {
 
 }  // namespace
 
+// --------------------------------------------------------------------------- static scratch
+// p2d_cfl_dt_noalloc's device memory: the block partials of k_cfl_max (3 per block, at most
+// P2D_NOALLOC_MAX_BLOCKS blocks) and, after them, the Scal its kernels reduce into.  One copy
+// per device, for the life of the process (1.7 MB).  Host code never passes `scratch` itself
+// to a kernel: it is a host-side stand-in; each device's address comes from
+// hipGetSymbolAddress (noalloc_scratch) and is kept in g_scratch.
+constexpr size_t NOALLOC_SCAL_DOUBLES = (sizeof(Scal) + 255) / 256 * 32;
+static_assert(sizeof(Scal) <= NOALLOC_SCAL_DOUBLES * sizeof(double), "Scal fits its slot");
+constexpr size_t NOALLOC_DOUBLES = 3 * (size_t)P2D_NOALLOC_MAX_BLOCKS + NOALLOC_SCAL_DOUBLES;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's copy of `scratch` (its address looked up once per device).
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return P2D_NO_DEVICE;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return P2D_HIP_ERROR;
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return P2D_OK;
+}
+}  // namespace
+
 // =========================================================================== C ABI
 extern "C" {
 
@@ -1233,14 +1267,8 @@ int p2d_cfl_dt(p2d_ctx* c, const double* phi, const double* u, const double* v, 
   CHK(launched_ok());
   CHK(sync_scal(c));
   if (c->hs->err) return P2D_NAN;
-  const double umax = c->hs->red[0], vmax = c->hs->red[1], kmax = c->hs->red[2];
-  const double ih = 1.0 / c->h, ih2 = 1.0 / (c->h * c->h);
-  const double cf = umax * ih + vmax * ih;
-  const double gf = std::sqrt(9.8 * ih);
-  const double vf = std::max(prm->mu_plus / prm->rho_plus, prm->mu_minus / prm->rho_minus) * (2 * ih2 + 2 * ih2);
-  const double sf = std::sqrt(prm->sigma * kmax / (std::min(prm->rho_plus, prm->rho_minus) * (c->h * c->h)));
-  const double cfl = 0.5 * (cf + vf + std::sqrt((cf + vf) * (cf + vf) + 4 * (gf * gf) + 4 * (sf * sf)));
-  *dt = cfl == 0 ? 0.0 : cfl_number / cfl;
+  *dt = cfl_step(c->hs->red[0], c->hs->red[1], c->hs->red[2], c->h, prm->rho_minus, prm->rho_plus, prm->mu_minus,
+                 prm->mu_plus, prm->sigma, cfl_number);
   return P2D_OK;
 }
 
@@ -1258,6 +1286,33 @@ void p2d_step_params_default(p2d_step_params* p) {
   p->weno_epsilon = 1e-6;
   p->reinit_iterations = 25;
   p->reinit_scaling = 5.0;
+}
+
+int p2d_cfl_dt_noalloc(int nx, int ny, double h, const double* phi, const double* u, const double* v,
+                       const p2d_step_params* prm, double cfl_number, double* dt, void* hip_stream) {
+  if (nx < 2 || ny < 2 || !(h > 0) || !u || !v || !prm || !dt || (!phi && prm->sigma != 0.0) ||
+      !(prm->rho_minus > 0) || !(prm->rho_plus > 0))
+    return P2D_BAD_ARG;
+  const dim3 g = bgrid(nx + 1, ny + 1);
+  if (count(g) > (size_t)P2D_NOALLOC_MAX_BLOCKS) {
+    fprintf(stderr,
+            "p2d_cfl_dt_noalloc: a %d x %d grid needs %zu reduction blocks; the static scratch holds %d "
+            "(P2D_NOALLOC_MAX_BLOCKS)\n",
+            nx, ny, count(g), P2D_NOALLOC_MAX_BLOCKS);
+    return P2D_BAD_ARG;
+  }
+  double* base = nullptr;
+  CHK(noalloc_scratch(&base));
+  double* partial = base;
+  Scal* s = reinterpret_cast<Scal*>(base + 3 * (size_t)P2D_NOALLOC_MAX_BLOCKS);
+  hipStream_t st = static_cast<hipStream_t>(hip_stream);
+  const double* dp = prm->sigma != 0.0 ? phi : nullptr;
+  k_clear_err<<<1, 1, 0, st>>>(s);
+  k_cfl_max<<<g, BLOCK, 0, st>>>(nx, ny, h, dp, u, v, partial, s);
+  k_fin_store<3><<<1, FINISH_THREADS, 0, st>>>(partial, (int)count(g), 7u, 0, s);
+  k_cfl_final<<<1, 1, 0, st>>>(s, h, prm->rho_minus, prm->rho_plus, prm->mu_minus, prm->mu_plus, prm->sigma,
+                               cfl_number, dt);
+  return launched_ok();
 }
 
 // One time step of the air-water solver with third-order TVD Runge-Kutta,

@@ -403,6 +403,37 @@ __global__ void __launch_bounds__(BLOCK) k_cfl_max(int nx, int ny, double h, con
   block_partial<3>(acc, 7u, partial);
 }
 
+// EULERIAN_FLUID_SOLVER_2D::CFLOneTimeStep's formula (EULERIAN_FLUID_SOLVER_2D.h:3517-3637)
+// from the maxima: advection c = max|u|/h + max|v|/h, gravity g = sqrt(9.8/h), viscosity
+// w = max(mu+/rho+, mu-/rho-)(2/h^2 + 2/h^2), surface tension s = sqrt(sigma max|kappa| /
+// (min rho h^2)); dt = cfl / (0.5 (c + w + sqrt((c + w)^2 + 4 g^2 + 4 s^2))), 0 when that
+// denominator is 0.  One expression order on host and device (no contraction).
+__host__ __device__ inline double cfl_step(double umax, double vmax, double kmax, double h, double rho_m,
+                                           double rho_p, double mu_m, double mu_p, double sigma, double cfl_number) {
+#pragma clang fp contract(off)
+  const double ih = 1.0 / h, ih2 = 1.0 / (h * h);
+  const double cf = umax * ih + vmax * ih;
+  const double gf = sqrt(9.8 * ih);
+  const double vf = fmax(mu_p / rho_p, mu_m / rho_m) * (2 * ih2 + 2 * ih2);
+  const double sf = sqrt(sigma * kmax / (fmin(rho_p, rho_m) * (h * h)));
+  const double cfl = 0.5 * (cf + vf + sqrt((cf + vf) * (cf + vf) + 4 * (gf * gf) + 4 * (sf * sf)));
+  return cfl == 0 ? 0.0 : cfl_number / cfl;
+}
+
+// The CFL step from the maxima k_cfl_max and k_fin_store<3> left in s->red[0..2], into *dt
+// (device memory); NaN when the curvature was undefined (s->err).
+__global__ void k_cfl_final(const Scal* s, double h, double rho_m, double rho_p, double mu_m, double mu_p,
+                            double sigma, double cfl_number, double* dt) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  *dt = s->err ? __longlong_as_double(0x7ff8000000000000LL)
+               : cfl_step(s->red[0], s->red[1], s->red[2], h, rho_m, rho_p, mu_m, mu_p, sigma, cfl_number);
+}
+
+// Clears the error flag of s (a fill kernel, so that graph capture records no memset node).
+__global__ void k_clear_err(Scal* s) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->err = 0;
+}
+
 // Sum of p = hi + lo into s->red[3] (for the mean); 1-D grid-stride launch.
 __global__ void __launch_bounds__(BLOCK) k_psum(size_t N, const double* __restrict__ ph, const double* __restrict__ pl,
                                                 double* partial, Scal* s) {

@@ -213,6 +213,20 @@ void p2d_step_params_default(p2d_step_params* params);
 int p2d_cfl_dt(p2d_ctx* ctx, const double* phi, const double* u, const double* v, const p2d_step_params* params,
                double cfl_number, double* dt, int mem);
 
+/* p2d_cfl_dt without allocation (no counterpart in the reference): for callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  No context: the grid is nx x ny cells of size h; phi (nx*ny, may be NULL
+ * when params->sigma is 0), u ((nx+1)*ny), v (nx*(ny+1)) and dt (one double) are DEVICE
+ * pointers.  Enqueues four kernels on hip_stream (NULL: the null stream) and returns without
+ * waiting; *dt then holds p2d_cfl_dt's value, or NaN where p2d_cfl_dt returns P2D_NAN.  Its
+ * reductions use a static device array of the library (1.7 MB per device, kept until the process
+ * ends), so it serves grids whose (nx+1) x (ny+1) band needs at most P2D_NOALLOC_MAX_BLOCKS
+ * blocks of 256 x 4 (any grid up to 8192 x 8192); larger ones get P2D_BAD_ARG.  Calls that overlap
+ * on one device race on that array: order them on one stream. */
+#define P2D_NOALLOC_MAX_BLOCKS 69632
+int p2d_cfl_dt_noalloc(int nx, int ny, double h, const double* phi, const double* u, const double* v,
+                       const p2d_step_params* params, double cfl_number, double* dt, void* hip_stream);
+
 /* One time step of the air-water solver with third-order TVD Runge-Kutta, in place on phi
  * (nx*ny), u ((nx+1)*ny), v (nx*(ny+1)): EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread
  * with order_for_time_advancing = 3 (EULERIAN_FLUID_SOLVER_2D.h:1365-1797, no vortex sheet).
