@@ -88,6 +88,10 @@ struct Scal {
   int pad;
 };
 
+// Index of the calling thread's wavefront in its block, as a wave-uniform (scalar) value:
+// tests on it become scalar branches instead of per-lane masks.
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)); }
+
 // --------------------------------------------------------------------------- reductions
 // Cross-lane moves of a 64-bit value (two 32-bit DPP moves); lanes without a
 // source receive 0.
@@ -123,7 +127,7 @@ __device__ __forceinline__ double wave_max(double v) { return wave_reduce<true>(
 template <int K, int NT>
 __device__ __forceinline__ void block_reduce(double (&v)[K], unsigned maxmask) {
   __shared__ double sh[K][NT / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = wave_id();
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = ((maxmask >> k) & 1) ? wave_max(v[k]) : wave_sum(v[k]);

@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
   double(*sh)[FTX + 4] = reinterpret_cast<double(*)[FTX + 4]>(&sh2[0][0]);
   double(*sl)[FTX + 4] = reinterpret_cast<double(*)[FTX + 4]>(&sl2[0][0]);
   const double bm = s->bmean;
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = threadIdx.x & 63, w = wave_id();
   int bx, by;
   xcd_tile(bx, by);
   const int i0 = bx * FTX, j0 = by * FTY;

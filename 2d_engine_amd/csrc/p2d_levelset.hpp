@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(BLOCK) k_reinit_step(int nx, int ny, double h,
   stage_weno_tile(g, phi, nx, ny, i0, j0);
   __syncthreads();
   const double ih = 1.0 / h;
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = threadIdx.x & 63, w = wave_id();
 #pragma unroll
   for (int m = 0; m < RI_H / (BLOCK / 64); ++m) {
     const int r = w + (BLOCK / 64) * m, i = i0 + c, j = j0 + r;
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(BLOCK) k_advect_ls(int nx, int ny, double h, d
   stage_weno_tile(g, phi, nx, ny, i0, j0);
   __syncthreads();
   const double ih = 1.0 / h;
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = threadIdx.x & 63, w = wave_id();
 #pragma unroll
   for (int m = 0; m < RI_H / (BLOCK / 64); ++m) {
     const int r = w + (BLOCK / 64) * m, i = i0 + c, j = j0 + r;
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(BLOCK) k_advect_vel(int nx, int ny, double h, 
   const int i0 = bx * RI_W, j0 = by * RI_H;
   stage_weno_tile(g, f, fnx, fny, i0, j0);
   __syncthreads();
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = threadIdx.x & 63, w = wave_id();
 #pragma unroll
   for (int m = 0; m < RI_H / (BLOCK / 64); ++m) {
     const int r = w + (BLOCK / 64) * m, i = i0 + c, j = j0 + r;

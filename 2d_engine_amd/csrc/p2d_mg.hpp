@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_setup_tiled(int nx, int ny, const 
     }
     __syncthreads();
   }
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = threadIdx.x & 63, w = wave_id();
   constexpr int NW = BLOCK / 64;
 #pragma unroll
   for (int m = 0; m < STY / NW; ++m) {
@@ -546,12 +546,14 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
     const T dA = m > 0 ? S.xA[m > 0 ? m - 1 : 0] : dn.a, dB = m > 0 ? S.xB[m > 0 ? m - 1 : 0] : dn.b;
     const T uA = m + 1 < KR ? S.xA[m + 1 < KR ? m + 1 : m] : up.a;
     const T uB = m + 1 < KR ? S.xB[m + 1 < KR ? m + 1 : m] : up.b;
+    // cells outside the grid or the region have 1/diag = 0 and right-hand side 0, and
+    // their neighbour sums are finite: they stay 0 without a per-lane test
     if (doA) {
       const T o = zero ? (T)0 : S.wA[m] * xl + S.wB[m] * S.xB[m] + S.sA[m] * dA + S.nA(m) * uA;
-      if (S.v[m]) S.xA[m] = (S.rA[m] + o) * S.iA[m];
+      S.xA[m] = (S.rA[m] + o) * S.iA[m];
     } else {
       const T o = zero ? (T)0 : S.wB[m] * S.xA[m] + S.eB[m] * xr + S.sB[m] * dB + S.nB(m) * uB;
-      if (S.v[m]) S.xB[m] = (S.rB[m] + o) * S.iB[m];
+      S.xB[m] = (S.rB[m] + o) * S.iB[m];
     }
   }
   Pair<T>* wr = bnd + (k & 1) * BND;
@@ -572,40 +574,23 @@ template <class T, bool FUSE, bool BUSY = false>
 __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE || BUSY>())))
 k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB, T* xsol, const T* pA, const T* pB,
          const T* __restrict__ qv, double* partial, Scal* s) {
-  if (s->done) return;
   __shared__ Pair<T> lds[TILE_LDS_PAIRS];
   const int H = 2 * nu + 1, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
   xcd_tile(bx, by);
   const int tx0 = bx * TX, ty0 = by * TY;
   const int gx0 = tx0 - HX, gy0 = ty0 - H;
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = threadIdx.x & 63, w = wave_id();
   const int giA = gx0 + 2 * c;
   const int nx = F.nx;
   const int hc = HX / 2, NCX = TX / 2;
   const bool pair_int = c >= hc && c < hc + NCX;       // both cells in the tile interior
   const bool pair_fp = c >= hc - 1 && c <= hc + NCX;   // restriction footprint
-  const T* rin = r_lvl;
-  T* rout = nullptr;
-  const T* pdir = nullptr;
-  double alpha = 0.0, shift = 0.0;
-  bool pend = false;
-  if (FUSE) {
-    const int rp = s->rpar;
-    rin = rp ? rB : rA;
-    rout = rp ? rA : rB;
-    pend = s->pending != 0;
-    alpha = s->alpha;
-    shift = s->rshift;
-    pdir = ((s->iter + 1) & 1) ? pB : pA;
-  }
-  const T ta = (T)alpha, tshift = (T)shift;
+  // The coefficients (and, below the finest level, the right-hand side and the weights)
+  // do not depend on the solver state: their loads are issued before `done` is read, so
+  // that their round trip overlaps that of the state.
   Strip<T> S;
   strip_stage<T>(S, F, giA, gy0, w, RY);
-  strip_diag<T>(S);
-  // interpolation weights of the restriction footprint: loaded with the rest on
-  // coarse levels, late in the sweeps where two blocks share a CU (the finest
-  // level, and BUSY coarse levels with more tiles than CUs): registers are short there
   T wxA[KR], wxB[KR], wyA[KR], wyB[KR];
   auto load_weights = [&]() {
 #pragma unroll
@@ -623,9 +608,38 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
       }
     }
   };
+  // interpolation weights of the restriction footprint: loaded with the rest on
+  // coarse levels, late in the sweeps where two blocks share a CU (the finest
+  // level, and BUSY coarse levels with more tiles than CUs): registers are short there
   if (!(FUSE || BUSY)) load_weights();
+  if (!FUSE) {
 #pragma unroll
-  for (int m = 0; m < KR; ++m) {
+    for (int m = 0; m < KR; ++m) {
+      Pair<T> r2{0, 0};
+      if (S.v[m]) r2 = ld_at<Pair<T>>(r_lvl, cell_u(giA, gy0 + KR * w + m, nx));
+      S.rA[m] = r2.a;
+      S.rB[m] = r2.b;
+    }
+  }
+  if (s->done) return;
+  const T* rin = r_lvl;
+  T* rout = nullptr;
+  const T* pdir = nullptr;
+  double alpha = 0.0, shift = 0.0;
+  bool pend = false;
+  if (FUSE) {
+    const int rp = s->rpar;
+    rin = rp ? rB : rA;
+    rout = rp ? rA : rB;
+    pend = s->pending != 0;
+    alpha = s->alpha;
+    shift = s->rshift;
+    pdir = ((s->iter + 1) & 1) ? pB : pA;
+  }
+  const T ta = (T)alpha, tshift = (T)shift;
+  strip_diag<T>(S);
+#pragma unroll
+  for (int m = 0; m < KR && FUSE; ++m) {
     const int r = KR * w + m;
     T ra = 0, rb = 0;
     if (S.v[m]) {
@@ -633,7 +647,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
       const Pair<T> r2 = ld_at<Pair<T>>(rin, k);
       ra = r2.a;
       rb = r2.b;
-      if (FUSE) {
+      {
         // the inner recursion runs in T: r -= a q, x += a p (a, shift rounded to T)
         if (pend) {
           const Pair<T> q2 = ld_at<Pair<T>>(qv, k);
@@ -772,7 +786,6 @@ template <class T, bool LEVEL0, bool BUSY = false>
 __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, LEVEL0 || BUSY>())))
 k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, const T* x_in, T* x_out, int nu,
           double* partial, Scal* s) {
-  if (s->done) return;
   __shared__ Pair<T> lds[TILE_LDS_PAIRS];
   __shared__ T xcs[PCH][PCW];
   const int H = 2 * nu, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
@@ -780,16 +793,21 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
   xcd_tile(bx, by);
   const int tx0 = bx * TX, ty0 = by * TY;
   const int gx0 = tx0 - HX, gy0 = ty0 - H;
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = threadIdx.x & 63, w = wave_id();
   const int giA = gx0 + 2 * c;
   const int nx = F.nx, cnx = Cc.nx, cny = Cc.ny;
   const int hc = HX / 2, NCX = TX / 2;
   const bool pair_int = c >= hc && c < hc + NCX;
   const int cxa = gx0 / 2 - 1, cya = (gy0 >> 1) - 1;  // first staged coarse column / row (gx0 is even)
-  const T* rin = LEVEL0 ? (s->rpar ? rB : rA) : r_lvl;
-  for (int t = threadIdx.x; t < PCH * PCW; t += TBLOCK) {
+  // Every load but the finest level's residual is independent of the solver state and
+  // is issued before `done` is read (the coarse values go to LDS after the test).
+  constexpr int XCS_PER_THREAD = (PCH * PCW + TBLOCK - 1) / TBLOCK;
+  T xcv[XCS_PER_THREAD];
+#pragma unroll
+  for (int q = 0; q < XCS_PER_THREAD; ++q) {
+    const int t = (int)threadIdx.x + q * TBLOCK;
     const int I = min(max(cxa + t % PCW, 0), cnx - 1), J = min(max(cya + t / PCW, 0), cny - 1);
-    xcs[t / PCW][t % PCW] = ld_at<T>(Cc.xb, cell_u(I, J, cnx));
+    xcv[q] = t < PCH * PCW ? ld_at<T>(Cc.xb, cell_u(I, J, cnx)) : (T)0;
   }
   Strip<T> S;
   strip_stage<T>(S, F, giA, gy0, w, RY);
@@ -799,7 +817,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
     Pair<T> rr{0, 0}, x0{0, 0}, wx2{0, 0}, wy2{0, 0};
     if (S.v[m]) {
       const unsigned k = cell_u(giA, gy0 + KR * w + m, nx);
-      rr = ld_at<Pair<T>>(rin, k);
+      if (!LEVEL0) rr = ld_at<Pair<T>>(r_lvl, k);
       x0 = ld_at<Pair<T>>(x_in, k);
       wx2 = ld_at<Pair<T>>(F.wx, k);
       wy2 = ld_at<Pair<T>>(F.wy, k);
@@ -812,6 +830,22 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
     wxB[m] = wx2.b;
     wyA[m] = wy2.a;
     wyB[m] = wy2.b;
+  }
+  if (s->done) return;
+  const T* rin = LEVEL0 ? (s->rpar ? rB : rA) : r_lvl;
+  if (LEVEL0) {
+#pragma unroll
+    for (int m = 0; m < KR; ++m) {
+      Pair<T> rr{0, 0};
+      if (S.v[m]) rr = ld_at<Pair<T>>(rin, cell_u(giA, gy0 + KR * w + m, nx));
+      S.rA[m] = rr.a;
+      S.rB[m] = rr.b;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < XCS_PER_THREAD; ++q) {
+    const int t = (int)threadIdx.x + q * TBLOCK;
+    if (t < PCH * PCW) xcs[t / PCW][t % PCW] = xcv[q];
   }
   __syncthreads();  // xcs
   // prolongation: A (column 2I) takes coarse I and I-1, B takes I and I+1
@@ -1117,24 +1151,46 @@ template <class T>
 __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restrict__ bw, const T* __restrict__ bs,
                                                const T* __restrict__ z, T* pA, T* pB, T* __restrict__ q,
                                                double* partial, Scal* s) {
-  if (s->done) return;
   // row layout: [1] west ring, [2 + 2c, 3 + 2c] pair c, [DTX + 2] east ring (pairs 2-element aligned)
   __shared__ Pair<T> ps2[DTY + 2][DTX / 2 + 2];
   T(*ps)[DTX + 4] = reinterpret_cast<T(*)[DTX + 4]>(&ps2[0][0]);
-  const double beta = s->beta, zm = s->zmean;
-  const bool first = s->iter == s->seg0;
-  const T* pold = (s->iter & 1) ? pB : pA;
-  T* pnew = (s->iter & 1) ? pA : pB;
-  const int c = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = threadIdx.x & 63, w = wave_id();
   int bx, by;
   xcd_tile(bx, by);
   const int i0 = bx * DTX, j0 = by * DTY;
   const int iA = i0 + 2 * c;
   const bool col_in = iA < nx;
+  // Each wave owns RPW consecutive rows, so a row's north faces are the next row's
+  // south ones, loaded once; the east face of B is the next lane's west face of A
+  // (DPP), only the last lane loads its own.  These loads depend on no solver state
+  // and are issued before it is read.
+  constexpr int NW = BLOCK / 64, RR = (DTY + 2 + NW - 1) / NW;
+  constexpr int RPW = DTY / NW;
+  const int rb = w * RPW;
+  T sA[RPW + 1], sB[RPW + 1], wA[RPW], wB[RPW], eB[RPW], eL[RPW];
+#pragma unroll
+  for (int m = 0; m <= RPW; ++m) {
+    const int j = j0 + rb + m;
+    const Pair<T> s2 = (col_in && j < ny) ? ld_at<Pair<T>>(bs, cell_u(iA, j, nx)) : Pair<T>{(T)0, (T)0};
+    sA[m] = s2.a;
+    sB[m] = s2.b;
+  }
+#pragma unroll
+  for (int m = 0; m < RPW; ++m) {
+    const int j = j0 + rb + m;
+    const Pair<T> w2 = (col_in && j < ny) ? ld_at<Pair<T>>(bw, cell_u(iA, j, nx)) : Pair<T>{(T)0, (T)0};
+    wA[m] = w2.a;
+    wB[m] = w2.b;
+    eL[m] = (c == 63 && iA + 2 < nx && j < ny) ? ld_at<T>(bw, cell_u(iA + 2, j, nx)) : (T)0;
+  }
+  if (s->done) return;
+  const double beta = s->beta, zm = s->zmean;
+  const bool first = s->iter == s->seg0;
+  const T* pold = (s->iter & 1) ? pB : pA;
+  T* pnew = (s->iter & 1) ? pA : pB;
   auto pform = [&](T zz, T po) -> T {
     return first ? (T)((double)zz - zm) : (T)(((double)zz - zm) + beta * (double)po);
   };
-  constexpr int NW = BLOCK / 64, RR = (DTY + 2 + NW - 1) / NW;
 #pragma unroll
   for (int m = 0; m < RR; ++m) {
     const int r = w + NW * m, jj = j0 - 1 + r;
@@ -1159,33 +1215,11 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     }
     ps[r][side ? DTX + 2 : 1] = v;
   }
-  // Each wave owns RPW consecutive rows, so a row's north faces are the next row's
-  // south ones, loaded once; the east face of B is the next lane's west face of A
-  // (DPP), only the last lane loads its own.  These loads do not depend on the LDS
-  // staging and are issued before the barrier.
-  constexpr int RPW = DTY / NW;
-  const int rb = w * RPW;
-  T sA[RPW + 1], sB[RPW + 1], wA[RPW], wB[RPW], eB[RPW];
-#pragma unroll
-  for (int m = 0; m <= RPW; ++m) {
-    const int j = j0 + rb + m;
-    const Pair<T> s2 = (col_in && j < ny) ? ld_at<Pair<T>>(bs, cell_u(iA, j, nx)) : Pair<T>{(T)0, (T)0};
-    sA[m] = s2.a;
-    sB[m] = s2.b;
-  }
 #pragma unroll
   for (int m = 0; m < RPW; ++m) {
-    const int j = j0 + rb + m;
-    const Pair<T> w2 = (col_in && j < ny) ? ld_at<Pair<T>>(bw, cell_u(iA, j, nx)) : Pair<T>{(T)0, (T)0};
-    wA[m] = w2.a;
-    wB[m] = w2.b;
-  }
-#pragma unroll
-  for (int m = 0; m < RPW; ++m) {
-    const int j = j0 + rb + m;
-    T e = lane_next(wA[m]);
-    if (c == 63 && iA + 2 < nx && j < ny) e = ld_at<T>(bw, cell_u(iA + 2, j, nx));
-    eB[m] = e;
+    // the lane move runs on every lane (a lane outside the active mask would read as 0)
+    const T e = lane_next(wA[m]);
+    eB[m] = c == 63 ? eL[m] : e;
   }
   __syncthreads();
   double acc[1] = {0.0};
