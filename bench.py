@@ -308,11 +308,12 @@ def time_configs(P2D, torch, W, dev, steps=5):
     stream = torch.cuda.current_stream(dev)
     out = {}
 
-    def run(w, precision, adjoint):
+    def run(w, precision, adjoint, **cfg):
         T = lambda a: torch.from_numpy(a).to(dev)
         torch.cuda.synchronize()
         free = [torch.cuda.mem_get_info(dev)[0]]
-        ctx = P2D.Context(w.n, w.n, w.h, precision=precision, rtol=1e-10, div_tol=1e-10, max_iterations=2000)
+        ctx = P2D.Context(w.n, w.n, w.h, precision=precision, rtol=1e-10, div_tol=1e-10, max_iterations=2000,
+                          **cfg)
         ctx.set_stream(stream.cuda_stream)
         free.append(torch.cuda.mem_get_info(dev)[0])
         phi, u0, v0 = T(w.phi), T(w.u), T(w.v)
@@ -349,7 +350,9 @@ def time_configs(P2D, torch, W, dev, steps=5):
 
     w1 = W.drops64(1024)
     out["c1_drops64_1024_fp64"] = run(w1, P2D.FP64, True)
-    out["c1_drops64_1024_fp64"]["solver"] = "FP64 MG-PCG (9-level V-cycle, 3+3 red-black sweeps)"
+    out["c1_drops64_1024_fp64"]["solver"] = "FP64 MG-PCG (8-level V-cycle, 3+3 red-black sweeps)"
+    out["c1_drops64_1024_fp64_2_2_sweeps"] = run(w1, P2D.FP64, False, mg_pre_sweeps=2, mg_post_sweeps=2)
+    out["c1_drops64_1024_fp64_2_2_sweeps"]["solver"] = "FP64 MG-PCG (8-level V-cycle, 2+2 red-black sweeps)"
     out["c1_drops64_1024_mixed"] = run(w1, P2D.MIXED, False)
     w4 = W.drops512(4096)
     out["c4_drops512_4096_mixed"] = run(w4, P2D.MIXED, True)

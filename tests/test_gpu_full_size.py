@@ -63,10 +63,15 @@ def test_rt2048_linearity_and_idempotence(P2D, W):
     assert d <= 1e-9 * max(np.abs(p1u).max(), np.abs(p1v).max())
 
 
-def test_drops64_1024_fp64(P2D, W):
+@pytest.mark.parametrize("nu", [3, 2])
+def test_drops64_1024_fp64(P2D, W, nu):
+    """The 1024^2 FP64 case with the default 3+3 sweeps and with the 2+2 red-black pre- and
+    post-sweeps the configuration names (the hierarchy keeps its full depth: a four-level
+    cycle would leave a 128^2 bottom that no exact solve handles at this cost)."""
     w = W.drops64(1024)
-    print("case full size drops64 1024 fp64", flush=True)
-    ctx = P2D.Context(w.n, w.n, w.h, precision=P2D.FP64, rtol=1e-10, div_tol=1e-10)
+    print(f"case full size drops64 1024 fp64 nu={nu}", flush=True)
+    ctx = P2D.Context(w.n, w.n, w.h, precision=P2D.FP64, rtol=1e-10, div_tol=1e-10, mg_pre_sweeps=nu,
+                      mg_post_sweeps=nu)
     ctx.set_fluid(w.phi, w.rho_minus, w.rho_plus, w.sigma)
     u, v, st = ctx.project(w.u, w.v, w.dt)
     div = np.abs(_div(u, v, w.h)).max()
