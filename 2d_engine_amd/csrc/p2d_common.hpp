@@ -190,17 +190,29 @@ __device__ __forceinline__ void block_partial_lds(double (&v)[K], unsigned maxma
 }
 
 // In a FINISH_THREADS-thread block: totals of G partials into thread 0's a.
-// Maxima are of non-negative quantities (start at 0).
+// Maxima are of non-negative quantities (start at 0).  A thread loads all its
+// partials (blocks t, t + FINISH_THREADS, ...; up to FIN_U of them) before it
+// adds any, so G <= FIN_U * FINISH_THREADS partials take one memory round trip;
+// the additions keep block order.
+constexpr int FIN_U = 4;
 template <int K>
 __device__ __forceinline__ void finish_reduce(double (&a)[K], const double* __restrict__ partial, int G,
                                               unsigned maxmask) {
 #pragma unroll
   for (int k = 0; k < K; ++k) a[k] = 0.0;
-  for (int b = threadIdx.x; b < G; b += FINISH_THREADS) {
+  for (int b0 = threadIdx.x; b0 < G; b0 += FIN_U * FINISH_THREADS) {
+    double x[FIN_U][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double x = partial[(size_t)k * G + b];
-      a[k] = ((maxmask >> k) & 1) ? fmax(a[k], x) : a[k] + x;
+    for (int u = 0; u < FIN_U; ++u) {
+      const int b = b0 + u * FINISH_THREADS;
+#pragma unroll
+      for (int k = 0; k < K; ++k) x[u][k] = b < G ? partial[(size_t)k * G + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < FIN_U; ++u) {
+      if (b0 + u * FINISH_THREADS >= G) break;
+#pragma unroll
+      for (int k = 0; k < K; ++k) a[k] = ((maxmask >> k) & 1) ? fmax(a[k], x[u][k]) : a[k] + x[u][k];
     }
   }
   block_reduce<K, FINISH_THREADS>(a, maxmask);

@@ -621,20 +621,25 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
       S.rB[m] = r2.b;
     }
   }
+  // the state the update needs is read with `done` (one round trip)
+  int rp = 0, pend_i = 0, it = 0;
+  double alpha = 0.0, shift = 0.0;
+  if (FUSE) {
+    rp = s->rpar;
+    pend_i = s->pending;
+    alpha = s->alpha;
+    shift = s->rshift;
+    it = s->iter;
+  }
   if (s->done) return;
   const T* rin = r_lvl;
   T* rout = nullptr;
   const T* pdir = nullptr;
-  double alpha = 0.0, shift = 0.0;
-  bool pend = false;
+  const bool pend = FUSE && pend_i != 0;
   if (FUSE) {
-    const int rp = s->rpar;
     rin = rp ? rB : rA;
     rout = rp ? rA : rB;
-    pend = s->pending != 0;
-    alpha = s->alpha;
-    shift = s->rshift;
-    pdir = ((s->iter + 1) & 1) ? pB : pA;
+    pdir = ((it + 1) & 1) ? pB : pA;
   }
   const T ta = (T)alpha, tshift = (T)shift;
   strip_diag<T>(S);
@@ -751,25 +756,29 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
 // spent, and a pending update counts as an iteration with its convergence test
 // on the mean-free norm sum r^2 - (sum r)^2 / N (the recursive residual's
 // constant component is roundoff that no iteration reduces).
+// The state is read together with the partials (one memory round trip); a
+// finished solve leaves it unchanged.
 __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __restrict__ partial, int G,
                                                              double ncells, Scal* s) {
-  if (s->done) return;
+  const int done = s->done, pending = s->pending, rpar = s->rpar, iter = s->iter, max_iter = s->max_iter;
+  const int flag = s->flag;
+  const double rz = s->rz, rel2 = s->rel2, r0_2 = s->r0_2, abs2 = s->abs2, ainf = s->ainf;
   double a[3];
   finish_reduce<3>(a, partial, G, 2u);
-  if (threadIdx.x == 0) {
-    s->rpar = 1 - s->rpar;
-    s->rshift = 0.0;
-    if (s->pending) {
-      a[0] = fmax(a[0] - a[2] * a[2] / ncells, 0.0);
-      s->r2 = a[0];
-      s->rinf = a[1];
-      s->rz_old = s->rz;
-      s->iter += 1;
-      const bool met = a[0] <= s->rel2 * s->r0_2 || (a[0] <= s->abs2 && a[1] <= s->ainf);
-      if (!(a[0] == a[0])) s->flag = 1;
-      if (met || s->iter >= s->max_iter || s->flag) s->done = 1;
-      s->pending = 0;
-    }
+  if (threadIdx.x != 0 || done) return;
+  s->rpar = 1 - rpar;
+  s->rshift = 0.0;
+  if (pending) {
+    a[0] = fmax(a[0] - a[2] * a[2] / ncells, 0.0);
+    s->r2 = a[0];
+    s->rinf = a[1];
+    s->rz_old = rz;
+    s->iter = iter + 1;
+    const bool met = a[0] <= rel2 * r0_2 || (a[0] <= abs2 && a[1] <= ainf);
+    const bool bad = flag || !(a[0] == a[0]);
+    if (bad) s->flag = 1;
+    if (met || iter + 1 >= max_iter || bad) s->done = 1;
+    s->pending = 0;
   }
 }
 
@@ -831,8 +840,9 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
     wyA[m] = wy2.a;
     wyB[m] = wy2.b;
   }
+  const int rp = LEVEL0 ? s->rpar : 0;
   if (s->done) return;
-  const T* rin = LEVEL0 ? (s->rpar ? rB : rA) : r_lvl;
+  const T* rin = LEVEL0 ? (rp ? rB : rA) : r_lvl;
   if (LEVEL0) {
 #pragma unroll
     for (int m = 0; m < KR; ++m) {
@@ -888,16 +898,16 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
 // Finish of the finest post kernel: zmean, rz = r.(z - zmean), beta.
 __global__ void __launch_bounds__(FINISH_THREADS) k_fin_post0(const double* __restrict__ partial, int G, double N,
                                                               Scal* s) {
-  if (s->done) return;
+  const int done = s->done, first = s->iter == s->seg0;
+  const double rz_old = s->rz_old;
   double a[3];
   finish_reduce<3>(a, partial, G, 0u);
-  if (threadIdx.x == 0) {
-    const double zm = a[1] / N;
-    const double rz = a[0] - zm * a[2];
-    s->zmean = zm;
-    s->rz = rz;
-    s->beta = s->iter == s->seg0 ? 0.0 : rz / s->rz_old;
-  }
+  if (threadIdx.x != 0 || done) return;
+  const double zm = a[1] / N;
+  const double rz = a[0] - zm * a[2];
+  s->zmean = zm;
+  s->rz = rz;
+  s->beta = first ? 0.0 : rz / rz_old;
 }
 
 // --------------------------------------------------------------------------- single-workgroup coarse V-cycle
@@ -970,7 +980,7 @@ __host__ __device__ inline SmallLayout small_layout(const Level<T>* lv, int nl) 
 
 template <class T>
 __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int nu_pre, int nu_post, const Scal* s) {
-  if (s->done) return;
+  const int done = s->done;  // tested once the loads are issued: their round trips overlap
   extern __shared__ unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
   const int nl = A.nl, t = threadIdx.x;
@@ -1016,6 +1026,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
   }
   if (nb <= BOTTOM_LDS_MAX)
     for (int e = t; e < nb * nb; e += SMALL_THREADS) sm[L.binv + e] = A.binv[e];
+  if (done) return;
 #pragma unroll
   for (int sl = 0; sl < SMALL_SLOTS; ++sl) id[sl] = cd[sl] != (T)0 ? (T)1 / cd[sl] : (T)0;
   __syncthreads();
@@ -1183,11 +1194,12 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     wB[m] = w2.b;
     eL[m] = (c == 63 && iA + 2 < nx && j < ny) ? ld_at<T>(bw, cell_u(iA + 2, j, nx)) : (T)0;
   }
-  if (s->done) return;
   const double beta = s->beta, zm = s->zmean;
-  const bool first = s->iter == s->seg0;
-  const T* pold = (s->iter & 1) ? pB : pA;
-  T* pnew = (s->iter & 1) ? pA : pB;
+  const int it = s->iter, seg0 = s->seg0;
+  if (s->done) return;
+  const bool first = it == seg0;
+  const T* pold = (it & 1) ? pB : pA;
+  T* pnew = (it & 1) ? pA : pB;
   auto pform = [&](T zz, T po) -> T {
     return first ? (T)((double)zz - zm) : (T)(((double)zz - zm) + beta * (double)po);
   };
@@ -1253,19 +1265,19 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
 // Finish of the direction kernel: alpha = rz / p.q and the update pending
 // (breakdown when p.q is not positive).
 __global__ void __launch_bounds__(FINISH_THREADS) k_fin_dir(const double* __restrict__ partial, int G, Scal* s) {
-  if (s->done) return;
+  const int done = s->done;
+  const double rz = s->rz;
   double a[1];
   finish_reduce<1>(a, partial, G, 0u);
-  if (threadIdx.x == 0) {
-    s->pq = a[0];
-    if (!(a[0] > 0.0)) {
-      s->done = 1;
-      s->flag = 1;
-      s->alpha = 0.0;
-    } else {
-      s->alpha = s->rz / a[0];
-      s->pending = 1;
-    }
+  if (threadIdx.x != 0 || done) return;
+  s->pq = a[0];
+  if (!(a[0] > 0.0)) {
+    s->done = 1;
+    s->flag = 1;
+    s->alpha = 0.0;
+  } else {
+    s->alpha = rz / a[0];
+    s->pending = 1;
   }
 }
 
