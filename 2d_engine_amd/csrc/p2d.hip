@@ -297,15 +297,21 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
       d.bs = b;
     }
     T *wx = nullptr, *wy = nullptr, *xa, *xb, *r;
+    unsigned* wh = nullptr;
     if (l + 1 < L) {
-      CHK(dalloc(c, &wx, n));
-      CHK(dalloc(c, &wy, n));
+      if (sizeof(T) == 4) {
+        CHK(dalloc(c, &wh, n));
+      } else {
+        CHK(dalloc(c, &wx, n));
+        CHK(dalloc(c, &wy, n));
+      }
     }
     CHK(dalloc(c, &xa, n));
     CHK(dalloc(c, &xb, n));
     CHK(dalloc(c, &r, n));
     d.wx = wx;
     d.wy = wy;
+    d.wh = wh;
     d.xa = xa;
     d.xb = xb;
     d.r = r;
@@ -324,21 +330,22 @@ int mg_setup(p2d_ctx* c, MG<T>& mg, bool cast32) {
   CHK(join_side(c));  // the setup may rewrite what a pending inverse reads
   for (int l = 0; l < m.L; ++l) {
     const Level<double>& f = m.lv[l];
-    T* wx = const_cast<T*>(mg.lv[l].wx);  // null on the bottom level
+    T* wx = const_cast<T*>(mg.lv[l].wx);  // null on the bottom level and on FP32 levels
     T* wy = const_cast<T*>(mg.lv[l].wy);
+    unsigned* wh = const_cast<unsigned*>(mg.lv[l].wh);  // FP32 levels but the bottom
     float* f32bw = cast32 ? const_cast<float*>(c->mg32.lv[l].bw) : nullptr;
     float* f32bs = cast32 ? const_cast<float*>(c->mg32.lv[l].bs) : nullptr;
     double* cbw = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bw) : nullptr;
     double* cbs = !common && l + 1 < m.L ? const_cast<double*>(m.lv[l + 1].bs) : nullptr;
-    if (!wx && !f32bw && !cbw) continue;
+    if (!wx && !wh && !f32bw && !cbw) continue;
     // large levels: the tiled sweep (each face reciprocal formed once); small ones
     // are latency-bound, where the per-cell sweep's shorter chain wins
     if ((long)f.nx * f.ny >= SETUP_TILED_MIN_CELLS)
       k_mg_setup_tiled<T><<<dim3((unsigned)((f.nx + STX - 1) / STX), (unsigned)((f.ny + STY - 1) / STY)), BLOCK, 0,
-                            c->stream>>>(f.nx, f.ny, f.bw, f.bs, wx, wy, f32bw, f32bs, cbw, cbs);
+                            c->stream>>>(f.nx, f.ny, f.bw, f.bs, wx, wy, wh, f32bw, f32bs, cbw, cbs);
     else
       k_mg_setup<T><<<dim3(blocks_for((size_t)f.nx), (unsigned)f.ny), BLOCK, 0, c->stream>>>(
-          f.nx, f.ny, f.bw, f.bs, wx, wy, f32bw, f32bs, cbw, cbs);
+          f.nx, f.ny, f.bw, f.bs, wx, wy, wh, f32bw, f32bs, cbw, cbs);
   }
   if (!common) {
     const Level<double>& B = m.lv[m.L - 1];

@@ -52,12 +52,76 @@ struct Level {
   int nx, ny;
   const T* bw;  // west-face coefficient per cell
   const T* bs;  // south-face coefficient per cell
-  const T* wx;  // parent interpolation weight per cell (x, y); none on the bottom level
+  const T* wx;  // parent interpolation weight per cell (x, y), FP64 levels; none on the bottom level
   const T* wy;
+  const unsigned* wh;  // FP32 levels: both weights of a cell in one word (see WCell); none on the bottom
   T* xa;  // pre-smoothed correction
   T* xb;  // post-smoothed correction (the coarse correction read by the finer level)
   T* r;   // right-hand side (restricted residual)
 };
+
+// Interpolation weights as stored.  FP64 levels keep wx and wy in two FP64
+// arrays.  FP32 levels keep both in one 32-bit word per cell (wh: wx in the low
+// half, wy in the high half), each rounded to binary16 through binary32 (round
+// to nearest even): the weights lie in [0, 1], the V-cycle stays symmetric (R =
+// P^T with the same stored weights), and the finest level's two reads of them
+// per cycle take 4 bytes per cell instead of 8.
+__device__ __forceinline__ unsigned w_half_bits(double w) {
+  return (unsigned)__builtin_bit_cast(unsigned short, (_Float16)(float)w);
+}
+__device__ __forceinline__ unsigned w_pack(double wx, double wy) { return w_half_bits(wx) | (w_half_bits(wy) << 16); }
+__device__ __forceinline__ float w_lo(unsigned u) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(u & 0xffffu)); }
+__device__ __forceinline__ float w_hi(unsigned u) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(u >> 16)); }
+template <class T>
+struct WCell;
+template <>
+struct WCell<float> {
+  unsigned u = 0;
+  __device__ __forceinline__ float x() const { return w_lo(u); }
+  __device__ __forceinline__ float y() const { return w_hi(u); }
+};
+template <>
+struct WCell<double> {
+  double wx = 0, wy = 0;
+  __device__ __forceinline__ double x() const { return wx; }
+  __device__ __forceinline__ double y() const { return wy; }
+};
+// The weights of the cell pair (k, k + 1).
+template <class T>
+__device__ __forceinline__ void ld_wpair(const Level<T>& F, unsigned k, WCell<T>& a, WCell<T>& b) {
+  if constexpr (sizeof(T) == 4) {
+    const Pair<unsigned> u = ld_at<Pair<unsigned>>(F.wh, k);
+    a.u = u.a;
+    b.u = u.b;
+  } else {
+    const Pair<double> x = ld_at<Pair<double>>(F.wx, k), y = ld_at<Pair<double>>(F.wy, k);
+    a.wx = x.a;
+    a.wy = y.a;
+    b.wx = x.b;
+    b.wy = y.b;
+  }
+}
+template <class T>
+__device__ __forceinline__ WCell<T> ld_wcell(const Level<T>& F, int e) {
+  WCell<T> w;
+  if constexpr (sizeof(T) == 4) {
+    w.u = F.wh[e];
+  } else {
+    w.wx = F.wx[e];
+    w.wy = F.wy[e];
+  }
+  return w;
+}
+// Stores the weights of cell k (FP32 levels: packed into wh; FP64: wx, wy).
+template <class T>
+__device__ __forceinline__ void st_weights(T* wx, T* wy, unsigned* wh, size_t k, double a, double b) {
+  if (wh) {
+    wh[k] = w_pack(a, b);
+  } else {
+    wx[k] = (T)a;
+    wy[k] = (T)b;
+  }
+}
 
 // Operator-dependent parent weight of fine index i along a row/column whose face
 // coefficients are face(k), k = 0..nf (walls 0): the resistance ratio
@@ -92,13 +156,14 @@ __device__ __forceinline__ double coarse_face(F face, int k, int nf) {
 
 // One setup sweep over a fine level (FP64 coefficients bw, bs), every output
 // optional (NULL): the parent weights of each cell, computed in FP64 and
-// stored as T; the FP32 copy of the coefficients; and, from the thread of the
+// stored as T (wx, wy) or packed (wh, FP32 levels); the FP32 copy of the coefficients; and, from the thread of the
 // lower-left child of each coarse cell, the coarse FP64 coefficients.
 // Grid dim3(ceil(nx/BLOCK), ny).
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_mg_setup(int nx, int ny, const double* __restrict__ bw,
                                                     const double* __restrict__ bs, T* __restrict__ wx,
-                                                    T* __restrict__ wy, float* __restrict__ f32bw,
+                                                    T* __restrict__ wy, unsigned* __restrict__ wh,
+                                                    float* __restrict__ f32bw,
                                                     float* __restrict__ f32bs, double* __restrict__ cbw,
                                                     double* __restrict__ cbs) {
   const int i = (int)(blockIdx.x * BLOCK + threadIdx.x), j = (int)blockIdx.y;
@@ -106,14 +171,13 @@ __global__ void __launch_bounds__(BLOCK) k_mg_setup(int nx, int ny, const double
   const size_t k = ci(i, j, nx);
   auto row = [&](int jj) { return [=](int q) { return q < nx ? bw[ci(q, jj, nx)] : 0.0; }; };
   auto col = [&](int ii) { return [=](int q) { return q < ny ? bs[ci(ii, q, nx)] : 0.0; }; };
-  if (wx) {
+  if (wx || wh) {
     double a = od_weight<double>(row(j), i, nx);
     double b = od_weight<double>(col(i), j, ny);
     const int Q = (i >> 1) + ((i & 1) ? 1 : -1), Qj = (j >> 1) + ((j & 1) ? 1 : -1);
     if (Q < 0 || Q > (nx >> 1) - 1) a = 1.0;
     if (Qj < 0 || Qj > (ny >> 1) - 1) b = 1.0;
-    wx[k] = (T)a;
-    wy[k] = (T)b;
+    st_weights<T>(wx, wy, wh, k, a, b);
   }
   if (f32bw) {
     f32bw[k] = (float)bw[k];
@@ -170,14 +234,15 @@ constexpr int STX = 64, STY = 16;
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_mg_setup_tiled(int nx, int ny, const double* __restrict__ bw,
                                                     const double* __restrict__ bs, T* __restrict__ wx,
-                                                    T* __restrict__ wy, float* __restrict__ f32bw,
+                                                    T* __restrict__ wy, unsigned* __restrict__ wh,
+                                                    float* __restrict__ f32bw,
                                                     float* __restrict__ f32bs, double* __restrict__ cbw,
                                                     double* __restrict__ cbs) {
   __shared__ double rx[STY + 1][STX + 3], ry[STY + 3][STX + 1];
   int bx, by;
   xcd_tile(bx, by);
   const int i0 = bx * STX, j0 = by * STY;
-  const bool need = wx || cbw;
+  const bool need = wx || wh || cbw;
   if (need) {
     for (int t = threadIdx.x; t < (STY + 1) * (STX + 3); t += BLOCK) {
       const int a = t % (STX + 3), r = t / (STX + 3), q = i0 - 1 + a, jj = j0 + r;
@@ -200,14 +265,13 @@ __global__ void __launch_bounds__(BLOCK) k_mg_setup_tiled(int nx, int ny, const 
     const size_t k = ci(i, j, nx);
     auto row = [&](int jj) { return [=](int q) { return rx[jj - j0][q - (i0 - 1)]; }; };
     auto col = [&](int ii) { return [=](int q) { return ry[q - (j0 - 1)][ii - i0]; }; };
-    if (wx) {
+    if (wx || wh) {
       double a = od_weight_rcp<double>(row(j), i, nx);
       double b = od_weight_rcp<double>(col(i), j, ny);
       const int Q = (i >> 1) + ((i & 1) ? 1 : -1), Qj = (j >> 1) + ((j & 1) ? 1 : -1);
       if (Q < 0 || Q > (nx >> 1) - 1) a = 1.0;
       if (Qj < 0 || Qj > (ny >> 1) - 1) b = 1.0;
-      wx[k] = (T)a;
-      wy[k] = (T)b;
+      st_weights<T>(wx, wy, wh, k, a, b);
     }
     if (f32bw) {
       f32bw[k] = (float)bw[k];
@@ -591,21 +655,14 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
   // that their round trip overlaps that of the state.
   Strip<T> S;
   strip_stage<T>(S, F, giA, gy0, w, RY);
-  T wxA[KR], wxB[KR], wyA[KR], wyB[KR];
+  WCell<T> wcA[KR], wcB[KR];
   auto load_weights = [&]() {
 #pragma unroll
     for (int m = 0; m < KR; ++m) {
       const int r = KR * w + m;
-      wxA[m] = wxB[m] = wyA[m] = wyB[m] = 0;
-      if (S.v[m] && pair_fp && r >= H - 1 && r <= H + TY) {
-        const unsigned k = cell_u(giA, gy0 + r, nx);
-        const Pair<T> wx2 = ld_at<Pair<T>>(F.wx, k);
-        const Pair<T> wy2 = ld_at<Pair<T>>(F.wy, k);
-        wxA[m] = wx2.a;
-        wxB[m] = wx2.b;
-        wyA[m] = wy2.a;
-        wyB[m] = wy2.b;
-      }
+      wcA[m] = WCell<T>{};
+      wcB[m] = WCell<T>{};
+      if (S.v[m] && pair_fp && r >= H - 1 && r <= H + TY) ld_wpair<T>(F, cell_u(giA, gy0 + r, nx), wcA[m], wcB[m]);
     }
   };
   // interpolation weights of the restriction footprint: loaded with the rest on
@@ -715,10 +772,11 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
     // a*: part of res for the parent row (weight wy), b*: for the neighbouring row (1 - wy);
     // p*: part for the parent column (wx), q*: for the neighbouring column (1 - wx).
     // (the (1 - w) shares as differences: res = a + b, a = pa + qa, b = pb + qb)
-    const T aA = resA[m] * wyA[m], bA = resA[m] - aA;
-    const T aB = resB[m] * wyB[m], bB = resB[m] - aB;
-    const T paA = wxA[m] * aA, qaA = aA - paA, pbA = wxA[m] * bA, qbA = bA - pbA;
-    const T paB = wxB[m] * aB, qaB = aB - paB, pbB = wxB[m] * bB, qbB = bB - pbB;
+    const T wxA = wcA[m].x(), wyA = wcA[m].y(), wxB = wcB[m].x(), wyB = wcB[m].y();
+    const T aA = resA[m] * wyA, bA = resA[m] - aA;
+    const T aB = resB[m] * wyB, bB = resB[m] - aB;
+    const T paA = wxA * aA, qaA = aA - paA, pbA = wxA * bA, qbA = bA - pbA;
+    const T paB = wxB * aB, qaB = aB - paB, pbB = wxB * bB, qbB = bB - pbB;
     // coarse column I (this lane) gathers q(2I-1) + p(2I) + p(2I+1) + q(2I+2)
     const T sa = (paA + paB) + (lane_prev(qaB) + lane_next(qaA));
     const T sb = (pbA + pbB) + (lane_prev(qbB) + lane_next(qbA));
@@ -820,25 +878,22 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
   }
   Strip<T> S;
   strip_stage<T>(S, F, giA, gy0, w, RY);
-  T wxA[KR], wxB[KR], wyA[KR], wyB[KR];
+  WCell<T> wcA[KR], wcB[KR];
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
-    Pair<T> rr{0, 0}, x0{0, 0}, wx2{0, 0}, wy2{0, 0};
+    Pair<T> rr{0, 0}, x0{0, 0};
+    wcA[m] = WCell<T>{};
+    wcB[m] = WCell<T>{};
     if (S.v[m]) {
       const unsigned k = cell_u(giA, gy0 + KR * w + m, nx);
       if (!LEVEL0) rr = ld_at<Pair<T>>(r_lvl, k);
       x0 = ld_at<Pair<T>>(x_in, k);
-      wx2 = ld_at<Pair<T>>(F.wx, k);
-      wy2 = ld_at<Pair<T>>(F.wy, k);
+      ld_wpair<T>(F, k, wcA[m], wcB[m]);
     }
     S.rA[m] = rr.a;
     S.rB[m] = rr.b;
     S.xA[m] = x0.a;
     S.xB[m] = x0.b;
-    wxA[m] = wx2.a;
-    wxB[m] = wx2.b;
-    wyA[m] = wy2.a;
-    wyB[m] = wy2.b;
   }
   const int rp = LEVEL0 ? s->rpar : 0;
   if (s->done) return;
@@ -868,7 +923,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
     J2 = J2 < 0 ? 0 : (J2 > cny - 1 ? cny - 1 : J2);
     const int IA2 = I - 1 < 0 ? 0 : I - 1, IB2 = I + 1 > cnx - 1 ? cnx - 1 : I + 1;
     const int a = I - cxa, aA = IA2 - cxa, aB = IB2 - cxa, b = J - cya, b2 = J2 - cya;
-    const T wxa = wxA[m], wya = wyA[m], wxb = wxB[m], wyb = wyB[m];
+    const T wxa = wcA[m].x(), wya = wcA[m].y(), wxb = wcB[m].x(), wyb = wcB[m].y();
     S.xA[m] += wxa * wya * xcs[b][a] + ((T)1 - wxa) * wya * xcs[b][aA] + wxa * ((T)1 - wya) * xcs[b2][a] +
                ((T)1 - wxa) * ((T)1 - wya) * xcs[b2][aA];
     S.xB[m] += wxb * wyb * xcs[b][a] + ((T)1 - wxb) * wyb * xcs[b][aB] + wxb * ((T)1 - wyb) * xcs[b2][a] +
@@ -1018,8 +1073,9 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
         if (nl == 1) sm[L.r[0] + e] = rr[sl];
       }
       if (q + 1 < nl) {
-        sm[L.wx[q] + e] = F.wx[e];
-        sm[L.wy[q] + e] = F.wy[e];
+        const WCell<T> wc = ld_wcell<T>(F, e);
+        sm[L.wx[q] + e] = wc.x();
+        sm[L.wy[q] + e] = wc.y();
       }
       sm[L.x[q] + e] = (T)0;
     }

@@ -39,12 +39,13 @@ HBM_PEAK_GBS = 8000.0
 # the solves perturbs them: the events' cache maintenance slows the timed kernel
 # by ~40% and the projection by ~7%.)
 KERNELS = [
-    (2, "k_mg_pre<float,true>", 45,
+    (2, "k_mg_pre<float,true>", 41,
      "Krylov update r -= a q, x += a p, 3 red-black sweeps from 0, residual, restriction: "
-     "reads r q beta_x beta_y wx wy x p (32 B), writes r x x_pre (12 B) and coarse r (1 B)"),
-    (3, "k_mg_post<float,true>", 29,
-     "prolongation, 3 black-red sweeps, r.z: reads r x_pre beta_x beta_y wx wy (24 B) and coarse x "
-     "(1 B), writes z (4 B)"),
+     "reads r q beta_x beta_y x p (24 B) and the packed binary16 weights wx|wy (4 B), writes r x x_pre "
+     "(12 B) and coarse r (1 B)"),
+    (3, "k_mg_post<float,true>", 25,
+     "prolongation, 3 black-red sweeps, r.z: reads r x_pre beta_x beta_y (16 B), the packed weights "
+     "(4 B) and coarse x (1 B), writes z (4 B)"),
     (0, "k_dir<float>", 24,
      "p = z - mean z + beta p_old, q = A p, p.q: reads z p_old beta_x beta_y (16 B), writes p q (8 B)"),
     (4, "k_correct_resid<float,float>", 64,
@@ -53,27 +54,29 @@ KERNELS = [
 DOMINANT = "k_mg_pre<float,true>"
 
 # Algorithmic bytes per cell of one whole mixed projection, by the same rule as KERNELS:
-# a V-cycle is the finest pre + post + dir (45 + 29 + 24) plus, on each coarser level l
-# (N / 4^l cells), pre (reads r beta_x beta_y wx wy, writes x and coarse r: 25) and post
-# (reads r x_pre beta_x beta_y wx wy and coarse x, writes x: 29), summing to 54 N / 3; a
-# refinement pass is 64; the once-per-projection kernels are the coefficients (reads phi,
-# writes beta_x beta_y jc in FP64: 32), the right-hand side (reads u v beta_x beta_y, writes
-# b and the FP32 residual: 44), the multigrid setup (FP64 beta in, FP32 beta and weights
-# out, coarse beta, over the hierarchy: 48), the face update (72) and the divergence check (16).
+# a V-cycle is the finest pre + post + dir (41 + 25 + 24) plus, on each coarser level l
+# (N / 4^l cells), pre (reads r beta_x beta_y and the packed weights, writes x and coarse r:
+# 21) and post (reads r x_pre beta_x beta_y, the packed weights and coarse x, writes x: 25),
+# summing to 46 N / 3; a refinement pass is 64; the once-per-projection kernels are the
+# coefficients (reads phi, writes beta_x beta_y jc in FP64: 32), the right-hand side (reads
+# u v beta_x beta_y, writes b and the FP32 residual: 44), the multigrid setup (FP64 beta in,
+# FP32 beta and packed weights out, coarse beta, over the hierarchy: 44), the face update
+# (72) and the divergence check (16).
 # An iteration is one full V-cycle with its direction; each pass ends with one more finest
-# pre-smoothing (its update and convergence test, 45), after which the rest of that
+# pre-smoothing (its update and convergence test, 41), after which the rest of that
 # V-cycle is not launched.
-VCYCLE_B_PER_CELL = 45 + 29 + 24 + 54.0 / 3.0
-HEAD_B_PER_CELL = 45
+VCYCLE_B_PER_CELL = 41 + 25 + 24 + 46.0 / 3.0
+HEAD_B_PER_CELL = 41
 REFINE_B_PER_CELL = 64
-SETUP_B_PER_CELL = 32 + 44 + 48 + 72 + 16
+SETUP_B_PER_CELL = 32 + 44 + 44 + 72 + 16
 
 
-# The FP64 path moves every Krylov vector, coefficient and weight in FP64: twice the
-# V-cycle and head bytes; its refinement reads b p_hi p_lo x beta_x beta_y and writes
-# p_hi p_lo r, all FP64 (72).
-FP64_VCYCLE_B_PER_CELL = 2 * VCYCLE_B_PER_CELL
-FP64_HEAD_B_PER_CELL = 2 * HEAD_B_PER_CELL
+# The FP64 path moves every Krylov vector, coefficient and weight (wx and wy apart) in FP64:
+# finest pre 90 (r q beta_x beta_y wx wy x p read, r x x_pre written, coarse r), post 58,
+# dir 48, coarser levels 2 x 54 N / 3; its refinement reads b p_hi p_lo x beta_x beta_y and
+# writes p_hi p_lo r, all FP64 (72).
+FP64_VCYCLE_B_PER_CELL = 90 + 58 + 48 + 108.0 / 3.0
+FP64_HEAD_B_PER_CELL = 90
 FP64_REFINE_B_PER_CELL = 72
 
 

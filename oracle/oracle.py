@@ -69,6 +69,7 @@ def lib():
             "orc_rhs_consistent": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
             "orc_update_consistent": (None, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pd]),
             "orc_update_reference_literal": (None, [_i, _i, _d, _d, _pd, _pd, _d, _d, _pd, _pd, _pd]),
+            "orc_half_via_float": (_d, [_d]),
             "orc_mg_num_levels": (_i, [_i, _i, _i, _i]),
             "orc_mg_create": (_vp, [_i, _i, _pd, _pd, _i, _i, _i, _i, _i]),
             "orc_mg_destroy": (None, [_vp]),
@@ -389,8 +390,15 @@ def update_reference_literal(phi, jc, rho_minus, rho_plus, p, u, v, h, dt=1.0):
 
 
 # ----------------------------------------------------------------------------- multigrid mirror
-# mode bit0: harmonic-normal coarse beta; bit1: operator-dependent interpolation
+# mode bit0: harmonic-normal coarse beta; bit1: operator-dependent interpolation;
+# bit2: weights rounded to binary16 through binary32 (the GPU's FP32 hierarchy)
 MG_DEFAULT_MODE = 3
+MG_MODE_F32 = MG_DEFAULT_MODE | 4
+
+
+def half_via_float(w):
+    """w rounded to binary32, then to binary16 (nearest, ties to even), as a float."""
+    return lib().orc_half_via_float(float(w))
 
 
 class MG:

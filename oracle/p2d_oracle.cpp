@@ -759,6 +759,8 @@ void orc_update_reference_literal(int nx, int ny, double h, double dt, const dou
 //      (od_weight; 3/4 for constant coefficients, 1 at a wall); R = P^T.
 // Mode 0 (tests only) keeps the plain forms: beta_H = (beta_a + beta_b)/2 and cell-centred
 // bilinear P with weights (9,3,3,1)/16.
+// Mode bit 2 (mode 7, the GPU's FP32 hierarchy) stores every weight rounded to binary16
+// through binary32 (orc_half_via_float), as the FP32 levels do; R = P^T still holds.
 // Smoother: red-black Gauss-Seidel, pre (red, black) x nu, post (black, red) x nu.
 // Bottom: pseudo-inverse of the coarsest operator (constants in its null space), so the
 // V-cycle treats the constant mode exactly as the singular fine operator does.
@@ -772,7 +774,7 @@ struct MGLevel {
 struct MG {
   std::vector<MGLevel> L;
   int nu_pre, nu_post;
-  int mode;                        // bit0: harmonic-normal coarse beta; bit1: operator-dependent P
+  int mode;                        // bit0: harmonic-normal coarse beta; bit1: operator-dependent P; bit2: binary16 weights
   std::vector<double> bottom_inv;  // dense inverse of the coarsest operator
 };
 
@@ -826,7 +828,17 @@ static double od_weight(const double* bface, int stride, int i, int nf) {
   if (!(near < 1e300) || !(far < 1e300)) return far >= near ? 1.0 : 0.0;
   return far / (near + far);
 }
-static void build_weights(MGLevel& f, int op_dep) {
+// w rounded to binary32, then to binary16 (round to nearest, ties to even; binary16
+// subnormals below 2^-14), as a double.  For |w| <= 65504.
+double orc_half_via_float(double w) {
+  const float f = (float)w;
+  if (f == 0.0f || !std::isfinite(f)) return f;
+  int e;
+  std::frexp(f, &e);                           // |f| in [2^(e-1), 2^e)
+  const int q = std::max(e - 1, -14) - 10;     // the binary16 quantum of f's binade is 2^q
+  return std::ldexp(std::nearbyint(std::ldexp((double)f, -q)), q);
+}
+static void build_weights(MGLevel& f, int op_dep, bool half) {
   f.wx.assign((size_t)f.nx * f.ny, 0.75);
   f.wy.assign((size_t)f.nx * f.ny, 0.75);
   for (int j = 0; j < f.ny; ++j)
@@ -840,8 +852,8 @@ static void build_weights(MGLevel& f, int op_dep) {
       }
       if (Q < 0 || Q > (f.nx >> 1) - 1) wx = 1.0;
       if (Qj < 0 || Qj > (f.ny >> 1) - 1) wy = 1.0;
-      f.wx[C(i, j, f.nx)] = wx;
-      f.wy[C(i, j, f.nx)] = wy;
+      f.wx[C(i, j, f.nx)] = half ? orc_half_via_float(wx) : wx;
+      f.wy[C(i, j, f.nx)] = half ? orc_half_via_float(wy) : wy;
     }
 }
 // 1-D weight of coarse index I for fine index i given the parent weight w.
@@ -957,7 +969,7 @@ void* orc_mg_create(int nx, int ny, const double* bx, const double* by, int max_
       std::memcpy(lv.by.data(), by, sizeof(double) * lv.by.size());
     } else {
       MGLevel& f = mg->L[l - 1];
-      build_weights(f, (mode >> 1) & 1);
+      build_weights(f, (mode >> 1) & 1, (mode >> 2) & 1);
       for (int J = 0; J < lv.ny; ++J)
         for (int I = 0; I <= lv.nx; ++I) {
           double v;

@@ -56,7 +56,8 @@ def test_vcycle_matches_oracle(P2D, O, W, prec, nu):
     bx, by, jc = _coeffs(O, w)
     ctx = P2D.Context(w.n, w.n, w.h, mg_pre_sweeps=nu, mg_post_sweeps=nu)
     ctx.set_coefficients(bx, by, w.phi, jc)
-    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu)
+    # the FP32 hierarchy stores its weights in binary16: its oracle is mode 7
+    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(6).standard_normal((w.n, w.n))
     r -= r.mean()
@@ -82,7 +83,7 @@ def test_vcycle_rectangular_matches_oracle(P2D, O, nx, ny, prec):
     bx, by = O.face_beta(phi, 1.0, 1000.0)
     ctx = P2D.Context(nx, ny, h, mg_pre_sweeps=2, mg_post_sweeps=2)
     ctx.set_coefficients(bx, by, phi, np.zeros_like(phi))
-    mg = O.MG(bx, by, nu_pre=2, nu_post=2)
+    mg = O.MG(bx, by, nu_pre=2, nu_post=2, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(7).standard_normal((ny, nx))
     r -= r.mean()
@@ -147,11 +148,13 @@ def test_vcycle_2048_with_two_block_coarse_level(P2D, O, W):
     ctx = P2D.Context(w.n, w.n, w.h)
     ctx.set_coefficients(bx, by, w.phi, jc)
     mg = O.MG(bx, by, nu_pre=3, nu_post=3)
+    mg16 = O.MG(bx, by, nu_pre=3, nu_post=3, mode=O.MG_MODE_F32)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(8).standard_normal((w.n, w.n))
     r -= r.mean()
     ref = mg.vcycle(r)
+    ref16 = mg16.vcycle(r)
     z = ctx.vcycle(r.astype(np.float32), prec=1)
-    assert np.linalg.norm(z - ref) <= 1e-4 * np.linalg.norm(ref)
+    assert np.linalg.norm(z - ref16) <= 1e-4 * np.linalg.norm(ref16)
     z64 = ctx.vcycle(r, prec=0)
     assert np.linalg.norm(z64 - ref) <= 1e-11 * np.linalg.norm(ref)

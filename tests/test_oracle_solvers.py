@@ -82,6 +82,37 @@ def test_vcycle_symmetric_positive_linear(O, mg_case):
     assert np.linalg.norm(mg.vcycle(2 * x - 3 * y) - (2 * Mx - 3 * My)) <= 1e-12 * np.linalg.norm(Mx)
 
 
+def test_half_rounding_matches_numpy(O):
+    """orc_half_via_float (how the FP32 hierarchy stores its weights) equals numpy's binary32 ->
+    binary16 casts (nearest, ties to even) on random weights, log-spread values down to the
+    binary16 subnormals, exact binary16 ties and the ends of [0, 1]."""
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.random(4000), np.exp(rng.uniform(np.log(1e-9), 0.0, 4000)),
+                        np.arange(1, 2048) * 2.0 ** -11 + 2.0 ** -12,     # ties between binary16 neighbours
+                        np.arange(1, 512) * 2.0 ** -25,                   # around the subnormal quantum 2^-24
+                        [0.0, 1.0, 0.99999, 0.9999999, 6.1e-5, 2.0 ** -14, 2.0 ** -24, 2.0 ** -25]])
+    got = np.array([O.half_via_float(v) for v in x])
+    ref = np.float16(np.float32(x)).astype(np.float64)
+    assert np.array_equal(got, ref)
+
+
+def test_vcycle_with_binary16_weights(O, mg_case):
+    """Mode 7 (the FP32 hierarchy): every weight rounded to binary16.  The cycle stays a
+    symmetric positive linear operator (R = P^T with the rounded weights) and differs from
+    mode 3 by the rounding only."""
+    w, bx, by, mg = mg_case
+    mg16 = O.MG(bx, by, mode=O.MG_MODE_F32)
+    rng = np.random.default_rng(2)
+    x, y = rng.standard_normal((2, w.n, w.n))
+    x -= x.mean()
+    y -= y.mean()
+    Mx, My = mg16.vcycle(x), mg16.vcycle(y)
+    assert abs(np.vdot(Mx, y) - np.vdot(x, My)) <= 1e-12 * abs(np.vdot(Mx, y))
+    assert np.vdot(x, Mx) > 0 and np.vdot(y, My) > 0
+    d = np.linalg.norm(Mx - mg.vcycle(x)) / np.linalg.norm(Mx)
+    assert 0 < d < 1e-2, d
+
+
 def test_preconditioned_spectrum(O, W):
     """On mean-zero vectors M is SPD and M A' has a bounded spread; constants are in M's null space."""
     w = W.make("drops64", 32)
