@@ -157,6 +157,15 @@ struct p2d_ctx {
   double* partial = nullptr;
   Scal* s = nullptr;
   Scal* hs = nullptr;  // pinned host mirror
+  // The host's replay of the Krylov bookkeeping of the inner iterations it has
+  // enqueued (the device's rpar, iter, pending, seg0 as each enqueued kernel will
+  // find them, while the solve runs): the finest-level kernels get their residual
+  // and direction buffers from it, so their loads need not wait for the state.
+  // Reloaded from hs at every synchronisation; once the device has set done, the
+  // kernels return without touching the buffers and a stale replay is harmless.
+  struct Track {
+    int rpar = 0, iter = 0, pending = 0, seg0 = 0;
+  } trk;
   int last_seg[2][MAX_PASSES] = {};  // iterations of each pass of the last solve (FP64, mixed)
   int last_passes[2] = {0, 0};       // refinement passes of the last solve
 };
@@ -244,6 +253,10 @@ int sync_scal(p2d_ctx* c) {
   HIPCHK(hipMemcpyAsync(c->hs, c->s, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   kt_harvest(c);
+  c->trk.rpar = c->hs->rpar;
+  c->trk.iter = c->hs->iter;
+  c->trk.pending = c->hs->pending;
+  c->trk.seg0 = c->hs->seg0;
   return P2D_OK;
 }
 
@@ -386,11 +399,18 @@ void vcycle_head(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const Level<T>* lv = mg.lv.data();
   const dim3 gpre = tgrid(lv[0].nx, lv[0].ny, 2 * nu1 + 1);
   const int kpre = sizeof(T) == 4 ? KT_PRE0_F32 : KT_PRE0_F64;
+  p2d_ctx::Track& t = c->trk;
   kt_begin(c, kpre);
-  k_mg_pre<T, true><<<gpre, TBLOCK, 0, c->stream>>>(lv[0], lv[1], nullptr, lv[0].xa, nu1, v.r[0], v.r[1], v.x,
-                                                    v.p[0], v.p[1], v.q, c->partial, c->s);
+  k_mg_pre<T, true><<<gpre, TBLOCK, 0, c->stream>>>(lv[0], lv[1], v.r[t.rpar], lv[0].xa, nu1, v.r[1 - t.rpar], v.x,
+                                                    v.p[(t.iter + 1) & 1], v.q, t.pending, c->partial, c->s);
   kt_end(c, kpre);
   k_fin_pre0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpre), (double)c->N, c->s);
+  // k_fin_pre0: the residual buffers swap; a pending update counts as an iteration
+  t.rpar = 1 - t.rpar;
+  if (t.pending) {
+    t.iter += 1;
+    t.pending = 0;
+  }
 }
 
 template <class T>
@@ -406,11 +426,10 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
     const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpre);
     if (busy(l, Hpre))
       k_mg_pre<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr,
-                                                            nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                            c->s);
+                                                            nullptr, nullptr, nullptr, 0, nullptr, c->s);
     else
       k_mg_pre<T, false><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr,
-                                                      nullptr, nullptr, nullptr, nullptr, nullptr, c->s);
+                                                      nullptr, nullptr, 0, nullptr, c->s);
   }
   (void)join_side(c);  // enqueue-only; a failure shows at the next launch check
   SmallArgs<T> sa{};
@@ -421,14 +440,14 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   for (int l = mg.l0 - 1; l >= 1; --l) {
     const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpost);
     if (busy(l, Hpost))
-      k_mg_post<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa,
-                                                             lv[l].xb, nu2, nullptr, c->s);
+      k_mg_post<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, lv[l].xb, nu2,
+                                                             nullptr, c->s);
     else
-      k_mg_post<T, false><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, nullptr, nullptr, lv[l].xa,
-                                                       lv[l].xb, nu2, nullptr, c->s);
+      k_mg_post<T, false><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, lv[l].xb, nu2, nullptr,
+                                                       c->s);
   }
   kt_begin(c, kpost);
-  k_mg_post<T, true><<<gpost, TBLOCK, 0, c->stream>>>(lv[0], lv[1], nullptr, v.r[0], v.r[1], lv[0].xa, v.z, nu2,
+  k_mg_post<T, true><<<gpost, TBLOCK, 0, c->stream>>>(lv[0], lv[1], v.r[c->trk.rpar], lv[0].xa, v.z, nu2,
                                                       c->partial, c->s);
   kt_end(c, kpost);
   k_fin_post0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpost), (double)c->N, c->s);
@@ -448,10 +467,13 @@ void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const Level<T>& l0 = mg.lv[0];
   const dim3 g = dirgrid(c->nx, c->ny);
   const int kd = sizeof(T) == 4 ? KT_DIR_F32 : KT_DIR_F64;
+  p2d_ctx::Track& t = c->trk;
   kt_begin(c, kd);
-  k_dir<T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, v.p[0], v.p[1], v.q, c->partial, c->s);
+  k_dir<T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, v.p[t.iter & 1], v.p[1 - (t.iter & 1)],
+                                       v.q, t.iter == t.seg0 ? 1 : 0, c->partial, c->s);
   kt_end(c, kd);
   k_fin_dir<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), c->s);
+  t.pending = 1;  // k_fin_dir
 }
 
 // Residual launch plus its finish (sum r^2, max |r|, sum r into s->red[0..2]).
@@ -505,6 +527,7 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
   const auto t_solve = std::chrono::steady_clock::now();
   k_inner_init<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? 0 : 1, rtol, atol, abs2,
                                        c->cfg.max_iterations);
+  c->trk = p2d_ctx::Track{};  // k_inner_init: rpar = iter = pending = seg0 = 0
   k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
   CHK(launched_ok());
   int status = P2D_OK, outer = 0, stalled = 0, iters = 0, seg_start = 0;
@@ -596,6 +619,8 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
     }
     k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
     k_inner_continue<<<1, 1, 0, c->stream>>>(c->s, (double)N);
+    c->trk.seg0 = c->trk.iter;  // k_inner_continue: a new direction sequence, nothing pending
+    c->trk.pending = 0;
     CHK(launched_ok());
     report = true;
     seg_start = iters;
@@ -1171,6 +1196,7 @@ int p2d_vcycle(p2d_ctx* c, const void* r, void* z, int prec, int mem) {
   CHK(ensure_mg(c, prec));
   // armed with no pending update: the fused finest-level kernel only copies r
   k_scal_arm<<<1, 1, 0, c->stream>>>(c->s, 0, 0.0, 0.0, 0);
+  c->trk = p2d_ctx::Track{};
   if (prec == P2D_PREC_F64) {
     HIPCHK(hipMemcpyAsync(c->v64.r[0], r, N * 8, host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, c->stream));
     vcycle<double>(c, c->mg64, c->v64);
@@ -1539,21 +1565,22 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
     switch (which) {
       case 0:
         k_dir<float><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, L32[0].bw, L32[0].bs, c->v32.z,
-                                                                   c->v32.p[0], c->v32.p[1], c->v32.q, c->partial, c->s);
+                                                                   c->v32.p[1], c->v32.p[0], c->v32.q, 0, c->partial,
+                                                                   c->s);
         break;
       case 1:
         k_dir<double><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, c->v64.z,
-                                                                    c->v64.p[0], c->v64.p[1], c->v64.q, c->partial, c->s);
+                                                                    c->v64.p[1], c->v64.p[0], c->v64.q, 0, c->partial,
+                                                                    c->s);
         break;
       case 2:
         k_mg_pre<float, true><<<tgrid(c->nx, c->ny, 2 * nu1 + 1), TBLOCK, 0, c->stream>>>(
-            L32[0], L32[1], nullptr, L32[0].xa, nu1, c->v32.r[0], c->v32.r[1], c->v32.x, c->v32.p[0], c->v32.p[1],
-            c->v32.q, c->partial, c->s);
+            L32[0], L32[1], c->v32.r[0], L32[0].xa, nu1, c->v32.r[1], c->v32.x, c->v32.p[0], c->v32.q, 1, c->partial,
+            c->s);
         break;
       case 3:
         k_mg_post<float, true><<<tgrid(c->nx, c->ny, 2 * nu2), TBLOCK, 0, c->stream>>>(
-            L32[0], L32[1], nullptr, c->v32.r[0], c->v32.r[1], L32[0].xa, c->v32.z, nu2, c->partial,
-            c->s);
+            L32[0], L32[1], c->v32.r[0], L32[0].xa, c->v32.z, nu2, c->partial, c->s);
         break;
       case 5:  // one re-initialisation step of the fluid's level set
         k_reinit_step<<<dgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->h, c->h / 5.0, 1e-6, c->phi,

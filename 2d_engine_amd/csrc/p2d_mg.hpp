@@ -629,15 +629,17 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
 // --------------------------------------------------------------------------- pre-smoothing tile kernel
 // Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, H = 2 nu + 1, x halo
 // HX = tile_hx(H) (even), TX = 128 - 2 HX.
-// FUSE (finest level): r_in = rbuf[s->rpar], the pending update r -= a q,
-// x += a p and the residual shift are applied first (tile interiors write r to
-// rbuf[1 - rpar] and x), and the block partials of ||r|| are stored.
+// FUSE (finest level): r_lvl is the current residual buffer; the update r -= a q,
+// x += a p (pend: an update is pending, p = pdir) and the residual shift are
+// applied first (tile interiors write r to rout and x), and the block partials of
+// ||r|| are stored.  The host picks the buffers from its replay of the solver
+// state, so their loads do not wait for the state.
 // Restriction r_H = P^T res: each cell splits its residual into the four
 // weight products; the x sums run across lanes, the y sums through LDS.
 template <class T, bool FUSE, bool BUSY = false>
 __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE || BUSY>())))
-k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB, T* xsol, const T* pA, const T* pB,
-         const T* __restrict__ qv, double* partial, Scal* s) {
+k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* xsol, const T* pdir,
+         const T* __restrict__ qv, int pend, double* partial, Scal* s) {
   __shared__ Pair<T> lds[TILE_LDS_PAIRS];
   const int H = 2 * nu + 1, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
@@ -678,57 +680,52 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rA, T* rB
       S.rB[m] = r2.b;
     }
   }
-  // the state the update needs is read with `done` (one round trip)
-  int rp = 0, pend_i = 0, it = 0;
+  // finest level: the update's operands, loaded before the state is read (the
+  // buffers come from the host); alpha and the shift are read with `done`
+  Pair<T> q2[KR], x2[KR], p2[KR];
+#pragma unroll
+  for (int m = 0; m < KR && FUSE; ++m) {
+    const int r = KR * w + m;
+    Pair<T> r2{0, 0};
+    q2[m] = x2[m] = p2[m] = Pair<T>{0, 0};
+    if (S.v[m]) {
+      const unsigned k = cell_u(giA, gy0 + r, nx);
+      r2 = ld_at<Pair<T>>(r_lvl, k);
+      if (pend) {
+        q2[m] = ld_at<Pair<T>>(qv, k);
+        if (pair_int && r >= H && r < H + TY) {
+          x2[m] = ld_at<Pair<T>>(xsol, k);
+          p2[m] = ld_at<Pair<T>>(pdir, k);
+        }
+      }
+    }
+    S.rA[m] = r2.a;
+    S.rB[m] = r2.b;
+  }
   double alpha = 0.0, shift = 0.0;
   if (FUSE) {
-    rp = s->rpar;
-    pend_i = s->pending;
     alpha = s->alpha;
     shift = s->rshift;
-    it = s->iter;
   }
   if (s->done) return;
-  const T* rin = r_lvl;
-  T* rout = nullptr;
-  const T* pdir = nullptr;
-  const bool pend = FUSE && pend_i != 0;
-  if (FUSE) {
-    rin = rp ? rB : rA;
-    rout = rp ? rA : rB;
-    pdir = ((it + 1) & 1) ? pB : pA;
-  }
   const T ta = (T)alpha, tshift = (T)shift;
   strip_diag<T>(S);
 #pragma unroll
   for (int m = 0; m < KR && FUSE; ++m) {
     const int r = KR * w + m;
-    T ra = 0, rb = 0;
-    if (S.v[m]) {
-      const unsigned k = cell_u(giA, gy0 + r, nx);
-      const Pair<T> r2 = ld_at<Pair<T>>(rin, k);
-      ra = r2.a;
-      rb = r2.b;
-      {
-        // the inner recursion runs in T: r -= a q, x += a p (a, shift rounded to T)
-        if (pend) {
-          const Pair<T> q2 = ld_at<Pair<T>>(qv, k);
-          ra -= ta * q2.a;
-          rb -= ta * q2.b;
-        }
-        ra -= tshift;
-        rb -= tshift;
-        if (pair_int && r >= H && r < H + TY) {
-          st_at(rout, k, Pair<T>{ra, rb});
-          if (pend) {
-            Pair<T> x2 = ld_at<Pair<T>>(xsol, k);
-            const Pair<T> p2 = ld_at<Pair<T>>(pdir, k);
-            x2.a += ta * p2.a;
-            x2.b += ta * p2.b;
-            st_at(xsol, k, x2);
-          }
-        }
-      }
+    if (!S.v[m]) continue;
+    const unsigned k = cell_u(giA, gy0 + r, nx);
+    // the inner recursion runs in T: r -= a q, x += a p (a, shift rounded to T)
+    T ra = S.rA[m], rb = S.rB[m];
+    if (pend) {
+      ra -= ta * q2[m].a;
+      rb -= ta * q2[m].b;
+    }
+    ra -= tshift;
+    rb -= tshift;
+    if (pair_int && r >= H && r < H + TY) {
+      st_at(rout, k, Pair<T>{ra, rb});
+      if (pend) st_at(xsol, k, Pair<T>{x2[m].a + ta * p2[m].a, x2[m].b + ta * p2[m].b});
     }
     S.rA[m] = ra;
     S.rB[m] = rb;
@@ -844,15 +841,14 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
 // Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, H = 2 nu (even): HX = H,
 // TX = 128 - 2H.  x = x_pre + P x_coarse (the coarse values around the tile are
 // staged in LDS while the strip's own loads are in flight), then nu (black,
-// red) sweeps; the tile interior is written to x_out.  LEVEL0: r =
-// rbuf[s->rpar], x_out = z, and the block partials of r.z, sum z, sum r are
-// stored.
+// red) sweeps; the tile interior is written to x_out.  LEVEL0: r_lvl is the
+// current residual buffer (picked by the host), x_out = z, and the block
+// partials of r.z, sum z, sum r are stored.
 constexpr int PCW = 66;                           // staged coarse columns
 constexpr int PCH = (TY + 4 * MAX_NU) / 2 + 3;    // staged coarse rows
 template <class T, bool LEVEL0, bool BUSY = false>
 __global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, LEVEL0 || BUSY>())))
-k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, const T* x_in, T* x_out, int nu,
-          double* partial, Scal* s) {
+k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* x_in, T* x_out, int nu, double* partial, Scal* s) {
   __shared__ Pair<T> lds[TILE_LDS_PAIRS];
   __shared__ T xcs[PCH][PCW];
   const int H = 2 * nu, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
@@ -866,8 +862,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
   const int hc = HX / 2, NCX = TX / 2;
   const bool pair_int = c >= hc && c < hc + NCX;
   const int cxa = gx0 / 2 - 1, cya = (gy0 >> 1) - 1;  // first staged coarse column / row (gx0 is even)
-  // Every load but the finest level's residual is independent of the solver state and
-  // is issued before `done` is read (the coarse values go to LDS after the test).
+  // Every load is issued before `done` is read (the coarse values go to LDS after the test).
   constexpr int XCS_PER_THREAD = (PCH * PCW + TBLOCK - 1) / TBLOCK;
   T xcv[XCS_PER_THREAD];
 #pragma unroll
@@ -886,7 +881,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
     wcB[m] = WCell<T>{};
     if (S.v[m]) {
       const unsigned k = cell_u(giA, gy0 + KR * w + m, nx);
-      if (!LEVEL0) rr = ld_at<Pair<T>>(r_lvl, k);
+      rr = ld_at<Pair<T>>(r_lvl, k);
       x0 = ld_at<Pair<T>>(x_in, k);
       ld_wpair<T>(F, k, wcA[m], wcB[m]);
     }
@@ -895,18 +890,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* rA, const T* rB, con
     S.xA[m] = x0.a;
     S.xB[m] = x0.b;
   }
-  const int rp = LEVEL0 ? s->rpar : 0;
   if (s->done) return;
-  const T* rin = LEVEL0 ? (rp ? rB : rA) : r_lvl;
-  if (LEVEL0) {
-#pragma unroll
-    for (int m = 0; m < KR; ++m) {
-      Pair<T> rr{0, 0};
-      if (S.v[m]) rr = ld_at<Pair<T>>(rin, cell_u(giA, gy0 + KR * w + m, nx));
-      S.rA[m] = rr.a;
-      S.rB[m] = rr.b;
-    }
-  }
 #pragma unroll
   for (int q = 0; q < XCS_PER_THREAD; ++q) {
     const int t = (int)threadIdx.x + q * TBLOCK;
@@ -1205,9 +1189,9 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 }
 
 // --------------------------------------------------------------------------- Krylov direction kernel
-// p = (z - mean z) + beta p_old (FP64 arithmetic, stored as T; p_old/p_new =
-// pbuf[iter & 1] / pbuf[(iter+1) & 1]), q = A0' p, alpha = rz / p.q; marks the
-// update pending.  Tile kernel over 128 x DTY cells: lane c of a wavefront holds
+// p = (z - mean z) + beta p_old (FP64 arithmetic, stored as T; first: the first
+// direction of a sequence, p_old unused; the host picks p_old and p_new from its
+// replay of the solver state), q = A0' p, the block partials of p.q.  Tile kernel over 128 x DTY cells: lane c of a wavefront holds
 // the column pair (2c, 2c+1) of a row, so every array moves with one two-element
 // load or store per lane.  p of the tile plus a one-cell ring is formed in LDS
 // (all loads issued at once), then the stencil reads LDS.  The finest level has
@@ -1216,8 +1200,8 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 constexpr int DTX = 128;  // columns per block of the direction kernel
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restrict__ bw, const T* __restrict__ bs,
-                                               const T* __restrict__ z, T* pA, T* pB, T* __restrict__ q,
-                                               double* partial, Scal* s) {
+                                               const T* __restrict__ z, const T* pold, T* pnew, T* __restrict__ q,
+                                               int first, double* partial, Scal* s) {
   // row layout: [1] west ring, [2 + 2c, 3 + 2c] pair c, [DTX + 2] east ring (pairs 2-element aligned)
   __shared__ Pair<T> ps2[DTY + 2][DTX / 2 + 2];
   T(*ps)[DTX + 4] = reinterpret_cast<T(*)[DTX + 4]>(&ps2[0][0]);
@@ -1250,12 +1234,31 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     wB[m] = w2.b;
     eL[m] = (c == 63 && iA + 2 < nx && j < ny) ? ld_at<T>(bw, cell_u(iA + 2, j, nx)) : (T)0;
   }
+  // z and p_old of the tile and its ring, loaded before the state is read
+  Pair<T> z2[RR], o2[RR];
+#pragma unroll
+  for (int m = 0; m < RR; ++m) {
+    const int r = w + NW * m, jj = j0 - 1 + r;
+    z2[m] = o2[m] = Pair<T>{(T)0, (T)0};
+    if (r < DTY + 2 && col_in && jj >= 0 && jj < ny) {
+      const unsigned k = cell_u(iA, jj, nx);
+      z2[m] = ld_at<Pair<T>>(z, k);
+      if (!first) o2[m] = ld_at<Pair<T>>(pold, k);
+    }
+  }
+  T zr = 0, orr = 0;
+  const bool ring = threadIdx.x < 2 * (DTY + 2);
+  const int side = (int)threadIdx.x / (DTY + 2), rr_ = (int)threadIdx.x % (DTY + 2);
+  if (ring) {
+    const int ii = side ? i0 + DTX : i0 - 1, jj = j0 - 1 + rr_;
+    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
+      const unsigned k = cell_u(ii, jj, nx);
+      zr = z[k];
+      if (!first) orr = pold[k];
+    }
+  }
   const double beta = s->beta, zm = s->zmean;
-  const int it = s->iter, seg0 = s->seg0;
   if (s->done) return;
-  const bool first = it == seg0;
-  const T* pold = (it & 1) ? pB : pA;
-  T* pnew = (it & 1) ? pA : pB;
   auto pform = [&](T zz, T po) -> T {
     return first ? (T)((double)zz - zm) : (T)(((double)zz - zm) + beta * (double)po);
   };
@@ -1265,23 +1268,14 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     if (r >= DTY + 2) continue;
     Pair<T> v{(T)0, (T)0};
     if (col_in && jj >= 0 && jj < ny) {
-      const unsigned k = cell_u(iA, jj, nx);
-      const Pair<T> z2 = ld_at<Pair<T>>(z, k);
-      const Pair<T> o2 = first ? Pair<T>{(T)0, (T)0} : ld_at<Pair<T>>(pold, k);
-      v.a = pform(z2.a, o2.a);
-      v.b = pform(z2.b, o2.b);
+      v.a = pform(z2[m].a, o2[m].a);
+      v.b = pform(z2[m].b, o2[m].b);
     }
     ps2[r][1 + c] = v;
   }
-  if (threadIdx.x < 2 * (DTY + 2)) {
-    const int side = (int)threadIdx.x / (DTY + 2), r = (int)threadIdx.x % (DTY + 2);
-    const int ii = side ? i0 + DTX : i0 - 1, jj = j0 - 1 + r;
-    T v = 0;
-    if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
-      const unsigned k = cell_u(ii, jj, nx);
-      v = pform(z[k], first ? (T)0 : pold[k]);
-    }
-    ps[r][side ? DTX + 2 : 1] = v;
+  if (ring) {
+    const int ii = side ? i0 + DTX : i0 - 1, jj = j0 - 1 + rr_;
+    ps[rr_][side ? DTX + 2 : 1] = (ii >= 0 && ii < nx && jj >= 0 && jj < ny) ? pform(zr, orr) : (T)0;
   }
 #pragma unroll
   for (int m = 0; m < RPW; ++m) {
