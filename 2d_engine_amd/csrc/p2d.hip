@@ -268,6 +268,18 @@ int join_side(p2d_ctx* c) {
   return P2D_OK;
 }
 
+// The single-workgroup cycle for nl levels (1 .. SMALL_LV).
+template <class T>
+const void* small_kernel(int nl) {
+  static_assert(SMALL_LV == 4, "one instantiation per level count");
+  switch (nl) {
+    case 1: return reinterpret_cast<const void*>(&k_mg_small<T, 1>);
+    case 2: return reinterpret_cast<const void*>(&k_mg_small<T, 2>);
+    case 3: return reinterpret_cast<const void*>(&k_mg_small<T, 3>);
+    default: return reinterpret_cast<const void*>(&k_mg_small<T, 4>);
+  }
+}
+
 // Levels: l = 0 uses the context's coefficients (bw0/bs0) when given; every
 // level owns xa, xb, r; levels above l0 own their interpolation weights.
 template <class T>
@@ -291,7 +303,7 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
   if (mg.l0 < 0) return P2D_BAD_ARG;
   mg.small_lds = small_bytes(mg.l0);
   if (mg.small_lds > 65536 &&
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_small<T>), hipFuncAttributeMaxDynamicSharedMemorySize,
+      hipFuncSetAttribute(small_kernel<T>(L - mg.l0), hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)mg.small_lds) != hipSuccess)
     return P2D_HIP_ERROR;
   for (int l = 0; l < L; ++l) {
@@ -436,7 +448,12 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   sa.nl = mg.L - mg.l0;
   for (int q = 0; q < sa.nl; ++q) sa.lv[q] = lv[mg.l0 + q];
   sa.binv = mg.binv;
-  k_mg_small<T><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s);
+  switch (sa.nl) {
+    case 1: k_mg_small<T, 1><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s); break;
+    case 2: k_mg_small<T, 2><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s); break;
+    case 3: k_mg_small<T, 3><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s); break;
+    default: k_mg_small<T, 4><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s); break;
+  }
   for (int l = mg.l0 - 1; l >= 1; --l) {
     const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpost);
     if (busy(l, Hpost))

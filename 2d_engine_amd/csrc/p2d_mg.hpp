@@ -1017,55 +1017,104 @@ __host__ __device__ inline SmallLayout small_layout(const Level<T>* lv, int nl) 
   return o;
 }
 
-template <class T>
+// NL = A.nl (a template parameter: the level loops unroll into straight-line
+// code, so the arguments and the first loads of every level are issued at once).
+template <class T, int NL>
 __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int nu_pre, int nu_post, const Scal* s) {
   const int done = s->done;  // tested once the loads are issued: their round trips overlap
   extern __shared__ unsigned char smem_raw[];
   T* sm = reinterpret_cast<T*>(smem_raw);
-  const int nl = A.nl, t = threadIdx.x;
+  constexpr int nl = NL;
+  const int t = threadIdx.x;
   const SmallLayout L = small_layout<T>(A.lv, nl);
   int nb = 0;
-  int ci_[SMALL_SLOTS], cj_[SMALL_SLOTS];
-  T cw[SMALL_SLOTS], ce[SMALL_SLOTS], cs[SMALL_SLOTS], cn[SMALL_SLOTS], cd[SMALL_SLOTS], id[SMALL_SLOTS],
-      rr[SMALL_SLOTS];
-  // every global load of the cycle: coefficients and the top level's r
-  // (registers), weights and the bottom pseudo-inverse (LDS); x cleared
+#pragma unroll
+  for (int q = 0; q < SMALL_LV; ++q)
+    if (q == nl - 1) nb = A.lv[q].nx * A.lv[q].ny;
+  // slots of levels below the bottom stay zero (cd = 0: their 1/diag is 0)
+  int ci_[SMALL_SLOTS] = {}, cj_[SMALL_SLOTS] = {};
+  T cw[SMALL_SLOTS] = {}, ce[SMALL_SLOTS] = {}, cs[SMALL_SLOTS] = {}, cn[SMALL_SLOTS] = {}, cd[SMALL_SLOTS] = {},
+    id[SMALL_SLOTS] = {}, rr[SMALL_SLOTS] = {};
+  // Every global load of the cycle is issued before any is used (one round trip):
+  // the bottom pseudo-inverse and the weights first (they go to LDS), then the
+  // coefficients and the top level's r (registers); x is cleared.
+  constexpr int BINV_PT = BOTTOM_LDS_MAX * BOTTOM_LDS_MAX / SMALL_THREADS;
+  static_assert(BINV_PT * SMALL_THREADS == BOTTOM_LDS_MAX * BOTTOM_LDS_MAX, "pseudo-inverse staging");
+  const bool binv_lds = nb <= BOTTOM_LDS_MAX;
+  T bv[BINV_PT];
+#pragma unroll
+  for (int k = 0; k < BINV_PT; ++k) {
+    const int e = t + k * SMALL_THREADS;
+    bv[k] = binv_lds && e < nb * nb ? A.binv[e] : (T)0;
+  }
+  WCell<T> wc[SMALL_SLOTS];
+#pragma unroll
+  for (int q = 0; q + 1 < SMALL_LV; ++q) {
+    if (q + 1 >= nl) break;
+    const Level<T> F = A.lv[q];
+    const int n = F.nx * F.ny;
+#pragma unroll
+    for (int u = 0; u < small_cpt(q); ++u) {
+      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS;
+      wc[sl] = e < n ? ld_wcell<T>(F, e) : WCell<T>{};
+    }
+  }
 #pragma unroll
   for (int q = 0; q < SMALL_LV; ++q) {
     if (q >= nl) break;
     const Level<T> F = A.lv[q];
     const int nx = F.nx, ny = F.ny, n = nx * ny;
-    if (q == nl - 1) nb = n;
 #pragma unroll
     for (int u = 0; u < small_cpt(q); ++u) {
       const int sl = small_slot(q, u), e = t + u * SMALL_THREADS;
       ci_[sl] = cj_[sl] = 0;
-      cw[sl] = ce[sl] = cs[sl] = cn[sl] = cd[sl] = id[sl] = rr[sl] = (T)0;
+      cw[sl] = ce[sl] = cs[sl] = cn[sl] = rr[sl] = (T)0;
       if (e >= n) continue;
       const int j = e / nx, i = e - j * nx;
       ci_[sl] = i;
       cj_[sl] = j;
-      const T w = F.bw[e], ea = i + 1 < nx ? F.bw[e + 1] : (T)0;
-      const T so = F.bs[e], no = j + 1 < ny ? F.bs[e + nx] : (T)0;
-      cd[sl] = w + ea + so + no;
-      cw[sl] = i > 0 ? w : (T)0;
-      ce[sl] = ea;
-      cs[sl] = j > 0 ? so : (T)0;
-      cn[sl] = no;
-      if (q == 0) {
-        rr[sl] = F.r[e];
-        if (nl == 1) sm[L.r[0] + e] = rr[sl];
-      }
+      cw[sl] = F.bw[e];
+      ce[sl] = i + 1 < nx ? F.bw[e + 1] : (T)0;
+      cs[sl] = F.bs[e];
+      cn[sl] = j + 1 < ny ? F.bs[e + nx] : (T)0;
+      if (q == 0) rr[sl] = F.r[e];
+    }
+  }
+  if (binv_lds) {
+#pragma unroll
+    for (int k = 0; k < BINV_PT; ++k) {
+      const int e = t + k * SMALL_THREADS;
+      if (e < nb * nb) sm[L.binv + e] = bv[k];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < SMALL_LV; ++q) {
+    if (q >= nl) break;
+    const int n = A.lv[q].nx * A.lv[q].ny;
+#pragma unroll
+    for (int u = 0; u < small_cpt(q); ++u) {
+      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS;
+      if (e >= n) continue;
       if (q + 1 < nl) {
-        const WCell<T> wc = ld_wcell<T>(F, e);
-        sm[L.wx[q] + e] = wc.x();
-        sm[L.wy[q] + e] = wc.y();
+        sm[L.wx[q] + e] = wc[sl].x();
+        sm[L.wy[q] + e] = wc[sl].y();
       }
       sm[L.x[q] + e] = (T)0;
     }
   }
-  if (nb <= BOTTOM_LDS_MAX)
-    for (int e = t; e < nb * nb; e += SMALL_THREADS) sm[L.binv + e] = A.binv[e];
+#pragma unroll
+  for (int sl = 0; sl < SMALL_SLOTS; ++sl) {
+    // the west / south faces at the walls (i = 0, j = 0) are stored zeros
+    cd[sl] = cw[sl] + ce[sl] + cs[sl] + cn[sl];
+    if (ci_[sl] == 0) cw[sl] = (T)0;
+    if (cj_[sl] == 0) cs[sl] = (T)0;
+  }
+  if (nl == 1 && t < nb) sm[L.r[0] + t] = rr[0];
+  if (nl == 1) {
+#pragma unroll
+    for (int u = 1; u < SMALL_CPT; ++u)
+      if (t + u * SMALL_THREADS < nb) sm[L.r[0] + t + u * SMALL_THREADS] = rr[u];
+  }
   if (done) return;
 #pragma unroll
   for (int sl = 0; sl < SMALL_SLOTS; ++sl) id[sl] = cd[sl] != (T)0 ? (T)1 / cd[sl] : (T)0;
@@ -1146,16 +1195,34 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     __syncthreads();
   }
   {
-    // bottom: x = binv r, one wavefront per row, FP64 accumulation
+    // bottom: x = binv r, FP64 accumulation.  A pseudo-inverse in LDS (nb <= 64):
+    // 16 lanes per row (all rows at once), each summing every 16th column, then
+    // the row's 16 lanes by DPP (quad permutations, row shifts by 4 and 8: the
+    // total lands in the row's last lane).  Larger bottoms: one wavefront per row.
     const T* Rb = sm + L.rb;
     T* Xb = sm + L.xb;
-    const T* Bi = nb <= BOTTOM_LDS_MAX ? sm + L.binv : A.binv;
-    const int lane = t & 63;
-    for (int a = t >> 6; a < nb; a += SMALL_THREADS / 64) {
+    static_assert(BOTTOM_LDS_MAX * 16 == SMALL_THREADS, "16 lanes per pseudo-inverse row");
+    if (binv_lds) {
+      const int a = t >> 4, k = t & 15;
       double acc = 0;
-      for (int qq = lane; qq < nb; qq += 64) acc += (double)Bi[(size_t)a * nb + qq] * (double)Rb[qq];
-      acc = wave_sum(acc);
-      if (lane == 0) Xb[a] = (T)acc;
+#pragma unroll
+      for (int m = 0; m < BOTTOM_LDS_MAX / 16; ++m) {
+        const int qq = k + 16 * m;
+        if (a < nb && qq < nb) acc += (double)sm[L.binv + a * nb + qq] * (double)Rb[qq];
+      }
+      acc += dpp_move<0xb1>(acc);   // quad_perm [1,0,3,2]
+      acc += dpp_move<0x4e>(acc);   // quad_perm [2,3,0,1]
+      acc += dpp_move<0x114>(acc);  // row_shr:4
+      acc += dpp_move<0x118>(acc);  // row_shr:8
+      if (k == 15 && a < nb) Xb[a] = (T)acc;
+    } else {
+      const int lane = t & 63;
+      for (int a = t >> 6; a < nb; a += SMALL_THREADS / 64) {
+        double acc = 0;
+        for (int qq = lane; qq < nb; qq += 64) acc += (double)A.binv[(size_t)a * nb + qq] * (double)Rb[qq];
+        acc = wave_sum(acc);
+        if (lane == 0) Xb[a] = (T)acc;
+      }
     }
     __syncthreads();
   }
