@@ -574,7 +574,14 @@ __device__ __forceinline__ void strip_stage(Strip<T>& S, const Level<T>& F, int 
 // FP32 uses the hardware reciprocal (1 ulp): the smoother only needs a close
 // 1/diag, and the IEEE division is ten instructions per cell.
 __device__ __forceinline__ float smoother_inv(float d) { return __builtin_amdgcn_rcpf(d); }
-__device__ __forceinline__ double smoother_inv(double d) { return 1.0 / d; }
+// FP64: the hardware reciprocal estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - d r) (each squares the relative error): within an ulp or two of
+// 1/d in five instructions, where the IEEE division takes about twenty.
+__device__ __forceinline__ double smoother_inv(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return fma(r, fma(-d, r, 1.0), r);
+}
 template <class T>
 __device__ __forceinline__ void strip_diag(Strip<T>& S) {
 #pragma unroll
