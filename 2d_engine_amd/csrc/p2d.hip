@@ -478,6 +478,8 @@ void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
 
 // The part of a CG iteration after the V-cycle's head: the rest of the V-cycle,
 // the new direction and alpha.
+// FP32 passes use the direction kernel and finish that can end a pass one head
+// early (k_dir<float, true>, k_fin_dir<true>).
 template <class T>
 void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   vcycle_tail<T>(c, mg, v);
@@ -485,23 +487,33 @@ void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const dim3 g = dirgrid(c->nx, c->ny);
   const int kd = sizeof(T) == 4 ? KT_DIR_F32 : KT_DIR_F64;
   p2d_ctx::Track& t = c->trk;
+  const T* pold = v.p[t.iter & 1];
+  T* pnew = v.p[1 - (t.iter & 1)];
+  const int first = t.iter == t.seg0 ? 1 : 0;
   kt_begin(c, kd);
-  k_dir<T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, v.p[t.iter & 1], v.p[1 - (t.iter & 1)],
-                                       v.q, t.iter == t.seg0 ? 1 : 0, c->partial, c->s);
-  kt_end(c, kd);
-  k_fin_dir<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), c->s);
+  if (sizeof(T) == 4) {
+    k_dir<T, true><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, pold, pnew, v.q, v.r[t.rpar], first,
+                                               c->partial, c->s);
+    kt_end(c, kd);
+    k_fin_dir<true><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), (double)c->N, c->s);
+  } else {
+    k_dir<T, false><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, pold, pnew, v.q, nullptr, first,
+                                                c->partial, c->s);
+    kt_end(c, kd);
+    k_fin_dir<false><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), (double)c->N, c->s);
+  }
   t.pending = 1;  // k_fin_dir
 }
 
 // Residual launch plus its finish (sum r^2, max |r|, sum r into s->red[0..2]).
 template <class T>
 void correct_resid(p2d_ctx* c, const double* b, const double* saw, const double* sas, const double* ph,
-                   const double* pl, const T* e, double* qh, double* ql, T* r) {
+                   const double* pl, const T* e, const T* pd, double* qh, double* ql, T* r) {
   const dim3 g = fgrid(c->nx, c->ny);
   const int kr = sizeof(T) == 4 ? KT_RESID_F32 : KT_RESID_F64;
   kt_begin(c, kr);
-  k_correct_resid<T, T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, saw, sas, b, ph, pl, e, qh, ql, r,
-                                                    c->partial, c->s);
+  k_correct_resid<T, T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, saw, sas, b, ph, pl, e, pd, qh, ql,
+                                                    r, c->partial, c->s);
   kt_end(c, kr);
   k_fin_store<3><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), 2u, 0, c->s);
 }
@@ -620,7 +632,10 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
     T* rdst = c->hs->rpar ? v.r[1] : v.r[0];
     const double* oh = outer > 0 ? solh[*cur] : nullptr;  // the first pass starts from p = 0
     const double* ol = outer > 0 ? soll[*cur] : nullptr;
-    correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, oh, ol, v.x, solh[1 - *cur], soll[1 - *cur], rdst);
+    // a pass that ended at its direction's finish leaves that update pending: the
+    // refinement applies it (direction p[iter & 1], alpha in the state)
+    const T* pd = c->hs->pending ? v.p[c->hs->iter & 1] : nullptr;
+    correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, oh, ol, v.x, pd, solh[1 - *cur], soll[1 - *cur], rdst);
     *cur = 1 - *cur;
     ++outer;
     if (iters >= c->cfg.max_iterations || outer >= c->cfg.max_outer || stalled >= 1) {
@@ -701,6 +716,9 @@ __global__ void k_scal_arm(Scal* s, int iter, double beta, double alpha, int pen
     s->rz_old = 1.0;
     s->zmean = 0.0;
     s->rshift = 0.0;
+    s->r2raw = 0.0;
+    s->rsum = 0.0;
+    s->near = 0;
     s->flag = 0;
     s->outer_done = 0;
     s->rel2 = 0.0;
@@ -1581,14 +1599,13 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
   auto launch = [&]() {
     switch (which) {
       case 0:
-        k_dir<float><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, L32[0].bw, L32[0].bs, c->v32.z,
-                                                                   c->v32.p[1], c->v32.p[0], c->v32.q, 0, c->partial,
-                                                                   c->s);
+        k_dir<float, true><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
+            c->nx, c->ny, L32[0].bw, L32[0].bs, c->v32.z, c->v32.p[1], c->v32.p[0], c->v32.q, c->v32.r[0], 0, c->partial,
+            c->s);
         break;
       case 1:
-        k_dir<double><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(c->nx, c->ny, c->bw, c->bs, c->v64.z,
-                                                                    c->v64.p[1], c->v64.p[0], c->v64.q, 0, c->partial,
-                                                                    c->s);
+        k_dir<double, false><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
+            c->nx, c->ny, c->bw, c->bs, c->v64.z, c->v64.p[1], c->v64.p[0], c->v64.q, nullptr, 0, c->partial, c->s);
         break;
       case 2:
         k_mg_pre<float, true><<<tgrid(c->nx, c->ny, 2 * nu1 + 1), TBLOCK, 0, c->stream>>>(
@@ -1605,8 +1622,8 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
         break;
       default:
         k_correct_resid<float, float><<<fgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
-            c->nx, c->ny, c->bw, c->bs, nullptr, nullptr, c->b, c->ph[0], c->pl[0], c->v32.x, c->ph[1], c->pl[1],
-            c->v32.r[0], c->partial, c->s);
+            c->nx, c->ny, c->bw, c->bs, nullptr, nullptr, c->b, c->ph[0], c->pl[0], c->v32.x, nullptr, c->ph[1],
+            c->pl[1], c->v32.r[0], c->partial, c->s);
     }
   };
   k_scal_arm<<<1, 1, 0, c->stream>>>(c->s, 1, 0.25, 1e-3, 1);

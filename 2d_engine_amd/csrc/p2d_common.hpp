@@ -78,6 +78,7 @@ struct Scal {
   double outi;
   double rshift;  // mean of a replaced residual, removed by the next fused update (its constant
                   // component is roundoff that no iterate can reduce: A0 p has zero sum)
+  double r2raw, rsum;  // sum r^2 and sum r of the current recursive residual (the last head's)
   double red[8];  // results of stand-alone reductions
   int iter, done, max_iter, flag;  // flag: 1 = breakdown / NaN
   int pending;                     // 1: a direction and alpha await the update x += a p, r -= a q
@@ -85,7 +86,7 @@ struct Scal {
   int err;                         // coefficient assembly error (zero level-set gradient)
   int seg0;                        // iteration at which the current direction sequence started (beta = 0)
   int outer_done;                  // the outer targets hold after a refinement
-  int pad;
+  int near;                        // the last head's residual is within NEAR_END of the pass's target
 };
 
 // Index of the calling thread's wavefront in its block, as a wave-uniform (scalar) value:

@@ -183,7 +183,8 @@ __global__ void __launch_bounds__(BLOCK) k_divergence(int nx, int ny, double h, 
 }
 
 // --------------------------------------------------------------------------- FP64 refinement step
-// On the double-double pressure: p_new = p_old + e (e may be NULL; ph/pl NULL: p_old = 0),
+// On the double-double pressure: p_new = p_old + e (e may be NULL; ph/pl NULL: p_old = 0;
+// pd: an inner update left pending, e + alpha pd formed in TE first),
 // r = (b' - mean b') - sum_f +-G_f with G_f = beta_f fdiff(p, s a) (west and
 // south faces +, east and north -), stored as R; block partials sum r^2,
 // max |r|, sum r.  saw/sas may be NULL (no jump).  Tile kernel over FTX x FTY
@@ -201,15 +202,16 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
                                                          const double* __restrict__ saw,
                                                          const double* __restrict__ sas, const double* __restrict__ b,
                                                          const double* __restrict__ ph, const double* __restrict__ pl,
-                                                         const TE* __restrict__ e, double* __restrict__ qh,
-                                                         double* __restrict__ ql, R* __restrict__ r,
-                                                         double* partial, const Scal* s) {
+                                                         const TE* __restrict__ e, const TE* __restrict__ pd,
+                                                         double* __restrict__ qh, double* __restrict__ ql,
+                                                         R* __restrict__ r, double* partial, const Scal* s) {
 #pragma clang fp contract(off)
   // row layout: [1] west ring, [2 + 2c, 3 + 2c] pair c, [FTX + 2] east ring
   __shared__ Pair<double> sh2[FTY + 2][FTX / 2 + 2], sl2[FTY + 2][FTX / 2 + 2];
   double(*sh)[FTX + 4] = reinterpret_cast<double(*)[FTX + 4]>(&sh2[0][0]);
   double(*sl)[FTX + 4] = reinterpret_cast<double(*)[FTX + 4]>(&sl2[0][0]);
   const double bm = s->bmean;
+  const TE ta = pd ? (TE)s->alpha : (TE)0;
   const int c = threadIdx.x & 63, w = wave_id();
   int bx, by;
   xcd_tile(bx, by);
@@ -238,7 +240,12 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
       const unsigned k = cell_u(iA, jj, nx);
       const Pair<double> oh = ph ? ld_at<Pair<double>>(ph, k) : Pair<double>{0.0, 0.0};
       const Pair<double> ol = ph ? ld_at<Pair<double>>(pl, k) : Pair<double>{0.0, 0.0};
-      const Pair<TE> e2 = e ? ld_at<Pair<TE>>(e, k) : Pair<TE>{(TE)0, (TE)0};
+      Pair<TE> e2 = e ? ld_at<Pair<TE>>(e, k) : Pair<TE>{(TE)0, (TE)0};
+      if (pd) {  // the inner pass's pending update, in TE as the head applies it
+        const Pair<TE> d2 = ld_at<Pair<TE>>(pd, k);
+        e2.a += ta * d2.a;
+        e2.b += ta * d2.b;
+      }
       pnew(oh.a, ol.a, (double)e2.a, h2.a, l2.a);
       pnew(oh.b, ol.b, (double)e2.b, h2.b, l2.b);
     }
@@ -251,7 +258,8 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
     double hh = 0.0, ll = 0.0;
     if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
       const size_t q = ci(ii, jj, nx);
-      pnew(ph ? ph[q] : 0.0, ph ? pl[q] : 0.0, e ? (double)e[q] : 0.0, hh, ll);
+      const TE ev = e ? (pd ? e[q] + ta * pd[q] : e[q]) : (TE)0;
+      pnew(ph ? ph[q] : 0.0, ph ? pl[q] : 0.0, (double)ev, hh, ll);
     }
     sh[rr][side ? FTX + 2 : 1] = hh;
     sl[rr][side ? FTX + 2 : 1] = ll;

@@ -426,6 +426,7 @@ __device__ __forceinline__ void segment_start(Scal* s, double ncells) {
   s->r0_2 = fmax(s->red[0] - ncells * m * m, 0.0);
   s->r2 = s->r0_2;
   s->rinf = s->red[1];
+  s->near = 0;
 }
 // Start of a solve from the initial residual's norms in red: the reference
 // norm b2 (mean-free), the outer targets (rtol on ||r||_2 relative to b2, abs2
@@ -454,7 +455,8 @@ __global__ void k_inner_init(Scal* s, double ncells, double rel2, int inner_abs,
   s->flag = 0;
   s->pending = 0;
   s->rpar = 0;
-  s->rz = s->rz_old = s->pq = s->alpha = s->beta = s->zmean = 0.0;
+  s->rz = s->rz_old = s->pq = s->alpha = s->beta = s->zmean = s->r2raw = s->rsum = 0.0;
+  s->near = 0;
   if (!(s->red[0] == s->red[0])) {
     s->done = 1;
     s->flag = 1;
@@ -814,6 +816,12 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
   }
 }
 
+// A residual within this factor (in sum r^2) of its pass's relative target makes
+// the next direction kernel prepare the early convergence test (k_dir<T, true>):
+// with the multigrid's ~0.15 reduction per iteration, the next update then
+// usually meets the target.
+constexpr double NEAR_END = 64.0;
+
 // Finish of the fused finest pre kernel: residual buffers swap, the shift is
 // spent, and a pending update counts as an iteration with its convergence test
 // on the mean-free norm sum r^2 - (sum r)^2 / N (the recursive residual's
@@ -830,8 +838,12 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
   if (threadIdx.x != 0 || done) return;
   s->rpar = 1 - rpar;
   s->rshift = 0.0;
+  s->r2raw = a[0];
+  s->rsum = a[2];
+  s->near = 0;
   if (pending) {
     a[0] = fmax(a[0] - a[2] * a[2] / ncells, 0.0);
+    s->near = a[0] <= NEAR_END * rel2 * r0_2 ? 1 : 0;
     s->r2 = a[0];
     s->rinf = a[1];
     s->rz_old = rz;
@@ -1265,17 +1277,22 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 // --------------------------------------------------------------------------- Krylov direction kernel
 // p = (z - mean z) + beta p_old (FP64 arithmetic, stored as T; first: the first
 // direction of a sequence, p_old unused; the host picks p_old and p_new from its
-// replay of the solver state), q = A0' p, the block partials of p.q.  Tile kernel over 128 x DTY cells: lane c of a wavefront holds
-// the column pair (2c, 2c+1) of a row, so every array moves with one two-element
+// replay of the solver state), q = A0' p, the block partials of p.q.  EARLY (the
+// FP32 passes), when the state says the pass is near its end: also r.q, q.q and
+// sum q with the current residual rcur, from which k_fin_dir<true> knows the next
+// residual's norm before the update is applied.  The choice depends on the solve's
+// own state only, so a solve ends its passes in the same places whatever ran before.
+// Tile kernel over 128 x DTY cells: lane c of a wavefront holds the column pair
+// (2c, 2c+1) of a row, so every array moves with one two-element
 // load or store per lane.  p of the tile plus a one-cell ring is formed in LDS
 // (all loads issued at once), then the stencil reads LDS.  The finest level has
 // an even nx (it coarsens at least once), so a pair is inside the grid or
 // outside it as a whole.  Grid dim3(ceil(nx/128), ceil(ny/DTY)), BLOCK threads.
 constexpr int DTX = 128;  // columns per block of the direction kernel
-template <class T>
+template <class T, bool EARLY>
 __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restrict__ bw, const T* __restrict__ bs,
                                                const T* __restrict__ z, const T* pold, T* pnew, T* __restrict__ q,
-                                               int first, double* partial, Scal* s) {
+                                               const T* __restrict__ rcur, int first, double* partial, Scal* s) {
   // row layout: [1] west ring, [2 + 2c, 3 + 2c] pair c, [DTX + 2] east ring (pairs 2-element aligned)
   __shared__ Pair<T> ps2[DTY + 2][DTX / 2 + 2];
   T(*ps)[DTX + 4] = reinterpret_cast<T(*)[DTX + 4]>(&ps2[0][0]);
@@ -1320,6 +1337,13 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
       if (!first) o2[m] = ld_at<Pair<T>>(pold, k);
     }
   }
+  const int near = EARLY ? s->near : 0;
+  Pair<T> rc[RPW];
+#pragma unroll
+  for (int m = 0; m < RPW; ++m) {
+    const int j = j0 + rb + m;
+    rc[m] = (near && col_in && j < ny) ? ld_at<Pair<T>>(rcur, cell_u(iA, j, nx)) : Pair<T>{(T)0, (T)0};
+  }
   T zr = 0, orr = 0;
   const bool ring = threadIdx.x < 2 * (DTY + 2);
   const int side = (int)threadIdx.x / (DTY + 2), rr_ = (int)threadIdx.x % (DTY + 2);
@@ -1358,7 +1382,7 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     eB[m] = c == 63 ? eL[m] : e;
   }
   __syncthreads();
-  double acc[1] = {0.0};
+  double acc[EARLY ? 4 : 1] = {};
 #pragma unroll
   for (int m = 0; m < RPW; ++m) {
     const int r = rb + m, j = j0 + r;
@@ -1382,26 +1406,65 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     st_at(q, k, Pair<T>{qa, qb});
     acc[0] += (double)pa * (double)qa;
     acc[0] += (double)pb * (double)qb;
+    if constexpr (EARLY) {
+      if (near) {
+        acc[1] += (double)rc[m].a * (double)qa + (double)rc[m].b * (double)qb;
+        acc[2] += (double)qa * (double)qa + (double)qb * (double)qb;
+        acc[3] += (double)qa + (double)qb;
+      }
+    }
   }
-  block_partial<1>(acc, 0u, partial);
+  if constexpr (EARLY) {
+    if (near) {
+      block_partial<4>(acc, 0u, partial);
+      return;
+    }
+  }
+  double a1[1] = {acc[0]};
+  block_partial<1>(a1, 0u, partial);
 }
 
 // Finish of the direction kernel: alpha = rz / p.q and the update pending
-// (breakdown when p.q is not positive).
-__global__ void __launch_bounds__(FINISH_THREADS) k_fin_dir(const double* __restrict__ partial, int G, Scal* s) {
-  const int done = s->done;
-  const double rz = s->rz;
-  double a[1];
-  finish_reduce<1>(a, partial, G, 0u);
+// (breakdown when p.q is not positive).  EARLY: the residual the update will
+// leave, r' = r - alpha q, has sum r'^2 = sum r^2 - 2 alpha r.q + alpha^2 q.q and
+// sum r' = sum r - alpha sum q (FP64 sums of the FP32 vectors); when its
+// mean-free norm already meets the pass's relative target, the pass ends here, one
+// iteration on and with the update still pending (the refinement applies it), and
+// the next head is not run.
+template <bool EARLY>
+__global__ void __launch_bounds__(FINISH_THREADS) k_fin_dir(const double* __restrict__ partial, int G, double ncells,
+                                                            Scal* s) {
+  const int done = s->done, iter = s->iter, near = EARLY ? s->near : 0;
+  const double rz = s->rz, r2raw = s->r2raw, rsum = s->rsum, rel2 = s->rel2, r0_2 = s->r0_2;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  if (near) {
+    finish_reduce<4>(a, partial, G, 0u);
+  } else {
+    double b[1];
+    finish_reduce<1>(b, partial, G, 0u);
+    a[0] = b[0];
+  }
   if (threadIdx.x != 0 || done) return;
   s->pq = a[0];
   if (!(a[0] > 0.0)) {
     s->done = 1;
     s->flag = 1;
     s->alpha = 0.0;
-  } else {
-    s->alpha = rz / a[0];
-    s->pending = 1;
+    return;
+  }
+  const double alpha = rz / a[0];
+  s->alpha = alpha;
+  s->pending = 1;
+  if (near) {
+    const double r2n = r2raw - 2.0 * alpha * a[1] + alpha * alpha * a[2];
+    const double sn = rsum - alpha * a[3];
+    const double mf = fmax(r2n - sn * sn / ncells, 0.0);
+    if (mf == mf && mf <= rel2 * r0_2) {
+      s->iter = iter + 1;
+      s->r2 = mf;
+      s->rinf = sqrt(fmax(r2n, 0.0));  // an upper bound of max |r'|
+      s->done = 1;
+    }
   }
 }
 
