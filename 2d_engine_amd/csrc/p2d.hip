@@ -47,10 +47,10 @@ inline unsigned gs_blocks(size_t n) { return (unsigned)std::min<size_t>(blocks_f
 inline dim3 bgrid(int ncols, int nrows) {
   return dim3(blocks_for((size_t)ncols), (unsigned)((nrows + BAND - 1) / BAND));
 }
-// Pre/post tiles: tile_tx(H) columns x TY rows of output per block.
-inline dim3 tgrid(int nx, int ny, int H) {
+// Pre/post tiles: tile_tx(H) columns x ty rows of output per block.
+inline dim3 tgrid(int nx, int ny, int H, int ty = TY) {
   const int TX = tile_tx(H);
-  return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + TY - 1) / TY));
+  return dim3((unsigned)((nx + TX - 1) / TX), (unsigned)((ny + ty - 1) / ty));
 }
 // Direction kernel: DTX x DTY cells per block.
 inline dim3 dirgrid(int nx, int ny) {
@@ -432,11 +432,18 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const Level<T>* lv = mg.lv.data();
   const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost);
   const int kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
-  // FP32 coarse levels with more tiles than CUs run two blocks per CU (BUSY variants)
+  // Coarse levels covered by at most two 8-wavefront tiles per CU take those
+  // (Tile<8>); the others take 16-wavefront tiles, and FP32 levels with more of
+  // those than CUs run two blocks per CU (BUSY variants).
+  constexpr int TYS = Tile<8>::ty;
+  auto small = [&](int l, int H) { return count(tgrid(lv[l].nx, lv[l].ny, H, TYS)) <= 2 * (size_t)c->num_cu; };
   auto busy = [&](int l, int H) { return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H)) > (size_t)c->num_cu; };
   for (int l = 1; l < mg.l0; ++l) {
     const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpre);
-    if (busy(l, Hpre))
+    if (small(l, Hpre))
+      k_mg_pre<T, false, false, 8><<<tgrid(lv[l].nx, lv[l].ny, Hpre, TYS), Tile<8>::threads, 0, c->stream>>>(
+          lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
+    else if (busy(l, Hpre))
       k_mg_pre<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr,
                                                             nullptr, nullptr, nullptr, 0, nullptr, c->s);
     else
@@ -456,7 +463,10 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   }
   for (int l = mg.l0 - 1; l >= 1; --l) {
     const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpost);
-    if (busy(l, Hpost))
+    if (small(l, Hpost))
+      k_mg_post<T, false, false, 8><<<tgrid(lv[l].nx, lv[l].ny, Hpost, TYS), Tile<8>::threads, 0, c->stream>>>(
+          lv[l], lv[l + 1], lv[l].r, lv[l].xa, lv[l].xb, nu2, nullptr, c->s);
+    else if (busy(l, Hpost))
       k_mg_post<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, lv[l].xb, nu2,
                                                              nullptr, c->s);
     else
@@ -491,7 +501,7 @@ void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   T* pnew = v.p[1 - (t.iter & 1)];
   const int first = t.iter == t.seg0 ? 1 : 0;
   kt_begin(c, kd);
-  if (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4) {
     k_dir<T, true><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, pold, pnew, v.q, v.r[t.rpar], first,
                                                c->partial, c->s);
     kt_end(c, kd);

@@ -25,12 +25,33 @@
 
 namespace p2d {
 
-constexpr int TBLOCK = 1024;               // pre/post tile kernels: 16 wavefronts
-constexpr int TWAVES = TBLOCK / 64;
-constexpr int TY = 32;                     // output rows per pre/post tile
+constexpr int TBLOCK = 1024;               // pre/post tile kernels of the finest level: 16 wavefronts
 constexpr int MAX_NU = 3;                  // sweeps per side supported
-constexpr int RYMAX = TY + 2 * (2 * MAX_NU + 1);  // region rows at most
 constexpr int NCXMAX = 64;                 // coarse columns of a tile at most (TX / 2 < 64)
+constexpr int KR = 3;                      // consecutive region rows per thread of a tile
+
+// Tile shapes of the pre/post kernels: NW wavefronts of KR region rows each and ty
+// output rows per tile, so that a region (ty + 2H rows, H <= 2 MAX_NU + 1) fits the
+// NW KR rows.  The 16-wavefront tiles (32 rows) serve the finest level and the
+// large coarse levels.  A coarse level small enough to be covered by at most two
+// 8-wavefront tiles (10 rows) per CU takes those instead: the half-sweeps are VALU
+// issue bound per SIMD, and a block of half the wavefronts on a CU that would
+// otherwise idle halves each one.
+template <int NW>
+struct Tile {
+  static constexpr int waves = NW, threads = 64 * NW;
+  static constexpr int ty = NW == 16 ? 32 : 10;
+  static constexpr int rymax = ty + 2 * (2 * MAX_NU + 1);  // region rows at most
+  static constexpr int bnd = NW * 2 * 64;                  // one boundary buffer: [wave][bottom, top][lane]
+  static constexpr int lds_pairs = 2 * bnd;                // two buffers (also the restriction's row sums
+                                                           // and the block reduction)
+  static constexpr int pch = (ty + 4 * MAX_NU) / 2 + 3;    // staged coarse rows of the post kernel
+  static_assert(KR * NW >= rymax, "a tile region must fit the strips");
+  static_assert(ty % 2 == 0, "tiles cover whole coarse rows");
+  static_assert(2 * rymax * NCXMAX <= 2 * lds_pairs, "restriction row sums must fit the boundary buffers");
+  static_assert(3 * threads * sizeof(double) <= lds_pairs * 2 * sizeof(float), "block reduction must fit them");
+};
+constexpr int TY = Tile<16>::ty;           // output rows of the finest level's tiles
 
 // Tiles are 128 columns wide; the x halo HX is the smallest even width >= H so
 // that the region starts at an even column; TX = 128 - 2 HX interior columns.
@@ -509,12 +530,6 @@ __global__ void k_inner_continue(Scal* s, double ncells) {
 // pair and row, the same one for the whole wavefront: no lane idles on the
 // colour test.  Cells outside the grid or the region hold 0 and are never
 // updated.
-constexpr int KR = 3;                                // consecutive region rows per thread
-constexpr int BND = TWAVES * 2 * 64;                 // one boundary buffer: [wave][bottom, top][lane]
-constexpr int TILE_LDS_PAIRS = 2 * BND;              // two buffers (the restriction's row sums and the block reduction reuse them)
-static_assert(KR * TWAVES >= RYMAX, "a tile region must fit the strips");
-static_assert(2 * RYMAX * NCXMAX <= 2 * TILE_LDS_PAIRS, "restriction row sums must fit the boundary buffers");
-static_assert(3 * TBLOCK * sizeof(double) <= TILE_LDS_PAIRS * 2 * sizeof(float), "block reduction must fit them");
 
 
 // Per row m of the strip: right-hand side, faces (wA west of A, wB west of B
@@ -595,10 +610,10 @@ __device__ __forceinline__ void strip_diag(Strip<T>& S) {
 
 // Boundary rows of the neighbouring strips from buffer rd: the top row of
 // wave w - 1 (below this strip) and the bottom row of wave w + 1 (above).
-template <class T>
+template <class T, int NW>
 __device__ __forceinline__ void strip_halo(const Pair<T>* rd, int c, int w, Pair<T>& dn, Pair<T>& up) {
   dn = w > 0 ? rd[((w - 1) * 2 + 1) * 64 + c] : Pair<T>{0, 0};
-  up = w + 1 < TWAVES ? rd[(w + 1) * 2 * 64 + c] : Pair<T>{0, 0};
+  up = w + 1 < NW ? rd[(w + 1) * 2 * 64 + c] : Pair<T>{0, 0};
 }
 
 // Half-sweep k of colour `color` (cells (i + j) & 1 == color; A, the even
@@ -607,11 +622,11 @@ __device__ __forceinline__ void strip_halo(const Pair<T>* rd, int c, int w, Pair
 // colour reads only cells of the other colour, which this half-sweep leaves
 // unchanged, so the rows update in place in any order.  zero: the iterate is
 // 0 (first pre-smoothing half-sweep), no neighbour is read.
-template <class T>
+template <class T, int NW>
 __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int w, int gy0, int color, bool zero,
                                            int k) {
   Pair<T> dn{0, 0}, up{0, 0};
-  if (!zero) strip_halo<T>(bnd + ((k + 1) & 1) * BND, c, w, dn, up);
+  if (!zero) strip_halo<T, NW>(bnd + ((k + 1) & 1) * Tile<NW>::bnd, c, w, dn, up);
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
     const bool doA = ((gy0 + KR * w + m) & 1) == color;  // the same for the whole wavefront
@@ -629,14 +644,14 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
       S.xB[m] = (S.rB[m] + o) * S.iB[m];
     }
   }
-  Pair<T>* wr = bnd + (k & 1) * BND;
+  Pair<T>* wr = bnd + (k & 1) * Tile<NW>::bnd;
   wr[(w * 2) * 64 + c] = Pair<T>{S.xA[0], S.xB[0]};
   wr[(w * 2 + 1) * 64 + c] = Pair<T>{S.xA[KR - 1], S.xB[KR - 1]};
   __syncthreads();
 }
 
 // --------------------------------------------------------------------------- pre-smoothing tile kernel
-// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, H = 2 nu + 1, x halo
+// Grid dim3(ceil(nx/TX), ceil(ny/ty)), 64 NW threads (tile shape Tile<NW>), H = 2 nu + 1, x halo
 // HX = tile_hx(H) (even), TX = 128 - 2 HX.
 // FUSE (finest level): r_lvl is the current residual buffer; the update r -= a q,
 // x += a p (pend: an update is pending, p = pdir) and the residual shift are
@@ -645,11 +660,13 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
 // state, so their loads do not wait for the state.
 // Restriction r_H = P^T res: each cell splits its residual into the four
 // weight products; the x sums run across lanes, the y sums through LDS.
-template <class T, bool FUSE, bool BUSY = false>
-__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE || BUSY>())))
+template <class T, bool FUSE, bool BUSY = false, int NW = 16>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE || BUSY>())))
 k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* xsol, const T* pdir,
          const T* __restrict__ qv, int pend, double* partial, Scal* s) {
-  __shared__ Pair<T> lds[TILE_LDS_PAIRS];
+  using Sh = Tile<NW>;
+  constexpr int TY = Sh::ty, RYMAX = Sh::rymax;
+  __shared__ Pair<T> lds[Sh::lds_pairs];
   const int H = 2 * nu + 1, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
   xcd_tile(bx, by);
@@ -741,15 +758,15 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
   }
   // on the finest level the weights' loads are issued before the last
   // half-sweep, whose work covers part of their latency
-  for (int k = 0; k < 2 * nu - 1; ++k) strip_half<T>(S, lds, c, w, gy0, k & 1, k == 0, k);
+  for (int k = 0; k < 2 * nu - 1; ++k) strip_half<T, NW>(S, lds, c, w, gy0, k & 1, k == 0, k);
   if (FUSE || BUSY) load_weights();
-  strip_half<T>(S, lds, c, w, gy0, 1, false, 2 * nu - 1);
+  strip_half<T, NW>(S, lds, c, w, gy0, 1, false, 2 * nu - 1);
   // residual (reference order: r - (diag x - offsum)) with the last published
   // boundary rows (buffer (2 nu - 1) & 1), and the interior write of x
   T resA[KR], resB[KR];
   {
     Pair<T> dn, up;
-    strip_halo<T>(lds + BND, c, w, dn, up);
+    strip_halo<T, NW>(lds + Sh::bnd, c, w, dn, up);
 #pragma unroll
     for (int m = 0; m < KR; ++m) {
       const int r = KR * w + m;
@@ -793,7 +810,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
   }
   __syncthreads();
   // coarse row J gathers B(2J-1) + A(2J) + A(2J+1) + B(2J+2)
-  for (int t = threadIdx.x; t < NCX * NCY; t += TBLOCK) {
+  for (int t = threadIdx.x; t < NCX * NCY; t += Sh::threads) {
     const int a = t % NCX, b = t / NCX, I = cx0 + a, J = cy0 + b;
     if (I >= Cc.nx || J >= Cc.ny) continue;
     const int jr = 2 * b + H;  // region row of fine row 2J
@@ -812,7 +829,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
       acc[1] = fmax(acc[1], fmax(fabs(ra), fabs(rb)));
       acc[2] += ra + rb;
     }
-    block_partial_lds<3, TBLOCK>(acc, 2u, partial, reinterpret_cast<double*>(lds));
+    block_partial_lds<3, Sh::threads>(acc, 2u, partial, reinterpret_cast<double*>(lds));
   }
 }
 
@@ -857,18 +874,19 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
 }
 
 // --------------------------------------------------------------------------- post-smoothing tile kernel
-// Grid dim3(ceil(nx/TX), ceil(ny/TY)), TBLOCK threads, H = 2 nu (even): HX = H,
+// Grid dim3(ceil(nx/TX), ceil(ny/ty)), 64 NW threads (tile shape Tile<NW>), H = 2 nu (even): HX = H,
 // TX = 128 - 2H.  x = x_pre + P x_coarse (the coarse values around the tile are
 // staged in LDS while the strip's own loads are in flight), then nu (black,
 // red) sweeps; the tile interior is written to x_out.  LEVEL0: r_lvl is the
 // current residual buffer (picked by the host), x_out = z, and the block
 // partials of r.z, sum z, sum r are stored.
 constexpr int PCW = 66;                           // staged coarse columns
-constexpr int PCH = (TY + 4 * MAX_NU) / 2 + 3;    // staged coarse rows
-template <class T, bool LEVEL0, bool BUSY = false>
-__global__ void __launch_bounds__(TBLOCK) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, LEVEL0 || BUSY>())))
+template <class T, bool LEVEL0, bool BUSY = false, int NW = 16>
+__global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, LEVEL0 || BUSY>())))
 k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* x_in, T* x_out, int nu, double* partial, Scal* s) {
-  __shared__ Pair<T> lds[TILE_LDS_PAIRS];
+  using Sh = Tile<NW>;
+  constexpr int TY = Sh::ty, PCH = Sh::pch, TBLK = Sh::threads;
+  __shared__ Pair<T> lds[Sh::lds_pairs];
   __shared__ T xcs[PCH][PCW];
   const int H = 2 * nu, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
@@ -882,11 +900,11 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* x_in, T* x_out, int 
   const bool pair_int = c >= hc && c < hc + NCX;
   const int cxa = gx0 / 2 - 1, cya = (gy0 >> 1) - 1;  // first staged coarse column / row (gx0 is even)
   // Every load is issued before `done` is read (the coarse values go to LDS after the test).
-  constexpr int XCS_PER_THREAD = (PCH * PCW + TBLOCK - 1) / TBLOCK;
+  constexpr int XCS_PER_THREAD = (PCH * PCW + TBLK - 1) / TBLK;
   T xcv[XCS_PER_THREAD];
 #pragma unroll
   for (int q = 0; q < XCS_PER_THREAD; ++q) {
-    const int t = (int)threadIdx.x + q * TBLOCK;
+    const int t = (int)threadIdx.x + q * TBLK;
     const int I = min(max(cxa + t % PCW, 0), cnx - 1), J = min(max(cya + t / PCW, 0), cny - 1);
     xcv[q] = t < PCH * PCW ? ld_at<T>(Cc.xb, cell_u(I, J, cnx)) : (T)0;
   }
@@ -912,7 +930,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* x_in, T* x_out, int 
   if (s->done) return;
 #pragma unroll
   for (int q = 0; q < XCS_PER_THREAD; ++q) {
-    const int t = (int)threadIdx.x + q * TBLOCK;
+    const int t = (int)threadIdx.x + q * TBLK;
     if (t < PCH * PCW) xcs[t / PCW][t % PCW] = xcv[q];
   }
   __syncthreads();  // xcs
@@ -934,10 +952,10 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* x_in, T* x_out, int 
   }
   strip_diag<T>(S);
   // publish the boundary rows for the first half-sweep (it reads buffer 1)
-  lds[BND + (w * 2) * 64 + c] = Pair<T>{S.xA[0], S.xB[0]};
-  lds[BND + (w * 2 + 1) * 64 + c] = Pair<T>{S.xA[KR - 1], S.xB[KR - 1]};
+  lds[Sh::bnd + (w * 2) * 64 + c] = Pair<T>{S.xA[0], S.xB[0]};
+  lds[Sh::bnd + (w * 2 + 1) * 64 + c] = Pair<T>{S.xA[KR - 1], S.xB[KR - 1]};
   __syncthreads();
-  for (int k = 0; k < 2 * nu; ++k) strip_half<T>(S, lds, c, w, gy0, 1 - (k & 1), false, k);
+  for (int k = 0; k < 2 * nu; ++k) strip_half<T, NW>(S, lds, c, w, gy0, 1 - (k & 1), false, k);
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
@@ -950,7 +968,7 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* x_in, T* x_out, int 
       acc[2] += (double)S.rA[m] + (double)S.rB[m];
     }
   }
-  if (LEVEL0) block_partial_lds<3, TBLOCK>(acc, 0u, partial, reinterpret_cast<double*>(lds));
+  if (LEVEL0) block_partial_lds<3, TBLK>(acc, 0u, partial, reinterpret_cast<double*>(lds));
 }
 
 // Finish of the finest post kernel: zmean, rz = r.(z - zmean), beta.
