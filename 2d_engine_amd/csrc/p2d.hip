@@ -432,23 +432,34 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const Level<T>* lv = mg.lv.data();
   const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost);
   const int kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
-  // Coarse levels covered by at most two 8-wavefront tiles per CU take those
-  // (Tile<8>); the others take 16-wavefront tiles, and FP32 levels with more of
-  // those than CUs run two blocks per CU (BUSY variants).
-  constexpr int TYS = Tile<8>::ty;
-  auto small = [&](int l, int H) { return count(tgrid(lv[l].nx, lv[l].ny, H, TYS)) <= 2 * (size_t)c->num_cu; };
-  auto busy = [&](int l, int H) { return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H)) > (size_t)c->num_cu; };
+  // Tile shape of a coarse level: 8-wavefront tiles (10 rows) when at most two of
+  // them per CU cover it; else, when 16-wavefront tiles (32 rows) would need more
+  // than one per CU, 12-wavefront tiles (22 rows) if at most two per CU cover it
+  // (fewer wavefronts per SIMD on the busiest CUs); else 16-wavefront tiles, FP32
+  // ones two blocks per CU when they outnumber the CUs (BUSY variants).
+  const size_t ncu = (size_t)c->num_cu;
+  auto shape = [&](int l, int H) {
+    if (count(tgrid(lv[l].nx, lv[l].ny, H, Tile<8>::ty)) <= 2 * ncu) return 8;
+    if (count(tgrid(lv[l].nx, lv[l].ny, H)) > ncu && count(tgrid(lv[l].nx, lv[l].ny, H, Tile<12>::ty)) <= 2 * ncu)
+      return 12;
+    return 16;
+  };
+  auto busy = [&](int l, int H) { return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H)) > ncu; };
   for (int l = 1; l < mg.l0; ++l) {
-    const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpre);
-    if (small(l, Hpre))
-      k_mg_pre<T, false, false, 8><<<tgrid(lv[l].nx, lv[l].ny, Hpre, TYS), Tile<8>::threads, 0, c->stream>>>(
-          lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
+    const int sh = shape(l, Hpre);
+    const Level<T>& F = lv[l];
+    if (sh == 8)
+      k_mg_pre<T, false, false, 8><<<tgrid(F.nx, F.ny, Hpre, Tile<8>::ty), Tile<8>::threads, 0, c->stream>>>(
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
+    else if (sh == 12)
+      k_mg_pre<T, false, false, 12><<<tgrid(F.nx, F.ny, Hpre, Tile<12>::ty), Tile<12>::threads, 0, c->stream>>>(
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
     else if (busy(l, Hpre))
-      k_mg_pre<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr,
-                                                            nullptr, nullptr, nullptr, 0, nullptr, c->s);
+      k_mg_pre<T, false, true><<<tgrid(F.nx, F.ny, Hpre), TBLOCK, 0, c->stream>>>(
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
     else
-      k_mg_pre<T, false><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, nu1, nullptr, nullptr,
-                                                      nullptr, nullptr, 0, nullptr, c->s);
+      k_mg_pre<T, false><<<tgrid(F.nx, F.ny, Hpre), TBLOCK, 0, c->stream>>>(
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
   }
   (void)join_side(c);  // enqueue-only; a failure shows at the next launch check
   SmallArgs<T> sa{};
@@ -462,16 +473,20 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
     default: k_mg_small<T, 4><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s); break;
   }
   for (int l = mg.l0 - 1; l >= 1; --l) {
-    const dim3 g = tgrid(lv[l].nx, lv[l].ny, Hpost);
-    if (small(l, Hpost))
-      k_mg_post<T, false, false, 8><<<tgrid(lv[l].nx, lv[l].ny, Hpost, TYS), Tile<8>::threads, 0, c->stream>>>(
-          lv[l], lv[l + 1], lv[l].r, lv[l].xa, lv[l].xb, nu2, nullptr, c->s);
+    const int sh = shape(l, Hpost);
+    const Level<T>& F = lv[l];
+    if (sh == 8)
+      k_mg_post<T, false, false, 8><<<tgrid(F.nx, F.ny, Hpost, Tile<8>::ty), Tile<8>::threads, 0, c->stream>>>(
+          F, lv[l + 1], F.r, F.xa, F.xb, nu2, nullptr, c->s);
+    else if (sh == 12)
+      k_mg_post<T, false, false, 12><<<tgrid(F.nx, F.ny, Hpost, Tile<12>::ty), Tile<12>::threads, 0, c->stream>>>(
+          F, lv[l + 1], F.r, F.xa, F.xb, nu2, nullptr, c->s);
     else if (busy(l, Hpost))
-      k_mg_post<T, false, true><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, lv[l].xb, nu2,
-                                                             nullptr, c->s);
+      k_mg_post<T, false, true><<<tgrid(F.nx, F.ny, Hpost), TBLOCK, 0, c->stream>>>(F, lv[l + 1], F.r, F.xa, F.xb,
+                                                                                      nu2, nullptr, c->s);
     else
-      k_mg_post<T, false><<<g, TBLOCK, 0, c->stream>>>(lv[l], lv[l + 1], lv[l].r, lv[l].xa, lv[l].xb, nu2, nullptr,
-                                                       c->s);
+      k_mg_post<T, false><<<tgrid(F.nx, F.ny, Hpost), TBLOCK, 0, c->stream>>>(F, lv[l + 1], F.r, F.xa, F.xb, nu2,
+                                                                              nullptr, c->s);
   }
   kt_begin(c, kpost);
   k_mg_post<T, true><<<gpost, TBLOCK, 0, c->stream>>>(lv[0], lv[1], v.r[c->trk.rpar], lv[0].xa, v.z, nu2,

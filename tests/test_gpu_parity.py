@@ -140,8 +140,9 @@ def test_projection_matches_golden(P2D, W, prec):
 
 
 def test_vcycle_2048_with_two_block_coarse_level(P2D, O, W):
-    """At 2048^2 the first coarse level (1024^2, 320 tiles) has more tiles than the GPU has
-    CUs and runs the two-blocks-per-CU tile variants; the cycle still matches the oracle."""
+    """At 2048^2 the coarse levels take all three tile shapes: the first one (1024^2) 12-wavefront
+    tiles, about two per CU, and the smaller ones 8-wavefront tiles; the cycle still matches the
+    oracle."""
     w = W.rt(2048)
     print("case vcycle 2048 prec=1", flush=True)
     bx, by, jc = _coeffs(O, w)
