@@ -64,7 +64,9 @@ DOMINANT = "k_mg_pre<float,true>"
 # (72) and the divergence check (16).
 # An iteration is one full V-cycle with its direction; each pass ends with one more finest
 # pre-smoothing (its update and convergence test, 41), after which the rest of that
-# V-cycle is not launched.
+# V-cycle is not launched.  An FP32 pass that converges at its direction finish skips
+# that head (its near-end direction kernels read the residual, 4 B/cell more, and the
+# refinement the pending direction, 4 more); the count keeps the head: an upper bound.
 VCYCLE_B_PER_CELL = 41 + 25 + 24 + 46.0 / 3.0
 HEAD_B_PER_CELL = 41
 REFINE_B_PER_CELL = 64
