@@ -55,7 +55,7 @@ class P2DConfig(ctypes.Structure):
         ("mg_pre_sweeps", ctypes.c_int), ("mg_post_sweeps", ctypes.c_int),
         ("rtol", ctypes.c_double), ("div_tol", ctypes.c_double),
         ("max_iterations", ctypes.c_int), ("max_outer", ctypes.c_int),
-        ("inner_rtol", ctypes.c_double), ("abs_tol", ctypes.c_double),
+        ("inner_rtol", ctypes.c_double), ("abs_tol", ctypes.c_double), ("mg_omega", ctypes.c_double),
     ]
 
 

@@ -79,6 +79,7 @@ struct Level {
   T* xa;  // pre-smoothed correction
   T* xb;  // post-smoothed correction (the coarse correction read by the finer level)
   T* r;   // right-hand side (restricted residual)
+  T om, omc;  // smoother over-relaxation omega and 1 - omega (SOR; omega = 1: Gauss-Seidel)
 };
 
 // Interpolation weights as stored.  FP64 levels keep wx and wy in two FP64
@@ -599,12 +600,13 @@ __device__ __forceinline__ double smoother_inv(double d) {
   r = fma(r, fma(-d, r, 1.0), r);
   return fma(r, fma(-d, r, 1.0), r);
 }
+// The smoother keeps omega / diag (SOR: x <- (1 - omega) x + omega x_GS = omc x + (r + offsum) omega / diag).
 template <class T>
-__device__ __forceinline__ void strip_diag(Strip<T>& S) {
+__device__ __forceinline__ void strip_diag(Strip<T>& S, T om) {
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
-    S.iA[m] = S.v[m] ? smoother_inv(S.wA[m] + S.wB[m] + S.sA[m] + S.nA(m)) : (T)0;
-    S.iB[m] = S.v[m] ? smoother_inv(S.wB[m] + S.eB[m] + S.sB[m] + S.nB(m)) : (T)0;
+    S.iA[m] = S.v[m] ? om * smoother_inv(S.wA[m] + S.wB[m] + S.sA[m] + S.nA(m)) : (T)0;
+    S.iB[m] = S.v[m] ? om * smoother_inv(S.wB[m] + S.eB[m] + S.sB[m] + S.nB(m)) : (T)0;
   }
 }
 
@@ -621,10 +623,11 @@ __device__ __forceinline__ void strip_halo(const Pair<T>* rd, int c, int w, Pair
 // publishes the strip's boundary rows to buffer k & 1.  A cell of the swept
 // colour reads only cells of the other colour, which this half-sweep leaves
 // unchanged, so the rows update in place in any order.  zero: the iterate is
-// 0 (first pre-smoothing half-sweep), no neighbour is read.
+// 0 (first pre-smoothing half-sweep), no neighbour is read.  omc = 1 - omega
+// (the strip's 1/diag carries omega).
 template <class T, int NW>
 __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int w, int gy0, int color, bool zero,
-                                           int k) {
+                                           int k, T omc) {
   Pair<T> dn{0, 0}, up{0, 0};
   if (!zero) strip_halo<T, NW>(bnd + ((k + 1) & 1) * Tile<NW>::bnd, c, w, dn, up);
 #pragma unroll
@@ -637,11 +640,19 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
     // cells outside the grid or the region have 1/diag = 0 and right-hand side 0, and
     // their neighbour sums are finite: they stay 0 without a per-lane test
     if (doA) {
-      const T o = zero ? (T)0 : S.wA[m] * xl + S.wB[m] * S.xB[m] + S.sA[m] * dA + S.nA(m) * uA;
-      S.xA[m] = (S.rA[m] + o) * S.iA[m];
+      if (zero) {
+        S.xA[m] = S.rA[m] * S.iA[m];
+      } else {
+        const T o = S.wA[m] * xl + S.wB[m] * S.xB[m] + S.sA[m] * dA + S.nA(m) * uA;
+        S.xA[m] = (S.rA[m] + o) * S.iA[m] + omc * S.xA[m];
+      }
     } else {
-      const T o = zero ? (T)0 : S.wB[m] * S.xA[m] + S.eB[m] * xr + S.sB[m] * dB + S.nB(m) * uB;
-      S.xB[m] = (S.rB[m] + o) * S.iB[m];
+      if (zero) {
+        S.xB[m] = S.rB[m] * S.iB[m];
+      } else {
+        const T o = S.wB[m] * S.xA[m] + S.eB[m] * xr + S.sB[m] * dB + S.nB(m) * uB;
+        S.xB[m] = (S.rB[m] + o) * S.iB[m] + omc * S.xB[m];
+      }
     }
   }
   Pair<T>* wr = bnd + (k & 1) * Tile<NW>::bnd;
@@ -735,7 +746,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
   }
   if (s->done) return;
   const T ta = (T)alpha, tshift = (T)shift;
-  strip_diag<T>(S);
+  strip_diag<T>(S, F.om);
 #pragma unroll
   for (int m = 0; m < KR && FUSE; ++m) {
     const int r = KR * w + m;
@@ -758,9 +769,9 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
   }
   // on the finest level the weights' loads are issued before the last
   // half-sweep, whose work covers part of their latency
-  for (int k = 0; k < 2 * nu - 1; ++k) strip_half<T, NW>(S, lds, c, w, gy0, k & 1, k == 0, k);
+  for (int k = 0; k < 2 * nu - 1; ++k) strip_half<T, NW>(S, lds, c, w, gy0, k & 1, k == 0, k, F.omc);
   if (FUSE || BUSY) load_weights();
-  strip_half<T, NW>(S, lds, c, w, gy0, 1, false, 2 * nu - 1);
+  strip_half<T, NW>(S, lds, c, w, gy0, 1, false, 2 * nu - 1, F.omc);
   // residual (reference order: r - (diag x - offsum)) with the last published
   // boundary rows (buffer (2 nu - 1) & 1), and the interior write of x
   T resA[KR], resB[KR];
@@ -950,12 +961,12 @@ k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* x_in, T* x_out, int 
     S.xB[m] += wxb * wyb * xcs[b][a] + ((T)1 - wxb) * wyb * xcs[b][aB] + wxb * ((T)1 - wyb) * xcs[b2][a] +
                ((T)1 - wxb) * ((T)1 - wyb) * xcs[b2][aB];
   }
-  strip_diag<T>(S);
+  strip_diag<T>(S, F.om);
   // publish the boundary rows for the first half-sweep (it reads buffer 1)
   lds[Sh::bnd + (w * 2) * 64 + c] = Pair<T>{S.xA[0], S.xB[0]};
   lds[Sh::bnd + (w * 2 + 1) * 64 + c] = Pair<T>{S.xA[KR - 1], S.xB[KR - 1]};
   __syncthreads();
-  for (int k = 0; k < 2 * nu; ++k) strip_half<T, NW>(S, lds, c, w, gy0, 1 - (k & 1), false, k);
+  for (int k = 0; k < 2 * nu; ++k) strip_half<T, NW>(S, lds, c, w, gy0, 1 - (k & 1), false, k, F.omc);
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int m = 0; m < KR; ++m) {
@@ -1153,8 +1164,10 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
       if (t + u * SMALL_THREADS < nb) sm[L.r[0] + t + u * SMALL_THREADS] = rr[u];
   }
   if (done) return;
+  // omega / diag (every level of the hierarchy shares omega)
+  const T om = A.lv[0].om, omc = A.lv[0].omc;
 #pragma unroll
-  for (int sl = 0; sl < SMALL_SLOTS; ++sl) id[sl] = cd[sl] != (T)0 ? (T)1 / cd[sl] : (T)0;
+  for (int sl = 0; sl < SMALL_SLOTS; ++sl) id[sl] = cd[sl] != (T)0 ? om / cd[sl] : (T)0;
   __syncthreads();
   // one red-black half-sweep of level q (cells (i + j) & 1 == color)
   auto half = [&](int q, int color, bool zero) {
@@ -1164,11 +1177,13 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     for (int u = 0; u < small_cpt(q); ++u) {
       const int sl = small_slot(q, u), e = t + u * SMALL_THREADS, i = ci_[sl], j = cj_[sl];
       if (e < n && ((i + j) & 1) == color) {
-        T o = 0;
-        if (!zero)
-          o = cw[sl] * X[i > 0 ? e - 1 : e] + ce[sl] * X[i + 1 < nx ? e + 1 : e] + cs[sl] * X[j > 0 ? e - nx : e] +
-              cn[sl] * X[j + 1 < ny ? e + nx : e];
-        X[e] = (rr[sl] + o) * id[sl];
+        if (zero) {
+          X[e] = rr[sl] * id[sl];
+        } else {
+          const T o = cw[sl] * X[i > 0 ? e - 1 : e] + ce[sl] * X[i + 1 < nx ? e + 1 : e] +
+                      cs[sl] * X[j > 0 ? e - nx : e] + cn[sl] * X[j + 1 < ny ? e + nx : e];
+          X[e] = (rr[sl] + o) * id[sl] + omc * X[e];
+        }
       }
     }
     __syncthreads();

@@ -32,7 +32,9 @@
 //                                    absolute rule and the off-by-one cap
 //   CG / IC(0)-PCG / Jacobi-PCG /    agreement on the mean-zero pressure (the
 //   MG-PCG                           system fixes it)
-//   multigrid V-cycle                symmetry, positivity, linearity, spectrum
+//   multigrid V-cycle                symmetry, positivity, linearity, spectrum; the
+//                                    smoother (SOR omega included) against a dense
+//                                    textbook red-black sweep around the nu = 0 cycle
 //   consistent update, adjoint       D u_proj = -dt r identity, central differences,
 //                                    idempotence and self-adjointness
 //   reference-literal update         closed forms: linear pressure (wall faces against
@@ -761,7 +763,9 @@ void orc_update_reference_literal(int nx, int ny, double h, double dt, const dou
 // bilinear P with weights (9,3,3,1)/16.
 // Mode bit 2 (mode 7, the GPU's FP32 hierarchy) stores every weight rounded to binary16
 // through binary32 (orc_half_via_float), as the FP32 levels do; R = P^T still holds.
-// Smoother: red-black Gauss-Seidel, pre (red, black) x nu, post (black, red) x nu.
+// Smoother: red-black Gauss-Seidel over-relaxed by omega (SOR: x <- (1 - omega) x + omega x_GS,
+// omega = 1 plain Gauss-Seidel), pre (red, black) x nu, post (black, red) x nu; the reverse
+// colour order after the coarse correction keeps M symmetric for every omega in (0, 2).
 // Bottom: pseudo-inverse of the coarsest operator (constants in its null space), so the
 // V-cycle treats the constant mode exactly as the singular fine operator does.
 
@@ -775,6 +779,7 @@ struct MG {
   std::vector<MGLevel> L;
   int nu_pre, nu_post;
   int mode;                        // bit0: harmonic-normal coarse beta; bit1: operator-dependent P; bit2: binary16 weights
+  double omega = 1.0;              // smoother over-relaxation
   std::vector<double> bottom_inv;  // dense inverse of the coarsest operator
 };
 
@@ -791,11 +796,13 @@ static double offsum(const MGLevel& l, const std::vector<double>& x, int i, int 
   if (j + 1 < ny) s += l.by[FY(i, j + 1, nx)] * x[C(i, j + 1, nx)];
   return s;
 }
-static void rb_sweep(MGLevel& l, int color) {
+static void rb_sweep(MGLevel& l, int color, double omega) {
   for (int j = 0; j < l.ny; ++j)
     for (int i = 0; i < l.nx; ++i) {
       if (((i + j) & 1) != color) continue;
-      l.x[C(i, j, l.nx)] = (l.r[C(i, j, l.nx)] + offsum(l, l.x, i, j)) / diag_at(l, i, j);
+      double& x = l.x[C(i, j, l.nx)];
+      const double gs = (l.r[C(i, j, l.nx)] + offsum(l, l.x, i, j)) / diag_at(l, i, j);
+      x = omega == 1.0 ? gs : omega * gs + (1.0 - omega) * x;
     }
 }
 static void residual(MGLevel& l) {
@@ -950,11 +957,13 @@ static double coarse_face(const double* b, int stride, int k, int nf) {
 }
 
 void* orc_mg_create(int nx, int ny, const double* bx, const double* by, int max_levels, int bottom_max, int nu_pre,
-                    int nu_post, int mode) {
+                    int nu_post, int mode, double omega) {
+  if (nu_pre < 0 || nu_post < 0 || !(omega > 0.0 && omega < 2.0)) return nullptr;
   MG* mg = new MG;
   mg->nu_pre = nu_pre;
   mg->nu_post = nu_post;
   mg->mode = mode;
+  mg->omega = omega;
   const int nl = orc_mg_num_levels(nx, ny, max_levels, bottom_max);
   mg->L.resize(nl);
   for (int l = 0; l < nl; ++l) {
@@ -1033,13 +1042,13 @@ static void vcycle(MG* mg, int l) {
     return;
   }
   std::fill(lv.x.begin(), lv.x.end(), 0.0);
-  for (int s = 0; s < mg->nu_pre; ++s) { rb_sweep(lv, 0); rb_sweep(lv, 1); }
+  for (int s = 0; s < mg->nu_pre; ++s) { rb_sweep(lv, 0, mg->omega); rb_sweep(lv, 1, mg->omega); }
   residual(lv);
   MGLevel& cv = mg->L[l + 1];
   restrict_(lv, cv);
   vcycle(mg, l + 1);
   prolong_add(lv, cv);
-  for (int s = 0; s < mg->nu_post; ++s) { rb_sweep(lv, 1); rb_sweep(lv, 0); }
+  for (int s = 0; s < mg->nu_post; ++s) { rb_sweep(lv, 1, mg->omega); rb_sweep(lv, 0, mg->omega); }
 }
 
 // z = M r_scaled: one V-cycle on A' x = r' (r' = h^2 r for the fine operator).
@@ -1055,10 +1064,11 @@ void orc_mg_vcycle(void* h, const double* rs, double* z) {
 // x updated in place, stops when ||r||_2 <= rtol ||b||_2 and ||r||_inf <= atol
 // (either test disabled by a non-positive tolerance).  Returns iterations.
 int orc_mg_pcg(int nx, int ny, double h, const double* bx, const double* by, double* x, const double* b, double rtol,
-               double atol, int max_iteration, int max_levels, int nu, int mode, double* out_res2, double* out_resinf) {
+               double atol, int max_iteration, int max_levels, int nu, int mode, double omega, double* out_res2,
+               double* out_resinf) {
   const int N = nx * ny;
   std::vector<double> bxs(bx, bx + (size_t)(nx + 1) * ny), bys(by, by + (size_t)nx * (ny + 1));
-  void* mg = orc_mg_create(nx, ny, bxs.data(), bys.data(), max_levels, 64, nu, nu, mode);
+  void* mg = orc_mg_create(nx, ny, bxs.data(), bys.data(), max_levels, 64, nu, nu, mode, omega);
   if (!mg) return -1;
   std::vector<double> r(N), z(N), p(N), q(N), rs(N), bm(b, b + N);
   // A0 is singular (constants); solve with the range component of b, i.e. b minus its mean.
@@ -1126,8 +1136,8 @@ int orc_project(int nx, int ny, double h, double dt, const double* phi, double r
   if (solver == 3) {
     std::vector<double> b(N);
     orc_rhs_consistent(nx, ny, h, dt, phi, jc.data(), bx.data(), by.data(), u, v, b.data());
-    iters = orc_mg_pcg(nx, ny, h, bx.data(), by.data(), x.data(), b.data(), tol, 0.0, max_iteration, 0, 2, MG_DEFAULT_MODE, &res,
-                       nullptr);
+    iters = orc_mg_pcg(nx, ny, h, bx.data(), by.data(), x.data(), b.data(), tol, 0.0, max_iteration, 0, 2, MG_DEFAULT_MODE,
+                       1.0, &res, nullptr);
     const double x0 = x[0];
     for (size_t k = 0; k < N; ++k) x[k] -= x0;  // report with the reference's pin p(0,0) = 0
   } else {
@@ -1170,7 +1180,8 @@ int orc_adjoint(int nx, int ny, double h, double dt, const double* phi, const do
       auto gyf = [&](int jj) { return (jj > 0 && jj < ny) ? by[FY(i, jj, nx)] * gv[FY(i, jj, nx)] : 0.0; };
       rhs[C(i, j, nx)] = -((gxf(i + 1) - gxf(i)) * ih + (gyf(j + 1) - gyf(j)) * ih);
     }
-  const int it = orc_mg_pcg(nx, ny, h, bx, by, mu.data(), rhs.data(), rtol, 0.0, 100000, 0, 2, MG_DEFAULT_MODE, nullptr, nullptr);
+  const int it =
+      orc_mg_pcg(nx, ny, h, bx, by, mu.data(), rhs.data(), rtol, 0.0, 100000, 0, 2, MG_DEFAULT_MODE, 1.0, nullptr, nullptr);
   if (it < 0) return -1;
   jump_faces(nx, ny, h, phi, jc, Jx, Jy);
   for (int j = 0; j < ny; ++j)
@@ -1202,7 +1213,8 @@ int orc_project_beta(int nx, int ny, double h, double dt, const double* phi, con
   const size_t N = (size_t)nx * ny;
   std::vector<double> b(N), x(N, 0.0);
   orc_rhs_consistent(nx, ny, h, dt, phi, jc, bx, by, u, v, b.data());
-  const int it = orc_mg_pcg(nx, ny, h, bx, by, x.data(), b.data(), rtol, 0.0, 100000, 0, 2, MG_DEFAULT_MODE, nullptr, nullptr);
+  const int it =
+      orc_mg_pcg(nx, ny, h, bx, by, x.data(), b.data(), rtol, 0.0, 100000, 0, 2, MG_DEFAULT_MODE, 1.0, nullptr, nullptr);
   if (it < 0) return -1;
   double m = 0;
   for (size_t k = 0; k < N; ++k) m += x[k];

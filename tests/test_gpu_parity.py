@@ -49,15 +49,17 @@ def test_stencil_matches_oracle(P2D, O, W, prec):
         assert np.abs(y - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("prec,nu", [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3)])
-def test_vcycle_matches_oracle(P2D, O, W, prec, nu):
+# omega: the smoother's over-relaxation (1: Gauss-Seidel)
+@pytest.mark.parametrize("prec,nu,omega", [(0, 1, 1.0), (0, 2, 1.0), (0, 3, 1.0), (1, 2, 1.0), (1, 3, 1.0),
+                                           (0, 1, 1.4), (0, 3, 1.4), (1, 3, 1.4), (0, 3, 1.7), (1, 2, 0.8)])
+def test_vcycle_matches_oracle(P2D, O, W, prec, nu, omega):
     w = W.make("drops64", 256)
-    print(f"case vcycle prec={prec} nu={nu}", flush=True)
+    print(f"case vcycle prec={prec} nu={nu} omega={omega}", flush=True)
     bx, by, jc = _coeffs(O, w)
-    ctx = P2D.Context(w.n, w.n, w.h, mg_pre_sweeps=nu, mg_post_sweeps=nu)
+    ctx = P2D.Context(w.n, w.n, w.h, mg_pre_sweeps=nu, mg_post_sweeps=nu, mg_omega=omega)
     ctx.set_coefficients(bx, by, w.phi, jc)
     # the FP32 hierarchy stores its weights in binary16: its oracle is mode 7
-    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32)
+    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32, omega=omega)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(6).standard_normal((w.n, w.n))
     r -= r.mean()
@@ -73,17 +75,18 @@ def test_vcycle_matches_oracle(P2D, O, W, prec, nu):
 # Rectangular grids: hierarchies whose single-workgroup cycle has three levels
 # (192x128, 160x96), two levels over a 135-cell bottom whose pseudo-inverse
 # stays in global memory (120x72), and only the 576-cell bottom (128x18).
-@pytest.mark.parametrize("nx,ny,prec", [(192, 128, 0), (160, 96, 0), (160, 96, 1), (120, 72, 0), (120, 72, 1),
-                                        (128, 18, 0)])
-def test_vcycle_rectangular_matches_oracle(P2D, O, nx, ny, prec):
-    print(f"case vcycle rectangular {nx}x{ny} prec={prec}", flush=True)
+@pytest.mark.parametrize("nx,ny,prec,omega", [(192, 128, 0, 1.0), (160, 96, 0, 1.0), (160, 96, 1, 1.0),
+                                              (120, 72, 0, 1.0), (120, 72, 1, 1.0), (128, 18, 0, 1.0),
+                                              (160, 96, 0, 1.4), (160, 96, 1, 1.4), (120, 72, 0, 1.4)])
+def test_vcycle_rectangular_matches_oracle(P2D, O, nx, ny, prec, omega):
+    print(f"case vcycle rectangular {nx}x{ny} prec={prec} omega={omega}", flush=True)
     h = 1.0 / nx
     X, Y = np.meshgrid((np.arange(nx) + 0.5) * h, (np.arange(ny) + 0.5) * h)
     phi = np.hypot(X - 0.4, Y - 0.45 * ny / nx) - 0.3 * ny / nx
     bx, by = O.face_beta(phi, 1.0, 1000.0)
-    ctx = P2D.Context(nx, ny, h, mg_pre_sweeps=2, mg_post_sweeps=2)
+    ctx = P2D.Context(nx, ny, h, mg_pre_sweeps=2, mg_post_sweeps=2, mg_omega=omega)
     ctx.set_coefficients(bx, by, phi, np.zeros_like(phi))
-    mg = O.MG(bx, by, nu_pre=2, nu_post=2, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32)
+    mg = O.MG(bx, by, nu_pre=2, nu_post=2, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32, omega=omega)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(7).standard_normal((ny, nx))
     r -= r.mean()
@@ -141,15 +144,17 @@ def test_projection_matches_golden(P2D, W, prec):
 
 def test_vcycle_2048_with_two_block_coarse_level(P2D, O, W):
     """At 2048^2 the coarse levels take all three tile shapes: the first one (1024^2) 12-wavefront
-    tiles, about two per CU, and the smaller ones 8-wavefront tiles; the cycle still matches the
-    oracle."""
+    tiles, about two per CU, and the smaller ones 8-wavefront tiles; the cycle (with the default
+    over-relaxation) still matches the oracle."""
     w = W.rt(2048)
     print("case vcycle 2048 prec=1", flush=True)
     bx, by, jc = _coeffs(O, w)
     ctx = P2D.Context(w.n, w.n, w.h)
     ctx.set_coefficients(bx, by, w.phi, jc)
-    mg = O.MG(bx, by, nu_pre=3, nu_post=3)
-    mg16 = O.MG(bx, by, nu_pre=3, nu_post=3, mode=O.MG_MODE_F32)
+    om = ctx.cfg.mg_omega
+    assert om == 1.4
+    mg = O.MG(bx, by, nu_pre=3, nu_post=3, omega=om)
+    mg16 = O.MG(bx, by, nu_pre=3, nu_post=3, mode=O.MG_MODE_F32, omega=om)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(8).standard_normal((w.n, w.n))
     r -= r.mean()

@@ -70,8 +70,11 @@ def _dense_A_scaled(bx, by):
     return A
 
 
-def test_vcycle_symmetric_positive_linear(O, mg_case):
+@pytest.mark.parametrize("omega", [1.0, 1.4])
+def test_vcycle_symmetric_positive_linear(O, mg_case, omega):
     w, bx, by, mg = mg_case
+    if omega != 1.0:
+        mg = O.MG(bx, by, omega=omega)
     rng = np.random.default_rng(1)
     x, y = rng.standard_normal((2, w.n, w.n))
     x -= x.mean()
@@ -80,6 +83,60 @@ def test_vcycle_symmetric_positive_linear(O, mg_case):
     assert abs(np.vdot(Mx, y) - np.vdot(x, My)) <= 1e-12 * abs(np.vdot(Mx, y))
     assert np.vdot(x, Mx) > 0 and np.vdot(y, My) > 0
     assert np.linalg.norm(mg.vcycle(2 * x - 3 * y) - (2 * Mx - 3 * My)) <= 1e-12 * np.linalg.norm(Mx)
+
+
+@pytest.mark.parametrize("nu", [1, 2, 3])
+@pytest.mark.parametrize("omega", [1.0, 1.4])
+def test_smoother_is_textbook_sor(O, W, nu, omega):
+    """The smoother, pinned against a dense textbook restatement: on a two-level hierarchy
+    (16^2 over an 8^2 bottom) the coarse correction of the nu-sweep cycle is the nu = 0
+    cycle M0 = P A_c^+ P^T, so the oracle's cycle must equal nu red-black SOR half-sweeps
+    (red, black) from x = 0, x += M0 (r - A x), then nu (black, red) half-sweeps, where a
+    half-sweep sets every cell c of its colour to (1 - omega) x_c + omega (r_c + sum_f beta_f
+    x_nb) / diag_c (the colours do not couple within a half-sweep)."""
+    w = W.circle(16)
+    bx, by = O.face_beta(w.phi, w.rho_minus, w.rho_plus)
+    A = _dense_A_scaled(bx, by)
+    n = A.shape[0]
+    m0 = O.MG(bx, by, nu_pre=0, nu_post=0)
+    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu, omega=omega)
+    assert m0.levels == 2 and mg.levels == 2
+    M0 = np.stack([m0.vcycle(np.eye(n)[k].reshape(16, 16)).ravel() for k in range(n)], axis=1)
+    jj, ii = np.divmod(np.arange(n), 16)
+    colour = (ii + jj) & 1
+    d = np.diag(A)
+
+    def half(x, r, c):
+        gs = (r - (A @ x - d * x)) / d
+        return np.where(colour == c, (1.0 - omega) * x + omega * gs, x)
+
+    def cycle(r):
+        x = np.zeros(n)
+        for _ in range(nu):
+            x = half(half(x, r, 0), r, 1)
+        x = x + M0 @ (r - A @ x)
+        for _ in range(nu):
+            x = half(half(x, r, 1), r, 0)
+        return x
+
+    rng = np.random.default_rng(5)
+    for _ in range(3):
+        r = rng.standard_normal(n)
+        ref = cycle(r)
+        got = mg.vcycle(r.reshape(16, 16)).ravel()
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_sor_cuts_iterations(O, W):
+    """Over-relaxation (omega = 1.4) needs fewer MG-PCG iterations than Gauss-Seidel on the
+    1000:1 workloads (a regression bound, not a pin)."""
+    for name, n in [("rt", 256), ("drops64", 256)]:
+        w = W.make(name, n)
+        bx, by = O.face_beta(w.phi, w.rho_minus, w.rho_plus)
+        b = -O.divergence(w.u, w.v, w.h)
+        _, it1, _, _ = O.mg_pcg(bx, by, b, w.h, rtol=1e-12, nu=3)
+        _, it14, _, _ = O.mg_pcg(bx, by, b, w.h, rtol=1e-12, nu=3, omega=1.4)
+        assert it14 < it1, (name, it1, it14)
 
 
 def test_half_rounding_matches_numpy(O):
