@@ -56,6 +56,7 @@ class P2DConfig(ctypes.Structure):
         ("rtol", ctypes.c_double), ("div_tol", ctypes.c_double),
         ("max_iterations", ctypes.c_int), ("max_outer", ctypes.c_int),
         ("inner_rtol", ctypes.c_double), ("abs_tol", ctypes.c_double), ("mg_omega", ctypes.c_double),
+        ("mg_omega_coarse", ctypes.c_double),
     ]
 
 

@@ -340,8 +340,9 @@ int alloc_mg(p2d_ctx* c, MG<T>& mg, const T* bw0, const T* bs0) {
     d.xa = xa;
     d.xb = xb;
     d.r = r;
-    d.om = (T)c->cfg.mg_omega;
-    d.omc = (T)(1.0 - c->cfg.mg_omega);
+    const double om = l == 0 ? c->cfg.mg_omega : c->cfg.mg_omega_coarse;  // SOR over-relaxation of the level
+    d.om = (T)om;
+    d.omc = (T)(1.0 - om);
   }
   return P2D_OK;
 }
@@ -855,6 +856,7 @@ void p2d_config_default(p2d_config* cfg, int nx, int ny, double h) {
   cfg->inner_rtol = 1e-4;
   cfg->abs_tol = 0.0;
   cfg->mg_omega = 1.4;
+  cfg->mg_omega_coarse = 1.5;
 }
 
 const char* p2d_status_string(int st) {
@@ -876,7 +878,8 @@ int p2d_create(const p2d_config* cfg, p2d_ctx** out) {
       cfg->mg_pre_sweeps > MAX_NU || cfg->mg_post_sweeps < 1 || cfg->mg_post_sweeps > MAX_NU ||
       cfg->max_iterations < 1 || cfg->max_outer < 1 || (cfg->precision != P2D_FP64 && cfg->precision != P2D_MIXED) ||
       (!(cfg->rtol > 0) && !(cfg->div_tol > 0) && !(cfg->abs_tol > 0)) || !(cfg->abs_tol >= 0) ||
-      !(cfg->inner_rtol > 0 && cfg->inner_rtol < 1) || !(cfg->mg_omega > 0 && cfg->mg_omega < 2))
+      !(cfg->inner_rtol > 0 && cfg->inner_rtol < 1) || !(cfg->mg_omega > 0 && cfg->mg_omega < 2) ||
+      !(cfg->mg_omega_coarse > 0 && cfg->mg_omega_coarse < 2))
     return P2D_BAD_ARG;
   // The tile kernels address a level's arrays with 32-bit byte offsets (ld_at / st_at):
   // the largest array, nx*ny doubles, must stay below 2^32 bytes.

@@ -79,7 +79,7 @@ struct Level {
   T* xa;  // pre-smoothed correction
   T* xb;  // post-smoothed correction (the coarse correction read by the finer level)
   T* r;   // right-hand side (restricted residual)
-  T om, omc;  // smoother over-relaxation omega and 1 - omega (SOR; omega = 1: Gauss-Seidel)
+  T om, omc;  // the level's smoother over-relaxation omega and 1 - omega (SOR; omega = 1: Gauss-Seidel)
 };
 
 // Interpolation weights as stored.  FP64 levels keep wx and wy in two FP64
@@ -1164,7 +1164,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
       if (t + u * SMALL_THREADS < nb) sm[L.r[0] + t + u * SMALL_THREADS] = rr[u];
   }
   if (done) return;
-  // omega / diag (every level of the hierarchy shares omega)
+  // omega / diag (the levels below the finest share omega)
   const T om = A.lv[0].om, omc = A.lv[0].omc;
 #pragma unroll
   for (int sl = 0; sl < SMALL_SLOTS; ++sl) id[sl] = cd[sl] != (T)0 ? om / cd[sl] : (T)0;

@@ -76,8 +76,9 @@ typedef struct p2d_config {
   double abs_tol;      /* ||r||_2 <= abs_tol (0 disables, the default): PROJECTION_2D's `tolerance`
                         * with the reference's absolute rule, res < tolerance^2 on r.r
                         * (CG_METHOD.cpp:52; its IC(0)-PCG tests r.z, PCG_METHOD.cpp:253) */
-  double mg_omega;     /* over-relaxation of the red-black smoother, 0 < omega < 2 (SOR; 1 = Gauss-Seidel;
-                        * default 1.4).  The V-cycle stays symmetric for every omega */
+  double mg_omega;     /* over-relaxation of the red-black smoother on the finest level, 0 < omega < 2
+                        * (SOR; 1 = Gauss-Seidel; default 1.4).  The V-cycle stays symmetric for every omega */
+  double mg_omega_coarse; /* ... on the coarser levels (default 1.5) */
 } p2d_config;
 
 typedef struct p2d_stats {

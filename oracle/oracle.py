@@ -71,11 +71,12 @@ def lib():
             "orc_update_reference_literal": (None, [_i, _i, _d, _d, _pd, _pd, _d, _d, _pd, _pd, _pd]),
             "orc_half_via_float": (_d, [_d]),
             "orc_mg_num_levels": (_i, [_i, _i, _i, _i]),
-            "orc_mg_create": (_vp, [_i, _i, _pd, _pd, _i, _i, _i, _i, _i, _d]),
+            "orc_mg_create": (_vp, [_i, _i, _pd, _pd, _i, _i, _i, _i, _i, _d, _d]),
+            "orc_mg_piece": (_i, [_vp, _i, _i, _pd, _pd]),
             "orc_mg_destroy": (None, [_vp]),
             "orc_mg_levels": (_i, [_vp]),
             "orc_mg_vcycle": (None, [_vp, _pd, _pd]),
-            "orc_mg_pcg": (_i, [_i, _i, _d, _pd, _pd, _pd, _pd, _d, _d, _i, _i, _i, _i, _d, _pd, _pd]),
+            "orc_mg_pcg": (_i, [_i, _i, _d, _pd, _pd, _pd, _pd, _d, _d, _i, _i, _i, _i, _d, _d, _pd, _pd]),
             "orc_project": (_i, [_i, _i, _d, _d, _pd, _d, _d, _d, _pd, _pd, _i, _d, _i, _i, _pd, _pd]),
             "orc_adjoint": (_i, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _d, _pd, _pd, _pd, _pd, _pd]),
             "orc_project_beta": (_i, [_i, _i, _d, _d, _pd, _pd, _pd, _pd, _pd, _pd, _d, _pd]),
@@ -402,14 +403,17 @@ def half_via_float(w):
 
 
 class MG:
-    """The V-cycle preconditioner (orc_mg_create): omega over-relaxes the red-black smoother."""
+    """The V-cycle preconditioner (orc_mg_create): omega over-relaxes the red-black smoother on
+    the finest level, omega_coarse (default: omega) on the levels below it."""
 
-    def __init__(self, bx, by, max_levels=0, bottom_max=64, nu_pre=2, nu_post=2, mode=MG_DEFAULT_MODE, omega=1.0):
+    def __init__(self, bx, by, max_levels=0, bottom_max=64, nu_pre=2, nu_post=2, mode=MG_DEFAULT_MODE, omega=1.0,
+                 omega_coarse=None):
         self.bx, self.by = _f64(bx), _f64(by)
         ny, nx1 = self.bx.shape
         self.nx, self.ny = nx1 - 1, ny
+        omc = omega if omega_coarse is None else omega_coarse
         self.h = lib().orc_mg_create(self.nx, self.ny, _p(self.bx), _p(self.by), max_levels, bottom_max,
-                                     nu_pre, nu_post, mode, float(omega))
+                                     nu_pre, nu_post, mode, float(omega), float(omc))
         if not self.h:
             raise RuntimeError("invalid sweeps or omega, or a singular bottom operator")
 
@@ -423,6 +427,14 @@ class MG:
         lib().orc_mg_vcycle(self.h, _p(rs), _p(z))
         return z
 
+    def piece(self, what, level, x, n_out):
+        """orc_mg_piece: 0 A'_level x, 1 P_level x, 2 P_level^T x, 3 bottom pseudo-inverse x (flat arrays)."""
+        x = _f64(np.ravel(x))
+        y = np.empty(n_out)
+        if lib().orc_mg_piece(self.h, what, level, _p(x), _p(y)) != 0:
+            raise ValueError("invalid level")
+        return y
+
     def __del__(self):
         if getattr(self, "h", None):
             lib().orc_mg_destroy(self.h)
@@ -434,14 +446,15 @@ def mg_num_levels(nx, ny, max_levels=0, bottom_max=64):
 
 
 def mg_pcg(bx, by, b, h, x0=None, rtol=1e-10, atol=0.0, max_iteration=10000, max_levels=0, nu=2, mode=None,
-           omega=1.0):
+           omega=1.0, omega_coarse=None):
     mode = MG_DEFAULT_MODE if mode is None else mode
     b = _f64(b)
     ny, nx = b.shape
     x = np.zeros_like(b) if x0 is None else _f64(x0).copy()
     r2, rinf = _d(), _d()
     it = lib().orc_mg_pcg(nx, ny, h, _p(_f64(bx)), _p(_f64(by)), _p(x), _p(b), rtol, atol, max_iteration,
-                          max_levels, nu, mode, float(omega), ctypes.byref(r2), ctypes.byref(rinf))
+                          max_levels, nu, mode, float(omega), float(omega if omega_coarse is None else omega_coarse),
+                          ctypes.byref(r2), ctypes.byref(rinf))
     if it < 0:
         raise RuntimeError("MG setup failed")
     return x, it, r2.value, rinf.value

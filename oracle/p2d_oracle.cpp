@@ -763,9 +763,10 @@ void orc_update_reference_literal(int nx, int ny, double h, double dt, const dou
 // bilinear P with weights (9,3,3,1)/16.
 // Mode bit 2 (mode 7, the GPU's FP32 hierarchy) stores every weight rounded to binary16
 // through binary32 (orc_half_via_float), as the FP32 levels do; R = P^T still holds.
-// Smoother: red-black Gauss-Seidel over-relaxed by omega (SOR: x <- (1 - omega) x + omega x_GS,
-// omega = 1 plain Gauss-Seidel), pre (red, black) x nu, post (black, red) x nu; the reverse
-// colour order after the coarse correction keeps M symmetric for every omega in (0, 2).
+// Smoother: red-black Gauss-Seidel over-relaxed by omega on the finest level and by
+// omega_coarse on the levels below it (SOR: x <- (1 - w) x + w x_GS, w = 1 plain Gauss-Seidel),
+// pre (red, black) x nu, post (black, red) x nu; the reverse colour order after the coarse
+// correction keeps M symmetric for every w in (0, 2).
 // Bottom: pseudo-inverse of the coarsest operator (constants in its null space), so the
 // V-cycle treats the constant mode exactly as the singular fine operator does.
 
@@ -779,7 +780,8 @@ struct MG {
   std::vector<MGLevel> L;
   int nu_pre, nu_post;
   int mode;                        // bit0: harmonic-normal coarse beta; bit1: operator-dependent P; bit2: binary16 weights
-  double omega = 1.0;              // smoother over-relaxation
+  double omega = 1.0;              // smoother over-relaxation on the finest level
+  double omega_coarse = 1.0;       // ... on the coarser levels
   std::vector<double> bottom_inv;  // dense inverse of the coarsest operator
 };
 
@@ -957,13 +959,15 @@ static double coarse_face(const double* b, int stride, int k, int nf) {
 }
 
 void* orc_mg_create(int nx, int ny, const double* bx, const double* by, int max_levels, int bottom_max, int nu_pre,
-                    int nu_post, int mode, double omega) {
-  if (nu_pre < 0 || nu_post < 0 || !(omega > 0.0 && omega < 2.0)) return nullptr;
+                    int nu_post, int mode, double omega, double omega_coarse) {
+  if (nu_pre < 0 || nu_post < 0 || !(omega > 0.0 && omega < 2.0) || !(omega_coarse > 0.0 && omega_coarse < 2.0))
+    return nullptr;
   MG* mg = new MG;
   mg->nu_pre = nu_pre;
   mg->nu_post = nu_post;
   mg->mode = mode;
   mg->omega = omega;
+  mg->omega_coarse = omega_coarse;
   const int nl = orc_mg_num_levels(nx, ny, max_levels, bottom_max);
   mg->L.resize(nl);
   for (int l = 0; l < nl; ++l) {
@@ -1042,13 +1046,65 @@ static void vcycle(MG* mg, int l) {
     return;
   }
   std::fill(lv.x.begin(), lv.x.end(), 0.0);
-  for (int s = 0; s < mg->nu_pre; ++s) { rb_sweep(lv, 0, mg->omega); rb_sweep(lv, 1, mg->omega); }
+  const double om = l == 0 ? mg->omega : mg->omega_coarse;
+  for (int s = 0; s < mg->nu_pre; ++s) { rb_sweep(lv, 0, om); rb_sweep(lv, 1, om); }
   residual(lv);
   MGLevel& cv = mg->L[l + 1];
   restrict_(lv, cv);
   vcycle(mg, l + 1);
   prolong_add(lv, cv);
-  for (int s = 0; s < mg->nu_post; ++s) { rb_sweep(lv, 1, mg->omega); rb_sweep(lv, 0, mg->omega); }
+  for (int s = 0; s < mg->nu_post; ++s) { rb_sweep(lv, 1, om); rb_sweep(lv, 0, om); }
+}
+
+// The hierarchy's pieces, for tests that rebuild the cycle from them: what = 0: y = A'_l x
+// on level l; 1: y = P_l x (x on level l + 1, y on level l); 2: y = P_l^T x (the
+// restriction); 3: y = the bottom pseudo-inverse times x (l ignored).  Returns -1 for an
+// invalid level.
+int orc_mg_piece(void* h, int what, int l, const double* x, double* y) {
+  MG* mg = (MG*)h;
+  const int L = (int)mg->L.size();
+  if (what == 3) {
+    MGLevel& b = mg->L.back();
+    const int n = b.nx * b.ny;
+    for (int a = 0; a < n; ++a) {
+      double s = 0;
+      for (int c = 0; c < n; ++c) s += mg->bottom_inv[(size_t)a * n + c] * x[c];
+      y[a] = s;
+    }
+    return 0;
+  }
+  if (l < 0 || l >= L || (what != 0 && l + 1 >= L)) return -1;
+  MGLevel& f = mg->L[l];
+  const size_t nf = (size_t)f.nx * f.ny;
+  if (what == 0) {
+    const std::vector<double> xv(x, x + nf);
+    for (int j = 0; j < f.ny; ++j)
+      for (int i = 0; i < f.nx; ++i) y[C(i, j, f.nx)] = diag_at(f, i, j) * xv[C(i, j, f.nx)] - offsum(f, xv, i, j);
+    return 0;
+  }
+  MGLevel& c = mg->L[l + 1];
+  const size_t nc = (size_t)c.nx * c.ny;
+  if (what == 1) {
+    std::vector<double> sx = f.x;
+    std::fill(f.x.begin(), f.x.end(), 0.0);
+    std::vector<double> cx = c.x;
+    std::memcpy(c.x.data(), x, nc * sizeof(double));
+    prolong_add(f, c);
+    std::memcpy(y, f.x.data(), nf * sizeof(double));
+    f.x.swap(sx);
+    c.x.swap(cx);
+    return 0;
+  }
+  if (what == 2) {
+    std::vector<double> sres = f.res, cr = c.r;
+    std::memcpy(f.res.data(), x, nf * sizeof(double));
+    restrict_(f, c);
+    std::memcpy(y, c.r.data(), nc * sizeof(double));
+    f.res.swap(sres);
+    c.r.swap(cr);
+    return 0;
+  }
+  return -1;
 }
 
 // z = M r_scaled: one V-cycle on A' x = r' (r' = h^2 r for the fine operator).
@@ -1064,11 +1120,11 @@ void orc_mg_vcycle(void* h, const double* rs, double* z) {
 // x updated in place, stops when ||r||_2 <= rtol ||b||_2 and ||r||_inf <= atol
 // (either test disabled by a non-positive tolerance).  Returns iterations.
 int orc_mg_pcg(int nx, int ny, double h, const double* bx, const double* by, double* x, const double* b, double rtol,
-               double atol, int max_iteration, int max_levels, int nu, int mode, double omega, double* out_res2,
-               double* out_resinf) {
+               double atol, int max_iteration, int max_levels, int nu, int mode, double omega, double omega_coarse,
+               double* out_res2, double* out_resinf) {
   const int N = nx * ny;
   std::vector<double> bxs(bx, bx + (size_t)(nx + 1) * ny), bys(by, by + (size_t)nx * (ny + 1));
-  void* mg = orc_mg_create(nx, ny, bxs.data(), bys.data(), max_levels, 64, nu, nu, mode, omega);
+  void* mg = orc_mg_create(nx, ny, bxs.data(), bys.data(), max_levels, 64, nu, nu, mode, omega, omega_coarse);
   if (!mg) return -1;
   std::vector<double> r(N), z(N), p(N), q(N), rs(N), bm(b, b + N);
   // A0 is singular (constants); solve with the range component of b, i.e. b minus its mean.
@@ -1137,7 +1193,7 @@ int orc_project(int nx, int ny, double h, double dt, const double* phi, double r
     std::vector<double> b(N);
     orc_rhs_consistent(nx, ny, h, dt, phi, jc.data(), bx.data(), by.data(), u, v, b.data());
     iters = orc_mg_pcg(nx, ny, h, bx.data(), by.data(), x.data(), b.data(), tol, 0.0, max_iteration, 0, 2, MG_DEFAULT_MODE,
-                       1.0, &res, nullptr);
+                       1.0, 1.0, &res, nullptr);
     const double x0 = x[0];
     for (size_t k = 0; k < N; ++k) x[k] -= x0;  // report with the reference's pin p(0,0) = 0
   } else {
@@ -1181,7 +1237,7 @@ int orc_adjoint(int nx, int ny, double h, double dt, const double* phi, const do
       rhs[C(i, j, nx)] = -((gxf(i + 1) - gxf(i)) * ih + (gyf(j + 1) - gyf(j)) * ih);
     }
   const int it =
-      orc_mg_pcg(nx, ny, h, bx, by, mu.data(), rhs.data(), rtol, 0.0, 100000, 0, 2, MG_DEFAULT_MODE, 1.0, nullptr, nullptr);
+      orc_mg_pcg(nx, ny, h, bx, by, mu.data(), rhs.data(), rtol, 0.0, 100000, 0, 2, MG_DEFAULT_MODE, 1.0, 1.0, nullptr, nullptr);
   if (it < 0) return -1;
   jump_faces(nx, ny, h, phi, jc, Jx, Jy);
   for (int j = 0; j < ny; ++j)
@@ -1214,7 +1270,7 @@ int orc_project_beta(int nx, int ny, double h, double dt, const double* phi, con
   std::vector<double> b(N), x(N, 0.0);
   orc_rhs_consistent(nx, ny, h, dt, phi, jc, bx, by, u, v, b.data());
   const int it =
-      orc_mg_pcg(nx, ny, h, bx, by, x.data(), b.data(), rtol, 0.0, 100000, 0, 2, MG_DEFAULT_MODE, 1.0, nullptr, nullptr);
+      orc_mg_pcg(nx, ny, h, bx, by, x.data(), b.data(), rtol, 0.0, 100000, 0, 2, MG_DEFAULT_MODE, 1.0, 1.0, nullptr, nullptr);
   if (it < 0) return -1;
   double m = 0;
   for (size_t k = 0; k < N; ++k) m += x[k];

@@ -46,9 +46,10 @@ def test_bad_arguments_rejected_without_compute(P2D):
     cfg.inner_rtol = 1.5
     assert N.lib.p2d_create(ctypes.byref(cfg), ctypes.byref(h)) == N.P2D_BAD_ARG
     for om in (0.0, 2.0, float("nan")):         # SOR needs 0 < omega < 2
-        cfg = N.default_config(8, 8, 0.125)
-        cfg.mg_omega = om
-        assert N.lib.p2d_create(ctypes.byref(cfg), ctypes.byref(h)) == N.P2D_BAD_ARG
+        for field in ("mg_omega", "mg_omega_coarse"):
+            cfg = N.default_config(8, 8, 0.125)
+            setattr(cfg, field, om)
+            assert N.lib.p2d_create(ctypes.byref(cfg), ctypes.byref(h)) == N.P2D_BAD_ARG
     assert N.status_string(N.P2D_NAN) == "NaN or breakdown"
 
 

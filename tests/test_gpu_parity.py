@@ -49,17 +49,19 @@ def test_stencil_matches_oracle(P2D, O, W, prec):
         assert np.abs(y - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
-# omega: the smoother's over-relaxation (1: Gauss-Seidel)
-@pytest.mark.parametrize("prec,nu,omega", [(0, 1, 1.0), (0, 2, 1.0), (0, 3, 1.0), (1, 2, 1.0), (1, 3, 1.0),
-                                           (0, 1, 1.4), (0, 3, 1.4), (1, 3, 1.4), (0, 3, 1.7), (1, 2, 0.8)])
-def test_vcycle_matches_oracle(P2D, O, W, prec, nu, omega):
+# omega, omega_coarse: the smoother's over-relaxation on the finest level and below it (1: Gauss-Seidel)
+@pytest.mark.parametrize("prec,nu,omega,omc", [(0, 1, 1.0, 1.0), (0, 2, 1.0, 1.0), (0, 3, 1.0, 1.0), (1, 2, 1.0, 1.0),
+                                               (1, 3, 1.0, 1.0), (0, 1, 1.4, 1.4), (0, 3, 1.4, 1.5), (1, 3, 1.4, 1.5),
+                                               (0, 3, 1.7, 1.2), (1, 2, 0.8, 1.6)])
+def test_vcycle_matches_oracle(P2D, O, W, prec, nu, omega, omc):
     w = W.make("drops64", 256)
-    print(f"case vcycle prec={prec} nu={nu} omega={omega}", flush=True)
+    print(f"case vcycle prec={prec} nu={nu} omega={omega} omega_coarse={omc}", flush=True)
     bx, by, jc = _coeffs(O, w)
-    ctx = P2D.Context(w.n, w.n, w.h, mg_pre_sweeps=nu, mg_post_sweeps=nu, mg_omega=omega)
+    ctx = P2D.Context(w.n, w.n, w.h, mg_pre_sweeps=nu, mg_post_sweeps=nu, mg_omega=omega, mg_omega_coarse=omc)
     ctx.set_coefficients(bx, by, w.phi, jc)
     # the FP32 hierarchy stores its weights in binary16: its oracle is mode 7
-    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32, omega=omega)
+    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32, omega=omega,
+              omega_coarse=omc)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(6).standard_normal((w.n, w.n))
     r -= r.mean()
@@ -79,12 +81,13 @@ def test_vcycle_matches_oracle(P2D, O, W, prec, nu, omega):
                                               (120, 72, 0, 1.0), (120, 72, 1, 1.0), (128, 18, 0, 1.0),
                                               (160, 96, 0, 1.4), (160, 96, 1, 1.4), (120, 72, 0, 1.4)])
 def test_vcycle_rectangular_matches_oracle(P2D, O, nx, ny, prec, omega):
+    """omega on every level (omega_coarse = omega)."""
     print(f"case vcycle rectangular {nx}x{ny} prec={prec} omega={omega}", flush=True)
     h = 1.0 / nx
     X, Y = np.meshgrid((np.arange(nx) + 0.5) * h, (np.arange(ny) + 0.5) * h)
     phi = np.hypot(X - 0.4, Y - 0.45 * ny / nx) - 0.3 * ny / nx
     bx, by = O.face_beta(phi, 1.0, 1000.0)
-    ctx = P2D.Context(nx, ny, h, mg_pre_sweeps=2, mg_post_sweeps=2, mg_omega=omega)
+    ctx = P2D.Context(nx, ny, h, mg_pre_sweeps=2, mg_post_sweeps=2, mg_omega=omega, mg_omega_coarse=omega)
     ctx.set_coefficients(bx, by, phi, np.zeros_like(phi))
     mg = O.MG(bx, by, nu_pre=2, nu_post=2, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32, omega=omega)
     assert mg.levels == ctx.mg_levels
@@ -151,10 +154,10 @@ def test_vcycle_2048_with_two_block_coarse_level(P2D, O, W):
     bx, by, jc = _coeffs(O, w)
     ctx = P2D.Context(w.n, w.n, w.h)
     ctx.set_coefficients(bx, by, w.phi, jc)
-    om = ctx.cfg.mg_omega
-    assert om == 1.4
-    mg = O.MG(bx, by, nu_pre=3, nu_post=3, omega=om)
-    mg16 = O.MG(bx, by, nu_pre=3, nu_post=3, mode=O.MG_MODE_F32, omega=om)
+    om, omc = ctx.cfg.mg_omega, ctx.cfg.mg_omega_coarse
+    assert (om, omc) == (1.4, 1.5)
+    mg = O.MG(bx, by, nu_pre=3, nu_post=3, omega=om, omega_coarse=omc)
+    mg16 = O.MG(bx, by, nu_pre=3, nu_post=3, mode=O.MG_MODE_F32, omega=om, omega_coarse=omc)
     assert mg.levels == ctx.mg_levels
     r = np.random.default_rng(8).standard_normal((w.n, w.n))
     r -= r.mean()

@@ -127,6 +127,55 @@ def test_smoother_is_textbook_sor(O, W, nu, omega):
         assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("omega,omega_coarse", [(1.0, 1.0), (1.4, 1.5), (1.2, 1.7)])
+def test_three_level_cycle_is_textbook(O, W, omega, omega_coarse):
+    """The whole V-cycle, pinned against a dense textbook recursion built from the hierarchy's own
+    pieces (orc_mg_piece: the level operators A'_l, the prolongations P_l, the restrictions and the
+    bottom pseudo-inverse) on 32^2 -> 16^2 -> 8^2: the restriction is exactly P^T, and the cycle
+    is nu SOR red-black half-sweep pairs with omega on the finest level and omega_coarse below it,
+    the coarse correction by recursion, then nu (black, red) pairs."""
+    w = W.circle(32)
+    bx, by = O.face_beta(w.phi, w.rho_minus, w.rho_plus)
+    nu = 2
+    mg = O.MG(bx, by, nu_pre=nu, nu_post=nu, omega=omega, omega_coarse=omega_coarse)
+    assert mg.levels == 3
+    side = [32, 16, 8]
+    n = [s * s for s in side]
+    A = [np.stack([mg.piece(0, l, np.eye(n[l])[k], n[l]) for k in range(n[l])], axis=1) for l in range(2)]
+    P = [np.stack([mg.piece(1, l, np.eye(n[l + 1])[k], n[l]) for k in range(n[l + 1])], axis=1) for l in range(2)]
+    R = [np.stack([mg.piece(2, l, np.eye(n[l])[k], n[l + 1]) for k in range(n[l])], axis=1) for l in range(2)]
+    B = np.stack([mg.piece(3, 0, np.eye(n[2])[k], n[2]) for k in range(n[2])], axis=1)
+    for l in range(2):
+        assert np.array_equal(R[l], P[l].T)
+    assert np.allclose(A[0], _dense_A_scaled(bx, by), rtol=0, atol=1e-12 * np.abs(A[0]).max())
+
+    def cycle(l, r):
+        if l == 2:
+            return B @ r
+        wl = omega if l == 0 else omega_coarse
+        jj, ii = np.divmod(np.arange(n[l]), side[l])
+        colour = (ii + jj) & 1
+        d = np.diag(A[l])
+
+        def half(x, c):
+            gs = (r - (A[l] @ x - d * x)) / d
+            return np.where(colour == c, (1.0 - wl) * x + wl * gs, x)
+        x = np.zeros(n[l])
+        for _ in range(nu):
+            x = half(half(x, 0), 1)
+        x = x + P[l] @ cycle(l + 1, P[l].T @ (r - A[l] @ x))
+        for _ in range(nu):
+            x = half(half(x, 1), 0)
+        return x
+
+    rng = np.random.default_rng(9)
+    for _ in range(3):
+        r = rng.standard_normal(n[0])
+        ref = cycle(0, r)
+        got = mg.vcycle(r.reshape(32, 32)).ravel()
+        assert np.linalg.norm(got - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
 def test_sor_cuts_iterations(O, W):
     """Over-relaxation (omega = 1.4) needs fewer MG-PCG iterations than Gauss-Seidel on the
     1000:1 workloads (a regression bound, not a pin)."""
@@ -137,6 +186,9 @@ def test_sor_cuts_iterations(O, W):
         _, it1, _, _ = O.mg_pcg(bx, by, b, w.h, rtol=1e-12, nu=3)
         _, it14, _, _ = O.mg_pcg(bx, by, b, w.h, rtol=1e-12, nu=3, omega=1.4)
         assert it14 < it1, (name, it1, it14)
+        if name == "drops64":   # many small interfaces: the coarse levels gain from more relaxation
+            _, it15, _, _ = O.mg_pcg(bx, by, b, w.h, rtol=1e-12, nu=3, omega=1.4, omega_coarse=1.5)
+            assert it15 <= it14, (name, it14, it15)
 
 
 def test_half_rounding_matches_numpy(O):
