@@ -416,8 +416,10 @@ void vcycle_head(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const int kpre = sizeof(T) == 4 ? KT_PRE0_F32 : KT_PRE0_F64;
   p2d_ctx::Track& t = c->trk;
   kt_begin(c, kpre);
+  // the pending update is the pass's first when no update has been applied since its start
+  const int xfirst = t.pending && t.iter == t.seg0 ? 1 : 0;
   k_mg_pre<T, true><<<gpre, TBLOCK, 0, c->stream>>>(lv[0], lv[1], v.r[t.rpar], lv[0].xa, nu1, v.r[1 - t.rpar], v.x,
-                                                    v.p[(t.iter + 1) & 1], v.q, t.pending, c->partial, c->s);
+                                                    v.p[(t.iter + 1) & 1], v.q, t.pending, xfirst, c->partial, c->s);
   kt_end(c, kpre);
   k_fin_pre0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpre), (double)c->N, c->s);
   // k_fin_pre0: the residual buffers swap; a pending update counts as an iteration
@@ -453,16 +455,16 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
     const Level<T>& F = lv[l];
     if (sh == 8)
       k_mg_pre<T, false, false, 8><<<tgrid(F.nx, F.ny, Hpre, Tile<8>::ty), Tile<8>::threads, 0, c->stream>>>(
-          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, c->s);
     else if (sh == 12)
       k_mg_pre<T, false, false, 12><<<tgrid(F.nx, F.ny, Hpre, Tile<12>::ty), Tile<12>::threads, 0, c->stream>>>(
-          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, c->s);
     else if (busy(l, Hpre))
       k_mg_pre<T, false, true><<<tgrid(F.nx, F.ny, Hpre), TBLOCK, 0, c->stream>>>(
-          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, c->s);
     else
       k_mg_pre<T, false><<<tgrid(F.nx, F.ny, Hpre), TBLOCK, 0, c->stream>>>(
-          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, nullptr, c->s);
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, c->s);
   }
   (void)join_side(c);  // enqueue-only; a failure shows at the next launch check
   SmallArgs<T> sa{};
@@ -585,7 +587,6 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
   k_inner_init<<<1, 1, 0, c->stream>>>(c->s, (double)N, rel * rel, mixed ? 0 : 1, rtol, atol, abs2,
                                        c->cfg.max_iterations);
   c->trk = p2d_ctx::Track{};  // k_inner_init: rpar = iter = pending = seg0 = 0
-  k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
   CHK(launched_ok());
   int status = P2D_OK, outer = 0, stalled = 0, iters = 0, seg_start = 0;
   double r2 = 0, rinf = 0, b2 = 0, best_r2 = 1e300, best_rinf = 1e300;
@@ -661,9 +662,12 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
     const double* oh = outer > 0 ? solh[*cur] : nullptr;  // the first pass starts from p = 0
     const double* ol = outer > 0 ? soll[*cur] : nullptr;
     // a pass that ended at its direction's finish leaves that update pending: the
-    // refinement applies it (direction p[iter & 1], alpha in the state)
+    // refinement applies it (direction p[iter & 1], alpha in the state); x holds the
+    // pass's correction only if a head applied an update (the first writes x = a p)
     const T* pd = c->hs->pending ? v.p[c->hs->iter & 1] : nullptr;
-    correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, oh, ol, v.x, pd, solh[1 - *cur], soll[1 - *cur], rdst);
+    const bool x_written = iters - seg_start - (c->hs->pending ? 1 : 0) > 0;
+    correct_resid<T>(c, rhs.b, rhs.saw, rhs.sas, oh, ol, x_written ? v.x : nullptr, pd, solh[1 - *cur],
+                     soll[1 - *cur], rdst);
     *cur = 1 - *cur;
     ++outer;
     if (iters >= c->cfg.max_iterations || outer >= c->cfg.max_outer || stalled >= 1) {
@@ -677,7 +681,6 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
       if (!(r2 == r2)) status = P2D_NAN;
       break;
     }
-    k_fill<T><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, v.x, (T)0);
     k_inner_continue<<<1, 1, 0, c->stream>>>(c->s, (double)N);
     c->trk.seg0 = c->trk.iter;  // k_inner_continue: a new direction sequence, nothing pending
     c->trk.pending = 0;
@@ -1640,8 +1643,8 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
         break;
       case 2:
         k_mg_pre<float, true><<<tgrid(c->nx, c->ny, 2 * nu1 + 1), TBLOCK, 0, c->stream>>>(
-            L32[0], L32[1], c->v32.r[0], L32[0].xa, nu1, c->v32.r[1], c->v32.x, c->v32.p[0], c->v32.q, 1, c->partial,
-            c->s);
+            L32[0], L32[1], c->v32.r[0], L32[0].xa, nu1, c->v32.r[1], c->v32.x, c->v32.p[0], c->v32.q, 1, 0,
+            c->partial, c->s);
         break;
       case 3:
         k_mg_post<float, true><<<tgrid(c->nx, c->ny, 2 * nu2), TBLOCK, 0, c->stream>>>(

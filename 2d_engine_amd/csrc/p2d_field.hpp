@@ -218,12 +218,14 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
   const int i0 = bx * FTX, j0 = by * FTY;
   const int iA = i0 + 2 * c;
   const bool col_in = iA < nx;
-  // p_new of one cell from p_old (hi, lo) and the correction
+  // p_new of one cell from p_old (hi, lo) and the correction (e, the pending update pd, both
+  // or neither)
+  const bool corr = e != nullptr || pd != nullptr;
   auto pnew = [&](double oh, double ol, double ev, double& hh, double& ll) {
     if (!ph) {
       hh = ev;
       ll = 0.0;
-    } else if (e) {
+    } else if (corr) {
       dd_add(oh, ol, ev, hh, ll);
     } else {
       hh = oh;
@@ -258,7 +260,8 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
     double hh = 0.0, ll = 0.0;
     if (ii >= 0 && ii < nx && jj >= 0 && jj < ny) {
       const size_t q = ci(ii, jj, nx);
-      const TE ev = e ? (pd ? e[q] + ta * pd[q] : e[q]) : (TE)0;
+      const TE e0 = e ? e[q] : (TE)0;
+      const TE ev = pd ? e0 + ta * pd[q] : e0;
       pnew(ph ? ph[q] : 0.0, ph ? pl[q] : 0.0, (double)ev, hh, ll);
     }
     sh[rr][side ? FTX + 2 : 1] = hh;

@@ -665,7 +665,8 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
 // Grid dim3(ceil(nx/TX), ceil(ny/ty)), 64 NW threads (tile shape Tile<NW>), H = 2 nu + 1, x halo
 // HX = tile_hx(H) (even), TX = 128 - 2 HX.
 // FUSE (finest level): r_lvl is the current residual buffer; the update r -= a q,
-// x += a p (pend: an update is pending, p = pdir) and the residual shift are
+// x += a p (pend: an update is pending, p = pdir; xfirst: it is the first of its
+// pass, x = a p, so x needs no clearing between passes) and the residual shift are
 // applied first (tile interiors write r to rout and x), and the block partials of
 // ||r|| are stored.  The host picks the buffers from its replay of the solver
 // state, so their loads do not wait for the state.
@@ -674,7 +675,7 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
 template <class T, bool FUSE, bool BUSY = false, int NW = 16>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE || BUSY>())))
 k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* xsol, const T* pdir,
-         const T* __restrict__ qv, int pend, double* partial, Scal* s) {
+         const T* __restrict__ qv, int pend, int xfirst, double* partial, Scal* s) {
   using Sh = Tile<NW>;
   constexpr int TY = Sh::ty, RYMAX = Sh::rymax;
   __shared__ Pair<T> lds[Sh::lds_pairs];
@@ -731,7 +732,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
       if (pend) {
         q2[m] = ld_at<Pair<T>>(qv, k);
         if (pair_int && r >= H && r < H + TY) {
-          x2[m] = ld_at<Pair<T>>(xsol, k);
+          if (!xfirst) x2[m] = ld_at<Pair<T>>(xsol, k);
           p2[m] = ld_at<Pair<T>>(pdir, k);
         }
       }
