@@ -435,7 +435,11 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const int nu1 = c->cfg.mg_pre_sweeps, nu2 = c->cfg.mg_post_sweeps;
   const int Hpre = 2 * nu1 + 1, Hpost = 2 * nu2;
   const Level<T>* lv = mg.lv.data();
-  const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost);
+  // The finest post-smoothing: FP32 in 16-wavefront tiles two blocks per CU; FP64, whose tiles
+  // take the registers of one 16-wavefront block per CU, in 8-wavefront tiles, two per CU, so
+  // that one block's loads overlap the other's sweeps (FP64 2048^2: 64 -> 57 us per launch).
+  constexpr int NW0 = sizeof(T) == 4 ? 16 : 8;
+  const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost, Tile<NW0>::ty);
   const int kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
   // Tile shape of a coarse level: 8-wavefront tiles (10 rows) when at most two of
   // them per CU cover it; else, when 16-wavefront tiles (32 rows) would need more
@@ -494,8 +498,8 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
                                                                               nullptr, c->s);
   }
   kt_begin(c, kpost);
-  k_mg_post<T, true><<<gpost, TBLOCK, 0, c->stream>>>(lv[0], lv[1], v.r[c->trk.rpar], lv[0].xa, v.z, nu2,
-                                                      c->partial, c->s);
+  k_mg_post<T, true, false, NW0><<<gpost, Tile<NW0>::threads, 0, c->stream>>>(lv[0], lv[1], v.r[c->trk.rpar],
+                                                                              lv[0].xa, v.z, nu2, c->partial, c->s);
   kt_end(c, kpost);
   k_fin_post0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpost), (double)c->N, c->s);
 }

@@ -1001,8 +1001,11 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_post0(const double* __re
 // --------------------------------------------------------------------------- single-workgroup coarse V-cycle
 // Levels l0..L-1 (at most SMALL_LV; the top one of at most SMALL_MAX_CELLS
 // cells, the others of at most SMALL_THREADS) in one workgroup.  Thread t owns
-// cells t + SMALL_THREADS u (u < SMALL_CPT) of the top level and cell t of
-// every level below, and keeps each owned cell's coefficients, diagonal and
+// the cell pairs (2p, 2p + 1), p = t + SMALL_THREADS v (v < SMALL_CPT / 2), of the
+// top level (horizontal neighbours, opposite colours: every red-black half-sweep
+// updates one cell of each pair, with no lane idle; slot 2v holds the pair's red
+// cell, 2v + 1 its black one) and cell t of every level below, and keeps each
+// owned cell's coefficients, diagonal and
 // right-hand side in registers ("slots"); LDS holds x of every level, r of
 // the levels below the top (filled by the restriction), the interpolation
 // weights above the bottom, the residual and the two restriction row-sum
@@ -1017,6 +1020,16 @@ static_assert(SMALL_CPT * SMALL_THREADS == SMALL_MAX_CELLS && SMALL_MAX_CELLS <=
               "levels below the top must have at most one cell per thread");
 __host__ __device__ constexpr int small_cpt(int q) { return q == 0 ? SMALL_CPT : 1; }
 __host__ __device__ constexpr int small_slot(int q, int u) { return q == 0 ? u : SMALL_CPT - 1 + q; }
+// The cell of thread t's slot u on level q (width nx) of the cycle: the top level's slots 2v
+// and 2v + 1 are the red and the black cell of the pair (2p, 2p + 1), p = t + SMALL_THREADS v
+// (one row: every level above the bottom has an even width; a bottom-only cycle has no sweeps,
+// so any one-to-one map serves it).
+__host__ __device__ inline int small_cell(int q, int t, int u, int nx) {
+  if (q != 0) return t;
+  const int e0 = 2 * (t + (u >> 1) * SMALL_THREADS), j = e0 / nx;
+  return e0 + ((u & 1) ^ (j & 1));
+}
+static_assert(SMALL_CPT % 2 == 0, "the top level's slots come in pairs");
 
 template <class T>
 struct SmallArgs {
@@ -1104,7 +1117,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     const int n = F.nx * F.ny;
 #pragma unroll
     for (int u = 0; u < small_cpt(q); ++u) {
-      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS;
+      const int sl = small_slot(q, u), e = small_cell(q, t, u, A.lv[q].nx);
       wc[sl] = e < n ? ld_wcell<T>(F, e) : WCell<T>{};
     }
   }
@@ -1115,7 +1128,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     const int nx = F.nx, ny = F.ny, n = nx * ny;
 #pragma unroll
     for (int u = 0; u < small_cpt(q); ++u) {
-      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS;
+      const int sl = small_slot(q, u), e = small_cell(q, t, u, A.lv[q].nx);
       ci_[sl] = cj_[sl] = 0;
       cw[sl] = ce[sl] = cs[sl] = cn[sl] = rr[sl] = (T)0;
       if (e >= n) continue;
@@ -1142,7 +1155,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     const int n = A.lv[q].nx * A.lv[q].ny;
 #pragma unroll
     for (int u = 0; u < small_cpt(q); ++u) {
-      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS;
+      const int sl = small_slot(q, u), e = small_cell(q, t, u, A.lv[q].nx);
       if (e >= n) continue;
       if (q + 1 < nl) {
         sm[L.wx[q] + e] = wc[sl].x();
@@ -1158,11 +1171,10 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     if (ci_[sl] == 0) cw[sl] = (T)0;
     if (cj_[sl] == 0) cs[sl] = (T)0;
   }
-  if (nl == 1 && t < nb) sm[L.r[0] + t] = rr[0];
   if (nl == 1) {
 #pragma unroll
-    for (int u = 1; u < SMALL_CPT; ++u)
-      if (t + u * SMALL_THREADS < nb) sm[L.r[0] + t + u * SMALL_THREADS] = rr[u];
+    for (int u = 0; u < SMALL_CPT; ++u)
+      if (small_cell(0, t, u, A.lv[0].nx) < nb) sm[L.r[0] + small_cell(0, t, u, A.lv[0].nx)] = rr[u];
   }
   if (done) return;
   // omega / diag (the levels below the finest share omega)
@@ -1170,22 +1182,31 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 #pragma unroll
   for (int sl = 0; sl < SMALL_SLOTS; ++sl) id[sl] = cd[sl] != (T)0 ? om / cd[sl] : (T)0;
   __syncthreads();
-  // one red-black half-sweep of level q (cells (i + j) & 1 == color)
-  auto half = [&](int q, int color, bool zero) {
+  // one red-black half-sweep of level q (cells (i + j) & 1 == color).  On the top level
+  // the cell of each pair with the colour is picked by selects: one update per pair.
+  auto half = [&](int q, int color, bool zero) __attribute__((always_inline)) {
     const int nx = A.lv[q].nx, ny = A.lv[q].ny, n = nx * ny;
     T* X = sm + L.x[q];
-#pragma unroll
-    for (int u = 0; u < small_cpt(q); ++u) {
-      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS, i = ci_[sl], j = cj_[sl];
-      if (e < n && ((i + j) & 1) == color) {
-        if (zero) {
-          X[e] = rr[sl] * id[sl];
-        } else {
-          const T o = cw[sl] * X[i > 0 ? e - 1 : e] + ce[sl] * X[i + 1 < nx ? e + 1 : e] +
-                      cs[sl] * X[j > 0 ? e - nx : e] + cn[sl] * X[j + 1 < ny ? e + nx : e];
-          X[e] = (rr[sl] + o) * id[sl] + omc * X[e];
-        }
+    auto update = [&](int e, int i, int j, T w_, T e_, T s_, T n_, T r_, T id_) __attribute__((always_inline)) {
+      if (zero) {
+        X[e] = r_ * id_;
+      } else {
+        const T o = w_ * X[i > 0 ? e - 1 : e] + e_ * X[i + 1 < nx ? e + 1 : e] + s_ * X[j > 0 ? e - nx : e] +
+                    n_ * X[j + 1 < ny ? e + nx : e];
+        X[e] = (r_ + o) * id_ + omc * X[e];
       }
+    };
+    if (q == 0) {
+#pragma unroll
+      for (int v = 0; v < SMALL_CPT / 2; ++v) {
+        const int sl = small_slot(0, 2 * v + color);  // the pair's cell of this colour
+        if (2 * (t + v * SMALL_THREADS) >= n) continue;
+        const int i = ci_[sl], j = cj_[sl];
+        update(i + j * nx, i, j, cw[sl], ce[sl], cs[sl], cn[sl], rr[sl], id[sl]);
+      }
+    } else {
+      const int sl = small_slot(q, 0), e = t, i = ci_[sl], j = cj_[sl];
+      if (e < n && ((i + j) & 1) == color) update(e, i, j, cw[sl], ce[sl], cs[sl], cn[sl], rr[sl], id[sl]);
     }
     __syncthreads();
   };
@@ -1203,7 +1224,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     T* R = sm + L.tmp;
 #pragma unroll
     for (int u = 0; u < small_cpt(q); ++u) {
-      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS, i = ci_[sl], j = cj_[sl];
+      const int sl = small_slot(q, u), e = small_cell(q, t, u, A.lv[q].nx), i = ci_[sl], j = cj_[sl];
       if (e >= n) continue;
       const T o = cw[sl] * X[i > 0 ? e - 1 : e] + ce[sl] * X[i + 1 < nx ? e + 1 : e] +
                   cs[sl] * X[j > 0 ? e - nx : e] + cn[sl] * X[j + 1 < ny ? e + nx : e];
@@ -1288,7 +1309,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
     const T* xc = sm + L.x[q + 1];
 #pragma unroll
     for (int u = 0; u < small_cpt(q); ++u) {
-      const int sl = small_slot(q, u), e = t + u * SMALL_THREADS, i = ci_[sl], j = cj_[sl];
+      const int sl = small_slot(q, u), e = small_cell(q, t, u, A.lv[q].nx), i = ci_[sl], j = cj_[sl];
       if (e >= n) continue;
       const T wx = sm[L.wx[q] + e], wy = sm[L.wy[q] + e];
       const int I = i >> 1, J = j >> 1;
