@@ -409,7 +409,7 @@ void invalidate(p2d_ctx* c) {
 // convergence test) and the tail (the rest; every tail kernel returns at once
 // once the test has set s->done).
 template <class T>
-void vcycle_head(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
+void vcycle_head(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int guard = 0) {
   const int nu1 = c->cfg.mg_pre_sweeps;
   const Level<T>* lv = mg.lv.data();
   const dim3 gpre = tgrid(lv[0].nx, lv[0].ny, 2 * nu1 + 1);
@@ -419,7 +419,8 @@ void vcycle_head(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   // the pending update is the pass's first when no update has been applied since its start
   const int xfirst = t.pending && t.iter == t.seg0 ? 1 : 0;
   k_mg_pre<T, true><<<gpre, TBLOCK, 0, c->stream>>>(lv[0], lv[1], v.r[t.rpar], lv[0].xa, nu1, v.r[1 - t.rpar], v.x,
-                                                    v.p[(t.iter + 1) & 1], v.q, t.pending, xfirst, c->partial, c->s);
+                                                    v.p[(t.iter + 1) & 1], v.q, t.pending, xfirst, guard, c->partial,
+                                                    c->s);
   kt_end(c, kpre);
   k_fin_pre0<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(gpre), (double)c->N, c->s);
   // k_fin_pre0: the residual buffers swap; a pending update counts as an iteration
@@ -459,16 +460,16 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
     const Level<T>& F = lv[l];
     if (sh == 8)
       k_mg_pre<T, false, false, 8><<<tgrid(F.nx, F.ny, Hpre, Tile<8>::ty), Tile<8>::threads, 0, c->stream>>>(
-          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, c->s);
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, nullptr, c->s);
     else if (sh == 12)
       k_mg_pre<T, false, false, 12><<<tgrid(F.nx, F.ny, Hpre, Tile<12>::ty), Tile<12>::threads, 0, c->stream>>>(
-          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, c->s);
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, nullptr, c->s);
     else if (busy(l, Hpre))
       k_mg_pre<T, false, true><<<tgrid(F.nx, F.ny, Hpre), TBLOCK, 0, c->stream>>>(
-          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, c->s);
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, nullptr, c->s);
     else
       k_mg_pre<T, false><<<tgrid(F.nx, F.ny, Hpre), TBLOCK, 0, c->stream>>>(
-          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, nullptr, c->s);
+          F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, nullptr, c->s);
   }
   (void)join_side(c);  // enqueue-only; a failure shows at the next launch check
   SmallArgs<T> sa{};
@@ -608,7 +609,7 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
     const auto t_launch0 = std::chrono::steady_clock::now();
     for (int k = 0; k < chunk; ++k) {
       if (tail_pending) cg_tail<T>(c, mg, v);
-      vcycle_head<T>(c, mg, v);
+      vcycle_head<T>(c, mg, v, k + 1 == chunk ? 1 : 0);
       tail_pending = true;
     }
     CHK(launched_ok());
@@ -1647,7 +1648,7 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
         break;
       case 2:
         k_mg_pre<float, true><<<tgrid(c->nx, c->ny, 2 * nu1 + 1), TBLOCK, 0, c->stream>>>(
-            L32[0], L32[1], c->v32.r[0], L32[0].xa, nu1, c->v32.r[1], c->v32.x, c->v32.p[0], c->v32.q, 1, 0,
+            L32[0], L32[1], c->v32.r[0], L32[0].xa, nu1, c->v32.r[1], c->v32.x, c->v32.p[0], c->v32.q, 1, 0, 0,
             c->partial, c->s);
         break;
       case 3:

@@ -86,7 +86,7 @@ struct Scal {
   int err;                         // coefficient assembly error (zero level-set gradient)
   int seg0;                        // iteration at which the current direction sequence started (beta = 0)
   int outer_done;                  // the outer targets hold after a refinement
-  int near;                        // the last head's residual is within NEAR_END of the pass's target
+  int near;                        // the last head's residual is near the pass's target (k_fin_pre0)
 };
 
 // Index of the calling thread's wavefront in its block, as a wave-uniform (scalar) value:

@@ -669,16 +669,19 @@ __device__ __forceinline__ void strip_half(Strip<T>& S, Pair<T>* bnd, int c, int
 // pass, x = a p, so x needs no clearing between passes) and the residual shift are
 // applied first (tile interiors write r to rout and x), and the block partials of
 // ||r|| are stored.  The host picks the buffers from its replay of the solver
-// state, so their loads do not wait for the state.
+// state, so their loads do not wait for the state; a guarded head (the last of an
+// enqueued chunk, which an early end at the direction finish usually makes
+// unnecessary) reads the state first and issues no load once the pass is done.
 // Restriction r_H = P^T res: each cell splits its residual into the four
 // weight products; the x sums run across lanes, the y sums through LDS.
 template <class T, bool FUSE, bool BUSY = false, int NW = 16>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, FUSE || BUSY>())))
 k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* xsol, const T* pdir,
-         const T* __restrict__ qv, int pend, int xfirst, double* partial, Scal* s) {
+         const T* __restrict__ qv, int pend, int xfirst, int guard, double* partial, Scal* s) {
   using Sh = Tile<NW>;
   constexpr int TY = Sh::ty, RYMAX = Sh::rymax;
   __shared__ Pair<T> lds[Sh::lds_pairs];
+  if (FUSE && guard && s->done) return;
   const int H = 2 * nu + 1, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
   int bx, by;
   xcd_tile(bx, by);
@@ -845,11 +848,13 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
   }
 }
 
-// A residual within this factor (in sum r^2) of its pass's relative target makes
-// the next direction kernel prepare the early convergence test (k_dir<T, true>):
-// with the multigrid's ~0.15 reduction per iteration, the next update then
-// usually meets the target.
+// The next direction kernel prepares the early convergence test (k_dir<T, true>)
+// when the residual is within NEAR_END (in sum r^2) of its pass's relative target,
+// or when one more iteration at the rate of the last one would bring it within
+// NEAR_RATE of the target: the over-relaxed multigrid cuts sum r^2 by ~100x per
+// iteration, so a fixed window alone would miss the last iteration of most passes.
 constexpr double NEAR_END = 64.0;
+constexpr double NEAR_RATE = 16.0;
 
 // Finish of the fused finest pre kernel: residual buffers swap, the shift is
 // spent, and a pending update counts as an iteration with its convergence test
@@ -861,7 +866,7 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
                                                              double ncells, Scal* s) {
   const int done = s->done, pending = s->pending, rpar = s->rpar, iter = s->iter, max_iter = s->max_iter;
   const int flag = s->flag;
-  const double rz = s->rz, rel2 = s->rel2, r0_2 = s->r0_2, abs2 = s->abs2, ainf = s->ainf;
+  const double rz = s->rz, rel2 = s->rel2, r0_2 = s->r0_2, abs2 = s->abs2, ainf = s->ainf, r2_prev = s->r2;
   double a[3];
   finish_reduce<3>(a, partial, G, 2u);
   if (threadIdx.x != 0 || done) return;
@@ -872,7 +877,8 @@ __global__ void __launch_bounds__(FINISH_THREADS) k_fin_pre0(const double* __res
   s->near = 0;
   if (pending) {
     a[0] = fmax(a[0] - a[2] * a[2] / ncells, 0.0);
-    s->near = a[0] <= NEAR_END * rel2 * r0_2 ? 1 : 0;
+    const double next = r2_prev > 0.0 ? a[0] * fmin(1.0, a[0] / r2_prev) : a[0];  // one more iteration at this rate
+    s->near = a[0] <= NEAR_END * rel2 * r0_2 || next <= NEAR_RATE * rel2 * r0_2 ? 1 : 0;
     s->r2 = a[0];
     s->rinf = a[1];
     s->rz_old = rz;
