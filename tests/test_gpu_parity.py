@@ -102,6 +102,36 @@ def test_vcycle_rectangular_matches_oracle(P2D, O, nx, ny, prec, omega):
         assert np.linalg.norm(z - ref) <= 1e-4 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_vcycle_tiled_setup_partial_tiles(P2D, O, prec):
+    """608 x 512 (311,296 cells): the finest level is set up by the tiled setup sweep, whose
+    64-column tiles do not divide 608, so its last tile column is partial and reads faces
+    beyond the grid as walls; the cycle (default over-relaxation) still matches the oracle.
+    (A partial last tile row cannot occur: a height that is not a multiple of 16 halves at most
+    three times, which leaves no legal bottom for a grid of 512^2 cells or more.)"""
+    nx, ny = 608, 512
+    print(f"case vcycle tiled setup {nx}x{ny} prec={prec}", flush=True)
+    h = 1.0 / nx
+    X, Y = np.meshgrid((np.arange(nx) + 0.5) * h, (np.arange(ny) + 0.5) * h)
+    phi = np.minimum(np.hypot(X - 0.93, Y - 0.41) - 0.05, np.hypot(X - 0.3, Y - 0.5) - 0.12)
+    bx, by = O.face_beta(phi, 1.0, 1000.0)
+    ctx = P2D.Context(nx, ny, h)
+    ctx.set_coefficients(bx, by, phi, np.zeros_like(phi))
+    om, omc = ctx.cfg.mg_omega, ctx.cfg.mg_omega_coarse
+    mg = O.MG(bx, by, nu_pre=3, nu_post=3, mode=O.MG_DEFAULT_MODE if prec == 0 else O.MG_MODE_F32, omega=om,
+              omega_coarse=omc)
+    assert mg.levels == ctx.mg_levels == 6   # 608x512 ... 19x16
+    r = np.random.default_rng(11).standard_normal((ny, nx))
+    r -= r.mean()
+    ref = mg.vcycle(r)
+    if prec == 0:
+        z = ctx.vcycle(r, prec=0)
+        assert np.linalg.norm(z - ref) <= 1e-11 * np.linalg.norm(ref)
+    else:
+        z = ctx.vcycle(r.astype(np.float32), prec=1)
+        assert np.linalg.norm(z - ref) <= 1e-4 * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("name,n,prec", [("circle", 128, 0), ("circle", 128, 1), ("rt", 256, 0), ("rt", 256, 1),
                                          ("drops64", 256, 0), ("drops64", 256, 1)])
 def test_projection_matches_oracle(P2D, O, W, name, n, prec):
