@@ -443,7 +443,7 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost, Tile<NW0>::ty);
   const int kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
   // Tile shape of a coarse level: 8-wavefront tiles (10 rows) when at most two of
-  // them per CU cover it; else, when 16-wavefront tiles (32 rows) would need more
+  // them per CU cover it; else, when 16-wavefront tiles (34 rows) would need more
   // than one per CU, 12-wavefront tiles (22 rows) if at most two per CU cover it
   // (fewer wavefronts per SIMD on the busiest CUs); else 16-wavefront tiles, FP32
   // ones two blocks per CU when they outnumber the CUs (BUSY variants).

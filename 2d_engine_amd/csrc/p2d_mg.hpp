@@ -32,14 +32,14 @@ constexpr int KR = 3;                      // consecutive region rows per thread
 
 // Tile shapes of the pre/post kernels: NW wavefronts of KR region rows each and ty
 // output rows per tile, so that a region (ty + 2H rows, H <= 2 MAX_NU + 1) fits the
-// NW KR rows: 16 wavefronts and 32 rows, 12 and 22, 8 and 10.  The half-sweeps are
+// NW KR rows: 16 wavefronts and 34 rows, 12 and 22, 8 and 10.  The half-sweeps are
 // VALU issue bound per SIMD, so on the coarse levels, where CUs would otherwise
 // idle, tiles of fewer wavefronts shorten each one; the host picks the shape per
 // level (the finest level keeps the 16-wavefront tiles, whose halos cost least).
 template <int NW>
 struct Tile {
   static constexpr int waves = NW, threads = 64 * NW;
-  static constexpr int ty = NW == 16 ? 32 : (NW == 12 ? 22 : 10);
+  static constexpr int ty = NW == 16 ? 34 : (NW == 12 ? 22 : 10);
   static_assert(NW == 16 || NW == 12 || NW == 8, "tile shapes");
   static constexpr int rymax = ty + 2 * (2 * MAX_NU + 1);  // region rows at most
   static constexpr int bnd = NW * 2 * 64;                  // one boundary buffer: [wave][bottom, top][lane]
