@@ -1339,7 +1339,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 // p = (z - mean z) + beta p_old (FP64 arithmetic, stored as T; first: the first
 // direction of a sequence, p_old unused; the host picks p_old and p_new from its
 // replay of the solver state), q = A0' p, the block partials of p.q.  EARLY (the
-// FP32 passes), when the state says the pass is near its end: also r.q, q.q and
+// solves' passes), when the state says the pass is near its end: also r.q, q.q and
 // sum q with the current residual rcur, from which k_fin_dir<true> knows the next
 // residual's norm before the update is applied.  The choice depends on the solve's
 // own state only, so a solve ends its passes in the same places whatever ran before.
