@@ -26,15 +26,20 @@ __global__ void __launch_bounds__(BLOCK) k_reinit_sign(size_t N, double h, const
 // x / y given z = RN(1/y), correctly rounded without the general division sequence:
 // q = x z, then one FMA correction q + (x - q y) z (Markstein 1990: with z the correctly
 // rounded reciprocal and q within one ulp of x / y, the remainder x - q y is exact and the
-// corrected sum rounds to x / y).  An exact quotient keeps q (and with it the
-// sign of a zero), and so does an infinite one (its remainder is NaN).  Valid for
-// |x / y| >= 2^-1000, where the product cannot lose bits to underflow; the WENO inputs are
-// scaled differences of level-set values, zero or far above that.
+// corrected sum rounds to x / y).  Valid for |x / y| >= 2^-1000, where the product cannot
+// lose bits to underflow; the WENO inputs are scaled differences of level-set values, zero
+// or far above that.  The remainder is formed as -(q y - x): it is the same number (round
+// to nearest is symmetric) except that an exact zero comes out as -0, so the correction
+// adds -0 to q and an exact quotient keeps q with the sign of its zero.  This equals the
+// form that tests the remainder for zero and selects q, bit for bit, on every finite x
+// (140 M random and edge-case inputs on the CPU), without the compare and select (~4 VALU
+// instructions per quotient, ~40 quotients per re-initialised cell).  An infinite x gives
+// NaN instead of an infinity; the level set is finite.
 __device__ __forceinline__ double div_known(double x, double y, double z) {
 #pragma clang fp contract(off)
   const double q = x * z;
-  const double r = fma(-q, y, x);
-  return (r == 0.0 || r != r) ? q : fma(r, z, q);
+  const double r = -fma(q, y, -x);
+  return fma(r, z, q);
 }
 __device__ __forceinline__ double div3(double x) { return div_known(x, 3.0, 1.0 / 3.0); }
 __device__ __forceinline__ double div6(double x) { return div_known(x, 6.0, 1.0 / 6.0); }
