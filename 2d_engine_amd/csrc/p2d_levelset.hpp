@@ -41,6 +41,25 @@ __device__ __forceinline__ double div_known(double x, double y, double z) {
   const double r = -fma(q, y, -x);
   return fma(r, z, q);
 }
+// a / b correctly rounded for positive normal a, b whose quotient and 1 / b are normal and
+// whose exponents differ by less than 768: the reciprocal estimate, two Newton steps
+// r += r (1 - b r) and one Markstein correction q + (a - b q) r.  These are the steps of the
+// general FP64 division sequence, whose operand scaling (v_div_scale), scaled FMA
+// (v_div_fmas) and special-case fixup (v_div_fixup) leave every value unchanged in that range,
+// so the result is the same correctly rounded quotient, three instructions shorter.  The
+// WENO weights use it: numerators 0.1, 0.3, 0.6 and 1, denominators (eps + s)^2 >= 1e-12
+// and their sums, far inside the range for any finite level set.
+__device__ __forceinline__ double div_pos(double a, double b) {
+#pragma clang fp contract(off)
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double e0 = fma(-b, r0, 1.0);
+  const double r1 = fma(r0, e0, r0);
+  const double e1 = fma(-b, r1, 1.0);
+  const double r2 = fma(r1, e1, r1);
+  const double q = a * r2;
+  const double rem = fma(-b, q, a);
+  return fma(rem, r2, q);
+}
 __device__ __forceinline__ double div3(double x) { return div_known(x, 3.0, 1.0 / 3.0); }
 __device__ __forceinline__ double div6(double x) { return div_known(x, 6.0, 1.0 / 6.0); }
 
@@ -51,13 +70,14 @@ __device__ __forceinline__ double weno5_d(double v1, double v2, double v3, doubl
   const double s2 = (double)13 / 12 * (v2 - 2 * v3 + v4) * (v2 - 2 * v3 + v4) + (double)1 / 4 * (v2 - v4) * (v2 - v4);
   const double s3 = (double)13 / 12 * (v3 - 2 * v4 + v5) * (v3 - 2 * v4 + v5) +
                     (double)1 / 4 * (3 * v3 - 4 * v4 + v5) * (3 * v3 - 4 * v4 + v5);
-  const double a1 = (double)1 / 10 * (double)1 / ((eps + s1) * (eps + s1));
-  const double a2 = (double)6 / 10 * (double)1 / ((eps + s2) * (eps + s2));
-  const double a3 = (double)3 / 10 * (double)1 / ((eps + s3) * (eps + s3));
+  // (T)1/10*(T)1/x is (0.1 * 1) / x: one rounded quotient of the constant
+  const double a1 = div_pos((double)1 / 10 * (double)1, (eps + s1) * (eps + s1));
+  const double a2 = div_pos((double)6 / 10 * (double)1, (eps + s2) * (eps + s2));
+  const double a3 = div_pos((double)3 / 10 * (double)1, (eps + s3) * (eps + s3));
   const double sum = a1 + a2 + a3;
   // the three quotients by sum share one correctly rounded reciprocal (Markstein's
   // correction holds for any divisor whose reciprocal is correctly rounded)
-  const double zs = 1.0 / sum;
+  const double zs = div_pos(1.0, sum);
   const double w1 = div_known(a1, sum, zs), w2 = div_known(a2, sum, zs), w3 = div_known(a3, sum, zs);
   // the reference's v*(T)7/6 is (v * 7) / 6, a rounded division (div3, div6 give the same
   // bits); -(T)1/6 is a constant
