@@ -90,33 +90,26 @@ __device__ __forceinline__ double weno5_d(double v1, double v2, double v3, doubl
 // (ADVECTION_METHOD_2D.h:936-1060): upwind by the sign of s times each
 // derivative, the larger magnitude at a shock, none at an expansion.  NaN when
 // no case applies (NaN derivatives): the cell is then left unchanged.
-__device__ __forceinline__ double godunov_h(double s, double mx, double px, double my, double py) {
+// Each direction contributes the square of its chosen derivative (+0 at an
+// expansion) and one square root takes their sum: the reference's cases take the
+// root of the same sums, since adding +0 to a square changes no bit, and the
+// lanes of a wavefront no longer diverge into separate square roots.
+__device__ __forceinline__ double godunov_part(double s, double sm, double sp, double m, double p) {
 #pragma clang fp contract(off)
-  const double smx = s * mx, spx = s * px, smy = s * my, spy = s * py;
-  bool hasx = true;
-  double xd = 0.0;
-  if ((spx >= 0) && (smx >= 0)) {
-    xd = mx;
-  } else if ((spx <= 0) && (smx <= 0)) {
-    xd = px;
-  } else if ((spx > 0) && (smx < 0)) {
-    hasx = false;
-  } else if ((spx < 0) && (smx > 0)) {
-    const double sss = s * (fabs(px) - fabs(mx)) / (px - mx);
-    xd = sss > 0 ? mx : px;
-  } else {
-    return __longlong_as_double(0x7ff8000000000000LL);
-  }
-  const double x2 = xd * xd;
-  if ((spy >= 0) && (smy >= 0)) return hasx ? sqrt(x2 + my * my) : sqrt(my * my);
-  if ((spy <= 0) && (smy <= 0)) return hasx ? sqrt(x2 + py * py) : sqrt(py * py);
-  if ((spy > 0) && (smy < 0)) return hasx ? sqrt(x2) : 0.0;
-  if ((spy < 0) && (smy > 0)) {
-    const double ss = s * (fabs(py) - fabs(my)) / (py - my);
-    if (ss > 0) return hasx ? sqrt(x2 + my * my) : sqrt(my * my);
-    return hasx ? sqrt(x2 + py * py) : sqrt(py * py);
+  if ((sp >= 0) && (sm >= 0)) return m * m;
+  if ((sp <= 0) && (sm <= 0)) return p * p;
+  if ((sp > 0) && (sm < 0)) return 0.0;
+  if ((sp < 0) && (sm > 0)) {
+    const double ss = s * (fabs(p) - fabs(m)) / (p - m);
+    return ss > 0 ? m * m : p * p;
   }
   return __longlong_as_double(0x7ff8000000000000LL);
+}
+__device__ __forceinline__ double godunov_h(double s, double mx, double px, double my, double py) {
+#pragma clang fp contract(off)
+  const double x2 = godunov_part(s, s * mx, s * px, mx, px);
+  const double y2 = godunov_part(s, s * my, s * py, my, py);
+  return sqrt(x2 + y2);
 }
 
 // Stages a 64 x 16 tile of phi plus a 3-cell ring in LDS with the ghost rule
