@@ -212,7 +212,7 @@ def time_reinit(P2D, torch, ctx, w, dev, reps=5):
             "velocity_advect_stage_ms": round(statistics.median(vel), 4),
             "viscosity_step_ms": round(statistics.median(visc), 4),
             "accounting": "re-initialisation step: reads phi and the sign function, writes phi (24 B/cell, FP64), "
-                          "~770 FP64 VALU instructions per cell in the compiled kernel (four WENO5 derivatives with 16 correctly rounded quotients, Godunov), VALU-bound; advection stage: "
+                          "~600 FP64 VALU instructions per cell in the compiled kernel (four WENO5 derivatives with 16 correctly rounded quotients, Godunov), VALU-bound; advection stage: "
                           "reads phi and the two face velocities, writes phi (32 B/cell), two WENO5 derivatives "
                           "per cell, timed with its two device copies; velocity stage (ENO3, both components): reads u, v, writes u, v, timed with its four device copies; explicit viscosity step (air/water 1.78e-5/1.137e-3): reads phi, u, v, writes u, v, timed with its four device copies"}
 
