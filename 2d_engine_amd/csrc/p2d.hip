@@ -513,8 +513,8 @@ void vcycle(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
 
 // The part of a CG iteration after the V-cycle's head: the rest of the V-cycle,
 // the new direction and alpha, with the direction kernel and finish that can end
-// a pass one head early (k_dir<T, true>, k_fin_dir<true>; both precisions: FP64
-// RT 2048^2 4.95 -> 4.88 ms, 4096^2 17.28 -> 17.02 ms, the same iterations).
+// a pass one head early (both precisions: FP64 RT 2048^2 4.95 -> 4.88 ms, 4096^2
+// 17.28 -> 17.02 ms, the same iterations).
 template <class T>
 void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   vcycle_tail<T>(c, mg, v);
@@ -526,10 +526,10 @@ void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   T* pnew = v.p[1 - (t.iter & 1)];
   const int first = t.iter == t.seg0 ? 1 : 0;
   kt_begin(c, kd);
-  k_dir<T, true><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, pold, pnew, v.q, v.r[t.rpar], first,
-                                             c->partial, c->s);
+  k_dir<T><<<g, BLOCK, 0, c->stream>>>(c->nx, c->ny, l0.bw, l0.bs, v.z, pold, pnew, v.q, v.r[t.rpar], first,
+                                       c->partial, c->s);
   kt_end(c, kd);
-  k_fin_dir<true><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), (double)c->N, c->s);
+  k_fin_dir<<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)count(g), (double)c->N, c->s);
   t.pending = 1;  // k_fin_dir
 }
 
@@ -1631,12 +1631,12 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
   auto launch = [&]() {
     switch (which) {
       case 0:
-        k_dir<float, true><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
+        k_dir<float><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
             c->nx, c->ny, L32[0].bw, L32[0].bs, c->v32.z, c->v32.p[1], c->v32.p[0], c->v32.q, c->v32.r[0], 0, c->partial,
             c->s);
         break;
       case 1:
-        k_dir<double, true><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
+        k_dir<double><<<dirgrid(c->nx, c->ny), BLOCK, 0, c->stream>>>(
             c->nx, c->ny, c->bw, c->bs, c->v64.z, c->v64.p[1], c->v64.p[0], c->v64.q, c->v64.r[0], 0, c->partial,
             c->s);
         break;

@@ -848,7 +848,7 @@ k_mg_pre(Level<T> F, Level<T> Cc, const T* r_lvl, T* x_lvl, int nu, T* rout, T* 
   }
 }
 
-// The next direction kernel prepares the early convergence test (k_dir<T, true>)
+// The next direction kernel prepares the early convergence test (k_dir)
 // when the residual is within NEAR_END (in sum r^2) of its pass's relative target,
 // or when one more iteration at the rate of the last one would bring it within
 // NEAR_RATE of the target: the over-relaxed multigrid cuts sum r^2 by ~100x per
@@ -1338,9 +1338,9 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 // --------------------------------------------------------------------------- Krylov direction kernel
 // p = (z - mean z) + beta p_old (FP64 arithmetic, stored as T; first: the first
 // direction of a sequence, p_old unused; the host picks p_old and p_new from its
-// replay of the solver state), q = A0' p, the block partials of p.q.  EARLY (the
-// solves' passes), when the state says the pass is near its end: also r.q, q.q and
-// sum q with the current residual rcur, from which k_fin_dir<true> knows the next
+// replay of the solver state), q = A0' p, the block partials of p.q.  When the
+// state says the pass is near its end: also r.q, q.q and
+// sum q with the current residual rcur, from which k_fin_dir knows the next
 // residual's norm before the update is applied.  The choice depends on the solve's
 // own state only, so a solve ends its passes in the same places whatever ran before.
 // Tile kernel over 128 x DTY cells: lane c of a wavefront holds the column pair
@@ -1350,7 +1350,7 @@ __global__ void __launch_bounds__(SMALL_THREADS) k_mg_small(SmallArgs<T> A, int 
 // an even nx (it coarsens at least once), so a pair is inside the grid or
 // outside it as a whole.  Grid dim3(ceil(nx/128), ceil(ny/DTY)), BLOCK threads.
 constexpr int DTX = 128;  // columns per block of the direction kernel
-template <class T, bool EARLY>
+template <class T>
 __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restrict__ bw, const T* __restrict__ bs,
                                                const T* __restrict__ z, const T* pold, T* pnew, T* __restrict__ q,
                                                const T* __restrict__ rcur, int first, double* partial, Scal* s) {
@@ -1398,7 +1398,7 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
       if (!first) o2[m] = ld_at<Pair<T>>(pold, k);
     }
   }
-  const int near = EARLY ? s->near : 0;
+  const int near = s->near;
   Pair<T> rc[RPW];
 #pragma unroll
   for (int m = 0; m < RPW; ++m) {
@@ -1443,7 +1443,7 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     eB[m] = c == 63 ? eL[m] : e;
   }
   __syncthreads();
-  double acc[EARLY ? 4 : 1] = {};
+  double acc[4] = {};
 #pragma unroll
   for (int m = 0; m < RPW; ++m) {
     const int r = rb + m, j = j0 + r;
@@ -1467,35 +1467,30 @@ __global__ void __launch_bounds__(BLOCK) k_dir(int nx, int ny, const T* __restri
     st_at(q, k, Pair<T>{qa, qb});
     acc[0] += (double)pa * (double)qa;
     acc[0] += (double)pb * (double)qb;
-    if constexpr (EARLY) {
-      if (near) {
-        acc[1] += (double)rc[m].a * (double)qa + (double)rc[m].b * (double)qb;
-        acc[2] += (double)qa * (double)qa + (double)qb * (double)qb;
-        acc[3] += (double)qa + (double)qb;
-      }
+    if (near) {
+      acc[1] += (double)rc[m].a * (double)qa + (double)rc[m].b * (double)qb;
+      acc[2] += (double)qa * (double)qa + (double)qb * (double)qb;
+      acc[3] += (double)qa + (double)qb;
     }
   }
-  if constexpr (EARLY) {
-    if (near) {
-      block_partial<4>(acc, 0u, partial);
-      return;
-    }
+  if (near) {
+    block_partial<4>(acc, 0u, partial);
+    return;
   }
   double a1[1] = {acc[0]};
   block_partial<1>(a1, 0u, partial);
 }
 
 // Finish of the direction kernel: alpha = rz / p.q and the update pending
-// (breakdown when p.q is not positive).  EARLY: the residual the update will
+// (breakdown when p.q is not positive).  Near a pass's end, the residual the update will
 // leave, r' = r - alpha q, has sum r'^2 = sum r^2 - 2 alpha r.q + alpha^2 q.q and
 // sum r' = sum r - alpha sum q (FP64 sums of the FP32 vectors); when its
 // mean-free norm already meets the pass's relative target, the pass ends here, one
 // iteration on and with the update still pending (the refinement applies it), and
 // the next head is not run.
-template <bool EARLY>
 __global__ void __launch_bounds__(FINISH_THREADS) k_fin_dir(const double* __restrict__ partial, int G, double ncells,
                                                             Scal* s) {
-  const int done = s->done, iter = s->iter, near = EARLY ? s->near : 0;
+  const int done = s->done, iter = s->iter, near = s->near;
   const double rz = s->rz, r2raw = s->r2raw, rsum = s->rsum, rel2 = s->rel2, r0_2 = s->r0_2;
   double a[4] = {0.0, 0.0, 0.0, 0.0};
   if (near) {
