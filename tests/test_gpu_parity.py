@@ -160,6 +160,41 @@ def test_projection_matches_oracle(P2D, O, W, name, n, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_projection_rectangular_matches_oracle(P2D, O, W, prec):
+    """A 384 x 320 grid (h = 1/384, the hierarchy ends at a 6 x 5 bottom) with two drops, surface
+    tension and seeded velocities: the GPU projection against the oracle's MG-PCG solve."""
+    nx, ny = 384, 320
+    h = 1.0 / nx
+    print(f"case projection rectangular {nx}x{ny} prec={prec}", flush=True)
+    x = (np.arange(nx) + 0.5) * h
+    y = (np.arange(ny) + 0.5) * h
+    X, Y = np.meshgrid(x, y)
+    phi = np.minimum(np.hypot(X - 0.4, Y - 0.45) - 0.15, np.hypot(X - 0.72, Y - 0.3) - 0.08)
+    nu, nv = ny * (nx + 1), (ny + 1) * nx
+    r = W.uniform(11, 5, nu + nv)
+    u = (2.0 * r[:nu] - 1.0).reshape(ny, nx + 1)
+    v = (2.0 * r[nu:] - 1.0).reshape(ny + 1, nx)
+    u, v = W.zero_walls(u, v)
+    rho_m, rho_p, sigma = 1.0, 1000.0, 0.0728
+    ctx = P2D.Context(nx, ny, h, precision=prec, rtol=1e-12, div_tol=1e-10)
+    ctx.set_fluid(phi, rho_m, rho_p, sigma)
+    up, vp, st = ctx.project(u, v, 1.0)
+    p = ctx.pressure(mean_zero=True)
+    p_ref, u_ref, v_ref, _ = O.project(phi, rho_m, rho_p, sigma, u, v, h, solver=O.SOLVER_MG, tol=1e-15)
+    p_ref = p_ref - p_ref.mean()
+    rel = np.linalg.norm(p - p_ref) / np.linalg.norm(p_ref)
+    div = np.abs(O.divergence(up, vp, h)).max()
+    print(f"  iters={st['iterations']} outer={st['outer']} p_rel={rel:.2e} div={div:.2e}")
+    assert st["status"] == 0
+    assert rel <= 1e-9
+    assert div <= 1e-10
+    du = np.abs(up - u_ref).max() / np.abs(u_ref).max()
+    dv = np.abs(vp - v_ref).max() / np.abs(v_ref).max()
+    assert du <= 1e-8 and dv <= 1e-8
+    assert np.all(up[:, 0] == 0) and np.all(up[:, -1] == 0) and np.all(vp[0] == 0) and np.all(vp[-1] == 0)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_projection_matches_golden(P2D, W, prec):
     """The config-0 case against the committed oracle fixture (no oracle run on the GPU box)."""
     import os
