@@ -159,13 +159,14 @@ def test_projection_matches_oracle(P2D, O, W, name, n, prec):
     assert np.all(u[:, 0] == 0) and np.all(u[:, -1] == 0) and np.all(v[0] == 0) and np.all(v[-1] == 0)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_projection_rectangular_matches_oracle(P2D, O, W, prec):
+@pytest.mark.parametrize("prec,rho_m,rho_p", [(0, 1.0, 1000.0), (1, 1.0, 1000.0), (1, 1000.0, 1.0)])
+def test_projection_rectangular_matches_oracle(P2D, O, W, prec, rho_m, rho_p):
     """A 384 x 320 grid (h = 1/384, the hierarchy ends at a 6 x 5 bottom) with two drops, surface
-    tension and seeded velocities: the GPU projection against the oracle's MG-PCG solve."""
+    tension and seeded velocities: the GPU projection against the oracle's MG-PCG solve; light
+    drops in a heavy fluid, and heavy drops in a light one."""
     nx, ny = 384, 320
     h = 1.0 / nx
-    print(f"case projection rectangular {nx}x{ny} prec={prec}", flush=True)
+    print(f"case projection rectangular {nx}x{ny} prec={prec} rho {rho_m}/{rho_p}", flush=True)
     x = (np.arange(nx) + 0.5) * h
     y = (np.arange(ny) + 0.5) * h
     X, Y = np.meshgrid(x, y)
@@ -175,7 +176,7 @@ def test_projection_rectangular_matches_oracle(P2D, O, W, prec):
     u = (2.0 * r[:nu] - 1.0).reshape(ny, nx + 1)
     v = (2.0 * r[nu:] - 1.0).reshape(ny + 1, nx)
     u, v = W.zero_walls(u, v)
-    rho_m, rho_p, sigma = 1.0, 1000.0, 0.0728
+    sigma = 0.0728
     ctx = P2D.Context(nx, ny, h, precision=prec, rtol=1e-12, div_tol=1e-10)
     ctx.set_fluid(phi, rho_m, rho_p, sigma)
     up, vp, st = ctx.project(u, v, 1.0)
@@ -192,6 +193,33 @@ def test_projection_rectangular_matches_oracle(P2D, O, W, prec):
     dv = np.abs(vp - v_ref).max() / np.abs(v_ref).max()
     assert du <= 1e-8 and dv <= 1e-8
     assert np.all(up[:, 0] == 0) and np.all(up[:, -1] == 0) and np.all(vp[0] == 0) and np.all(vp[-1] == 0)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_projection_single_phase(P2D, O, W, prec):
+    """No interface: the level set is negative everywhere (with a nonzero gradient, so the
+    curvature of the surface-tension term is defined), every face has beta = 1 / rho_minus and no
+    jump term enters; the solve is the plain Neumann Poisson problem."""
+    n = 160
+    h = 1.0 / n
+    print(f"case projection single phase {n}^2 prec={prec}", flush=True)
+    X, Y = np.meshgrid((np.arange(n) + 0.5) * h, (np.arange(n) + 0.5) * h)
+    phi = -0.5 - 0.2 * X - 0.1 * Y
+    u, v = W.uniform_velocity(n, 13)
+    ctx = P2D.Context(n, n, h, precision=prec, rtol=1e-12, div_tol=1e-10)
+    ctx.set_fluid(phi, 1.0, 1000.0, 0.0728)
+    up, vp, st = ctx.project(u, v, 1.0)
+    bx, by, jc = ctx.coefficients()
+    assert np.all(bx[:, 1:-1] == 1.0) and np.all(by[1:-1] == 1.0)
+    p = ctx.pressure(mean_zero=True)
+    p_ref, _, _, _ = O.project(phi, 1.0, 1000.0, 0.0728, u, v, h, solver=O.SOLVER_MG, tol=1e-15)
+    p_ref = p_ref - p_ref.mean()
+    rel = np.linalg.norm(p - p_ref) / np.linalg.norm(p_ref)
+    div = np.abs(O.divergence(up, vp, h)).max()
+    print(f"  iters={st['iterations']} outer={st['outer']} p_rel={rel:.2e} div={div:.2e}")
+    assert st["status"] == 0
+    assert rel <= 1e-9
+    assert div <= 1e-10
 
 
 @pytest.mark.parametrize("prec", [0, 1])
