@@ -439,8 +439,10 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   // The finest post-smoothing: FP32 in 16-wavefront tiles two blocks per CU; FP64, whose tiles
   // take the registers of one 16-wavefront block per CU, in 8-wavefront tiles, two per CU, so
   // that one block's loads overlap the other's sweeps (FP64 2048^2: 64 -> 57 us per launch).
+  // Its halo is 2 nu, so its tiles take ty_post rows (36 and 12), one pre-smoothing halo
+  // row more on each side than the pre kernel's (FP64 RT 2048^2 4.91 -> 4.82 ms).
   constexpr int NW0 = sizeof(T) == 4 ? 16 : 8;
-  const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost, Tile<NW0>::ty);
+  const dim3 gpost = tgrid(lv[0].nx, lv[0].ny, Hpost, Tile<NW0>::ty_post);
   const int kpost = sizeof(T) == 4 ? KT_POST0_F32 : KT_POST0_F64;
   // Tile shape of a coarse level: 8-wavefront tiles (10 rows) when at most two of
   // them per CU cover it; else, when 16-wavefront tiles (34 rows) would need more
@@ -1646,7 +1648,7 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
             c->partial, c->s);
         break;
       case 3:
-        k_mg_post<float, true><<<tgrid(c->nx, c->ny, 2 * nu2), TBLOCK, 0, c->stream>>>(
+        k_mg_post<float, true><<<tgrid(c->nx, c->ny, 2 * nu2, Tile<16>::ty_post), TBLOCK, 0, c->stream>>>(
             L32[0], L32[1], c->v32.r[0], L32[0].xa, c->v32.z, nu2, c->partial, c->s);
         break;
       case 5:  // one re-initialisation step of the fluid's level set
