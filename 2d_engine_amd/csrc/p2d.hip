@@ -1163,8 +1163,12 @@ int p2d_get_coefficients(p2d_ctx* c, double* bx, double* by, double* jc, int mem
   CHK(launched_ok());
   if (bx) CHK(stage_out(c, bx, c->sa[2], c->NU, mem));
   if (by) CHK(stage_out(c, by, c->sa[3], c->NV, mem));
-  if (jc) CHK(stage_out(c, jc, c->jc, c->N, mem));
+  if (jc && c->has_jc) CHK(stage_out(c, jc, c->jc, c->N, mem));
+  if (jc && !c->has_jc && mem != P2D_MEM_HOST)  // no surface tension: jc = 0 (k_coeff leaves it unwritten)
+    k_fill<double><<<blocks_for(c->N), BLOCK, 0, c->stream>>>(c->N, jc, 0.0);
+  CHK(launched_ok());
   HIPCHK(hipStreamSynchronize(c->stream));
+  if (jc && !c->has_jc && mem == P2D_MEM_HOST) std::fill(jc, jc + c->N, 0.0);
   return P2D_OK;
 }
 

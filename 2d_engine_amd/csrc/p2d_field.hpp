@@ -60,14 +60,13 @@ __global__ void __launch_bounds__(BLOCK) k_coeff(int nx, int ny, double h, const
     }
     bw[k] = w;
     bs[k] = so;
-    double jv = 0.0;
+    // sigma = 0: no jump term, jc is not read (p2d_get_coefficients reports zeros)
     if (sigma != 0.0) {
       bool bad = false;
       const double kap = curvature_at(phi, nx, ny, h, i, j, &bad);
       if (bad) s->err = 1;
-      jv = sigma * kap;
+      jc[k] = sigma * kap;
     }
-    jc[k] = jv;
   }
 }
 
