@@ -77,6 +77,7 @@ class P2DStepParams(ctypes.Structure):
         ("rho_minus", ctypes.c_double), ("rho_plus", ctypes.c_double), ("sigma", ctypes.c_double),
         ("mu_minus", ctypes.c_double), ("mu_plus", ctypes.c_double), ("mollification", ctypes.c_double),
         ("weno_epsilon", ctypes.c_double), ("reinit_iterations", ctypes.c_int), ("reinit_scaling", ctypes.c_double),
+        ("rk_order", ctypes.c_int),
     ]
 
 
@@ -412,11 +413,12 @@ class Context:
         return out.value
 
     def advance(self, phi, u, v, dt, allow_not_converged=False, **params):
-        """One TVD-RK3 air-water time step (EULERIAN_FLUID_SOLVER_2D.h:1365-1797): level-set
-        advection and re-initialisation, then three velocity stages (advection, gravity,
-        viscosity, wall velocities, projection).  Keyword parameters are the fields of
-        p2d_step_params (gravity_x, gravity_y, rho_minus, rho_plus, sigma, mu_minus, mu_plus,
-        mollification, weno_epsilon, reinit_iterations, reinit_scaling).  Torch CUDA tensors are
+        """One air-water time step (EULERIAN_FLUID_SOLVER_2D.h:1142-1797; TVD-RK3 unless
+        rk_order says 1 or 2): level-set advection and re-initialisation, then the velocity
+        stages (advection, gravity, viscosity, wall velocities, projection).  Keyword parameters
+        are the fields of p2d_step_params (gravity_x, gravity_y, rho_minus, rho_plus, sigma,
+        mu_minus, mu_plus, mollification, weno_epsilon, reinit_iterations, reinit_scaling,
+        rk_order).  Torch CUDA tensors are
         advanced in place; numpy arrays are copied.  Returns (phi, u, v, stats of the last
         projection)."""
         prm = self._step_params(params)

@@ -1392,6 +1392,7 @@ void p2d_step_params_default(p2d_step_params* p) {
   p->weno_epsilon = 1e-6;
   p->reinit_iterations = 25;
   p->reinit_scaling = 5.0;
+  p->rk_order = 3;
 }
 
 int p2d_cfl_dt_noalloc(int nx, int ny, double h, const double* phi, const double* u, const double* v,
@@ -1434,7 +1435,8 @@ int p2d_cfl_dt_noalloc(int nx, int ny, double h, const double* phi, const double
 int p2d_advance(p2d_ctx* c, double* phi, double* u, double* v, double dt, const p2d_step_params* prm, p2d_stats* out,
                 int mem) {
   if (!c || !phi || !u || !v || !prm || !(dt > 0) || c->nx < 3 || c->ny < 3 || !(prm->rho_minus > 0) ||
-      !(prm->rho_plus > 0) || prm->reinit_iterations < 0 || !(prm->reinit_scaling > 0) || !(prm->weno_epsilon >= 0))
+      !(prm->rho_plus > 0) || prm->reinit_iterations < 0 || !(prm->reinit_scaling > 0) || !(prm->weno_epsilon >= 0) ||
+      prm->rk_order < 1 || prm->rk_order > 3)
     return P2D_BAD_ARG;
   if (!c->st_phi) {
     CHK(dalloc(c, &c->st_phi, c->N));
@@ -1450,24 +1452,47 @@ int p2d_advance(p2d_ctx* c, double* phi, double* u, double* v, double dt, const 
   CHK(stage_in(c, V, v, c->NV, mem));
   const double eps = prm->weno_epsilon;
   const double third = 1 / (double)3, two_thirds = 2 / (double)3;
-  // level-set advection, TVD-RK3 (EULERIAN_FLUID_SOLVER_2D.h:1688-1716)
-  HIPCHK(hipMemcpyAsync(Pn, P, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  const int order = prm->rk_order;
+  const size_t pb = c->N * sizeof(double);
+  // level-set advection by the step's velocity: one stage (order 1,
+  // EULERIAN_FLUID_SOLVER_2D.h:1191), two averaged 1/2-1/2 (order 2, :1308-1313), or
+  // TVD-RK3 (:1688-1716)
   double* T1 = c->v64.x;
-  ls_advect_stage(c, P, U, V, dt, eps, T1);
-  ls_advect_stage(c, T1, U, V, dt, eps, P);
-  lincomb(c, c->N, 0.75, Pn, 0.25, P, P);
-  ls_advect_stage(c, P, U, V, dt, eps, T1);
-  lincomb(c, c->N, third, Pn, two_thirds, T1, P);
-  // re-initialisation, the same combination (:1718-1741)
+  if (order == 1) {
+    ls_advect_stage(c, P, U, V, dt, eps, T1);
+    HIPCHK(hipMemcpyAsync(P, T1, pb, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(Pn, P, pb, hipMemcpyDeviceToDevice, c->stream));
+    ls_advect_stage(c, P, U, V, dt, eps, T1);
+    ls_advect_stage(c, T1, U, V, dt, eps, P);
+    if (order == 2) {
+      lincomb(c, c->N, 0.5, Pn, 0.5, P, P);
+    } else {
+      lincomb(c, c->N, 0.75, Pn, 0.25, P, P);
+      ls_advect_stage(c, P, U, V, dt, eps, T1);
+      lincomb(c, c->N, third, Pn, two_thirds, T1, P);
+    }
+  }
+  // re-initialisation, the same combination (order 1: :1193; order 2: :1324-1329; order 3:
+  // :1718-1741)
   if (prm->reinit_iterations > 0) {
     const int it = prm->reinit_iterations;
     const double sc = prm->reinit_scaling;
-    HIPCHK(hipMemcpyAsync(Pn, P, c->N * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    double* r = reinit_run(c, P, T1, it, sc, eps);
-    r = reinit_run(c, r, r == P ? T1 : P, it, sc, eps);
-    lincomb(c, c->N, 0.75, Pn, 0.25, r, P);
-    r = reinit_run(c, P, T1, it, sc, eps);
-    lincomb(c, c->N, third, Pn, two_thirds, r, P);
+    if (order == 1) {
+      double* r = reinit_run(c, P, T1, it, sc, eps);
+      if (r != P) HIPCHK(hipMemcpyAsync(P, r, pb, hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      HIPCHK(hipMemcpyAsync(Pn, P, pb, hipMemcpyDeviceToDevice, c->stream));
+      double* r = reinit_run(c, P, T1, it, sc, eps);
+      r = reinit_run(c, r, r == P ? T1 : P, it, sc, eps);
+      if (order == 2) {
+        lincomb(c, c->N, 0.5, Pn, 0.5, r, P);
+      } else {
+        lincomb(c, c->N, 0.75, Pn, 0.25, r, P);
+        r = reinit_run(c, P, T1, it, sc, eps);
+        lincomb(c, c->N, third, Pn, two_thirds, r, P);
+      }
+    }
   }
   CHK(launched_ok());
   // the projection's fluid: the new level set
@@ -1503,25 +1528,38 @@ int p2d_advance(p2d_ctx* c, double* phi, double* u, double* v, double dt, const 
     else if (rc != P2D_OK) return rc;
     return P2D_OK;
   };
-  // velocity, TVD-RK3 (:1743-1797)
-  HIPCHK(hipMemcpyAsync(Un, U, c->NU * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipMemcpyAsync(Vn, V, c->NV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  // velocity: SolveForRK once (order 1, :1197), twice averaged 1/2-1/2 (order 2,
+  // :1334-1362), or TVD-RK3 (:1743-1797)
+  if (order >= 2) {
+    HIPCHK(hipMemcpyAsync(Un, U, c->NU * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(Vn, V, c->NV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  }
   c->reuse_coeff = false;
   int src = stage();
-  if (src == P2D_OK) src = stage();
+  if (src == P2D_OK && order >= 2) src = stage();
   if (src != P2D_OK) {
     c->reuse_coeff = false;
     return src;
   }
-  lincomb(c, c->NU, 0.75, Un, 0.25, cu, U);
-  lincomb(c, c->NV, 0.75, Vn, 0.25, cv, V);
-  cu = U;
-  cv = V;
-  src = stage();
-  c->reuse_coeff = false;
-  if (src != P2D_OK) return src;
-  lincomb(c, c->NU, third, Un, two_thirds, cu, U);
-  lincomb(c, c->NV, third, Vn, two_thirds, cv, V);
+  if (order == 1) {
+    c->reuse_coeff = false;
+    if (cu != U) HIPCHK(hipMemcpyAsync(U, cu, c->NU * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (cv != V) HIPCHK(hipMemcpyAsync(V, cv, c->NV * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  } else if (order == 2) {
+    c->reuse_coeff = false;
+    lincomb(c, c->NU, 0.5, Un, 0.5, cu, U);
+    lincomb(c, c->NV, 0.5, Vn, 0.5, cv, V);
+  } else {
+    lincomb(c, c->NU, 0.75, Un, 0.25, cu, U);
+    lincomb(c, c->NV, 0.75, Vn, 0.25, cv, V);
+    cu = U;
+    cv = V;
+    src = stage();
+    c->reuse_coeff = false;
+    if (src != P2D_OK) return src;
+    lincomb(c, c->NU, third, Un, two_thirds, cu, U);
+    lincomb(c, c->NV, third, Vn, two_thirds, cv, V);
+  }
   CHK(launched_ok());
   CHK(stage_out(c, phi, P, c->N, mem));
   CHK(stage_out(c, u, U, c->NU, mem));

@@ -360,8 +360,9 @@ __global__ void __launch_bounds__(BLOCK) k_viscosity(int nx, int ny, double h, d
 
 // Time-step helpers.  out = a x + b y (the TVD-RK3 stage combinations,
 // EULERIAN_FLUID_SOLVER_2D.h:1700-1795: 0.75 / 0.25 and 1/3 / 2/3).
+// out may be y (element k is read before it is written).
 __global__ void __launch_bounds__(BLOCK) k_lincomb(size_t n, double a, const double* __restrict__ x, double b,
-                                                   const double* __restrict__ y, double* __restrict__ out) {
+                                                   const double* y, double* out) {
 #pragma clang fp contract(off)
   for (size_t k = blockIdx.x * (size_t)BLOCK + threadIdx.x; k < n; k += (size_t)gridDim.x * BLOCK)
     out[k] = a * x[k] + b * y[k];

@@ -201,6 +201,9 @@ typedef struct p2d_step_params {
   double weno_epsilon;          /* WENO epsilon of advection and re-initialisation (1e-6) */
   int reinit_iterations;        /* Sussman steps per Reinitialization call (25); 0 = none */
   double reinit_scaling;        /* scaling_number_for_reinitialization (5) */
+  int rk_order;                 /* order_for_time_advancing (EULERIAN_FLUID_SOLVER_2D.h:1142-1797):
+                                 * 1 forward Euler, 2 two stages averaged 1/2-1/2, 3 TVD-RK3 (3, the
+                                 * scripts' value) */
 } p2d_step_params;
 
 void p2d_step_params_default(p2d_step_params* params);
@@ -230,14 +233,15 @@ int p2d_cfl_dt(p2d_ctx* ctx, const double* phi, const double* u, const double* v
 int p2d_cfl_dt_noalloc(int nx, int ny, double h, const double* phi, const double* u, const double* v,
                        const p2d_step_params* params, double cfl_number, double* dt, void* hip_stream);
 
-/* One time step of the air-water solver with third-order TVD Runge-Kutta, in place on phi
- * (nx*ny), u ((nx+1)*ny), v (nx*(ny+1)): EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread
- * with order_for_time_advancing = 3 (EULERIAN_FLUID_SOLVER_2D.h:1365-1797, no vortex sheet).
- * Level set: three WENO5 advection stages (p2d_advect_levelset) combined 3/4-1/4 and 1/3-2/3;
- * re-initialisation: three p2d_reinitialize calls combined the same way; velocity: three
+/* One time step of the air-water solver, in place on phi (nx*ny), u ((nx+1)*ny), v (nx*(ny+1)):
+ * EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread with order_for_time_advancing =
+ * params->rk_order (EULERIAN_FLUID_SOLVER_2D.h:1142-1797, no vortex sheet).  Order 3: level set
+ * by three WENO5 advection stages (p2d_advect_levelset) combined 3/4-1/4 and 1/3-2/3;
+ * re-initialisation by three p2d_reinitialize calls combined the same way; velocity by three
  * SolveForRK stages (ENO3 advection, gravity, viscosity, wall velocities 0, p2d_project with the
- * new level set's fluid) combined the same way.  stats: the last projection's (status
- * P2D_NOT_CONVERGED if any projection did not converge).  The caller chooses dt (CFL). */
+ * new level set's fluid) combined the same way.  Order 2: two stages of each, averaged 1/2-1/2;
+ * order 1: one stage of each.  stats: the last projection's (status P2D_NOT_CONVERGED if any
+ * projection did not converge).  The caller chooses dt (CFL). */
 int p2d_advance(p2d_ctx* ctx, double* phi, double* u, double* v, double dt, const p2d_step_params* params,
                 p2d_stats* stats, int mem);
 

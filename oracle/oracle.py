@@ -225,20 +225,39 @@ def rk3(y, stage):
     return out if tup else out[0]
 
 
+def rk(order, y, stage):
+    """The time-advancing combination of order 1, 2 or 3 (order_for_time_advancing,
+    EULERIAN_FLUID_SOLVER_2D.h:1142-1797, no vortex sheet): 1 is one stage (:1191, :1193,
+    :1197); 2 is two stages averaged, 1/2 y + 1/2 L(L(y)) (:1308-1313, :1324-1329,
+    :1349-1360); 3 is rk3."""
+    if order == 3:
+        return rk3(y, stage)
+    if order == 1:
+        return stage(y)
+    if order != 2:
+        raise ValueError("order must be 1, 2 or 3")
+    tup = isinstance(y, tuple)
+    ys = y if tup else (y,)
+    y1 = stage(stage(y))
+    y1 = y1 if tup else (y1,)
+    out = tuple(0.5 * a + 0.5 * b for a, b in zip(ys, y1))
+    return out if tup else out[0]
+
+
 def advance(phi, u, v, h, dt, gravity=(0.0, -9.8), rho_minus=1.0, rho_plus=1000.0, sigma=0.0, mu_minus=0.0,
             mu_plus=0.0, mollification=None, weno_epsilon=1e-6, reinit_iterations=25, reinit_scaling=5.0,
-            solver=None, tol=1e-13):
-    """One TVD-RK3 air-water time step composed of the restated stages,
-    EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread with order 3 and no vortex sheet
-    (EULERIAN_FLUID_SOLVER_2D.h:1688-1797): rk3 of the level-set advection stage by the
-    step's initial velocity, rk3 of the Reinitialization call, then rk3 of SolveForRK
+            solver=None, tol=1e-13, rk_order=3):
+    """One air-water time step composed of the restated stages,
+    EULERIAN_FLUID_SOLVER_2D::AdvanceOneTimeStepThread with order rk_order (default 3) and no
+    vortex sheet (EULERIAN_FLUID_SOLVER_2D.h:1142-1797): rk of the level-set advection stage by
+    the step's initial velocity, rk of the Reinitialization call, then rk of SolveForRK
     (EULERIAN_FLUID_SOLVER_2D.h:1857-1877: ENO3 advection, ApplyGraivity :3249-3269,
     viscosity, wall velocities 0 :2227-2260, PROJECTION_2D::Solve with the new level set).
     Returns (phi, u, v)."""
     u0, v0 = _f64(u).copy(), _f64(v).copy()
-    p = rk3(_f64(phi).copy(), lambda q: advect_levelset(q, u0, v0, h, dt, weno_epsilon))
+    p = rk(rk_order, _f64(phi).copy(), lambda q: advect_levelset(q, u0, v0, h, dt, weno_epsilon))
     if reinit_iterations > 0:
-        p = rk3(p, lambda q: reinit_sussman(q, h, reinit_iterations, reinit_scaling, weno_epsilon))
+        p = rk(rk_order, p, lambda q: reinit_sussman(q, h, reinit_iterations, reinit_scaling, weno_epsilon))
     eps = 1.5 * h if mollification is None or mollification <= 0 else mollification
     solver = SOLVER_IC0 if solver is None else solver
 
@@ -255,7 +274,7 @@ def advance(phi, u, v, h, dt, gravity=(0.0, -9.8), rho_minus=1.0, rho_plus=1000.
         _, a, b, _ = project(p, rho_minus, rho_plus, sigma, a, b, h, dt, solver=solver, tol=tol)
         return a, b
 
-    a, b = rk3((u0, v0), stage)
+    a, b = rk(rk_order, (u0, v0), stage)
     return p, a, b
 
 

@@ -127,6 +127,7 @@ def test_parameter_defaults_and_visc_argument_checks(P2D):
     N.lib.p2d_step_params_default(ctypes.byref(prm))
     assert (prm.gravity_x, prm.gravity_y) == (0.0, -9.8)
     assert prm.reinit_iterations == 25 and prm.reinit_scaling == 5.0 and prm.weno_epsilon == 1e-6
+    assert prm.rk_order == 3                    # the scripts' order_for_time_advancing
     ow = N.P2DOilWaterParams()
     N.lib.p2d_oil_water_params_default(ctypes.byref(ow))
     assert (ow.Re, ow.rho_oil, ow.rho_water, ow.mu_oil, ow.mu_water) == (0.94983, 0.905, 0.995, 6.01, 0.01)

@@ -24,12 +24,13 @@ def _close(a, b, tol=1e-8):
     return np.abs(a - b).max() <= tol * scale
 
 
-@pytest.mark.parametrize("visc", [False, True])
-def test_advance_matches_oracle_two_steps(P2D, O, W, visc):
+# rk_order: order_for_time_advancing (3 in the scripts; 1 and 2 the reference's other choices)
+@pytest.mark.parametrize("visc,rk_order", [(False, 3), (True, 3), (False, 1), (True, 2)])
+def test_advance_matches_oracle_two_steps(P2D, O, W, visc, rk_order):
     n = 64
-    print(f"case advance n={n} viscosity={visc}", flush=True)
+    print(f"case advance n={n} viscosity={visc} rk_order={rk_order}", flush=True)
     phi, u, v, h = _start(O, W, n)
-    kw = dict(rho_minus=1.0, rho_plus=1000.0, sigma=0.0728)
+    kw = dict(rho_minus=1.0, rho_plus=1000.0, sigma=0.0728, rk_order=rk_order)
     if visc:
         kw.update(mu_minus=1.8e-5, mu_plus=1e-3)
     dt = 0.4 * h
@@ -89,6 +90,9 @@ def test_advance_rejects_bad_parameters(P2D):
         ctx.advance(z, np.zeros((n, n + 1)), np.zeros((n + 1, n)), -1.0)
     with pytest.raises(P2D.P2DError):
         ctx.advance(z, np.zeros((n, n + 1)), np.zeros((n + 1, n)), 0.1, rho_minus=0.0)
+    for order in (0, 4):
+        with pytest.raises(P2D.P2DError):
+            ctx.advance(z, np.zeros((n, n + 1)), np.zeros((n + 1, n)), 0.1, rk_order=order)
 
 
 @pytest.mark.parametrize("name,n", [("circle", 64), ("drops64", 128), ("rt", 256)])

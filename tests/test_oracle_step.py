@@ -23,6 +23,23 @@ def test_rk3_amplification_is_cubic_taylor(O):
     assert abs(b[0] - 2 * (1 - 0.5 + 0.125 - 0.125 / 6)) <= 1e-15
 
 
+def test_rk_orders_1_and_2_amplification(O):
+    """order_for_time_advancing 1 and 2 with forward-Euler stages: 1 + z (forward Euler) and
+    1 + z + z^2/2 (two stages averaged, Heun's TVD scheme); order 3 is rk3."""
+    for z in (-0.5, -0.1, 0.3, 1.0):
+        one = O.rk(1, np.array([1.0]), lambda y: y + z * y)[0]
+        two = O.rk(2, np.array([1.0]), lambda y: y + z * y)[0]
+        three = O.rk(3, np.array([1.0]), lambda y: y + z * y)[0]
+        assert abs(one - (1 + z)) <= 1e-15
+        assert abs(two - (1 + z + z * z / 2)) <= 1e-15
+        assert abs(three - (1 + z + z * z / 2 + z ** 3 / 6)) <= 1e-15
+    a, b = O.rk(2, (np.array([1.0]), np.array([2.0])), lambda t: (t[0] * 1.5, t[1] * 0.5))
+    assert abs(a[0] - (0.5 + 0.5 * 2.25)) <= 1e-15 and abs(b[0] - 2 * (0.5 + 0.5 * 0.25)) <= 1e-15
+    import pytest
+    with pytest.raises(ValueError):
+        O.rk(4, np.array([1.0]), lambda y: y)
+
+
 def _flat(n, y0=0.43):
     h = 1.0 / n
     y = (np.arange(n) + 0.5) * h
