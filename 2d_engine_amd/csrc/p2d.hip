@@ -450,15 +450,19 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
   // (fewer wavefronts per SIMD on the busiest CUs); else 16-wavefront tiles, FP32
   // ones two blocks per CU when they outnumber the CUs (BUSY variants).
   const size_t ncu = (size_t)c->num_cu;
-  auto shape = [&](int l, int H) {
-    if (count(tgrid(lv[l].nx, lv[l].ny, H, Tile<8>::ty)) <= 2 * ncu) return 8;
-    if (count(tgrid(lv[l].nx, lv[l].ny, H)) > ncu && count(tgrid(lv[l].nx, lv[l].ny, H, Tile<12>::ty)) <= 2 * ncu)
+  auto shape = [&](int l, int H, bool post) {
+    const int t8 = post ? Tile<8>::ty_post : Tile<8>::ty, t12 = post ? Tile<12>::ty_post : Tile<12>::ty;
+    const int t16 = post ? Tile<16>::ty_post : Tile<16>::ty;
+    if (count(tgrid(lv[l].nx, lv[l].ny, H, t8)) <= 2 * ncu) return 8;
+    if (count(tgrid(lv[l].nx, lv[l].ny, H, t16)) > ncu && count(tgrid(lv[l].nx, lv[l].ny, H, t12)) <= 2 * ncu)
       return 12;
     return 16;
   };
-  auto busy = [&](int l, int H) { return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H)) > ncu; };
+  auto busy = [&](int l, int H, bool post) {
+    return sizeof(T) == 4 && count(tgrid(lv[l].nx, lv[l].ny, H, post ? Tile<16>::ty_post : Tile<16>::ty)) > ncu;
+  };
   for (int l = 1; l < mg.l0; ++l) {
-    const int sh = shape(l, Hpre);
+    const int sh = shape(l, Hpre, false);
     const Level<T>& F = lv[l];
     if (sh == 8)
       k_mg_pre<T, false, false, 8><<<tgrid(F.nx, F.ny, Hpre, Tile<8>::ty), Tile<8>::threads, 0, c->stream>>>(
@@ -466,7 +470,7 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
     else if (sh == 12)
       k_mg_pre<T, false, false, 12><<<tgrid(F.nx, F.ny, Hpre, Tile<12>::ty), Tile<12>::threads, 0, c->stream>>>(
           F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, nullptr, c->s);
-    else if (busy(l, Hpre))
+    else if (busy(l, Hpre, false))
       k_mg_pre<T, false, true><<<tgrid(F.nx, F.ny, Hpre), TBLOCK, 0, c->stream>>>(
           F, lv[l + 1], F.r, F.xa, nu1, nullptr, nullptr, nullptr, nullptr, 0, 0, 0, nullptr, c->s);
     else
@@ -485,20 +489,21 @@ void vcycle_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
     default: k_mg_small<T, 4><<<1, SMALL_THREADS, mg.small_lds, c->stream>>>(sa, nu1, nu2, c->s); break;
   }
   for (int l = mg.l0 - 1; l >= 1; --l) {
-    const int sh = shape(l, Hpost);
+    const int sh = shape(l, Hpost, true);
     const Level<T>& F = lv[l];
     if (sh == 8)
-      k_mg_post<T, false, false, 8><<<tgrid(F.nx, F.ny, Hpost, Tile<8>::ty), Tile<8>::threads, 0, c->stream>>>(
+      k_mg_post<T, false, false, 8><<<tgrid(F.nx, F.ny, Hpost, Tile<8>::ty_post), Tile<8>::threads, 0, c->stream>>>(
           F, lv[l + 1], F.r, F.xa, F.xb, nu2, nullptr, c->s);
     else if (sh == 12)
-      k_mg_post<T, false, false, 12><<<tgrid(F.nx, F.ny, Hpost, Tile<12>::ty), Tile<12>::threads, 0, c->stream>>>(
+      k_mg_post<T, false, false, 12><<<tgrid(F.nx, F.ny, Hpost, Tile<12>::ty_post), Tile<12>::threads, 0,
+                                        c->stream>>>(
           F, lv[l + 1], F.r, F.xa, F.xb, nu2, nullptr, c->s);
-    else if (busy(l, Hpost))
-      k_mg_post<T, false, true><<<tgrid(F.nx, F.ny, Hpost), TBLOCK, 0, c->stream>>>(F, lv[l + 1], F.r, F.xa, F.xb,
-                                                                                      nu2, nullptr, c->s);
+    else if (busy(l, Hpost, true))
+      k_mg_post<T, false, true><<<tgrid(F.nx, F.ny, Hpost, Tile<16>::ty_post), TBLOCK, 0, c->stream>>>(
+          F, lv[l + 1], F.r, F.xa, F.xb, nu2, nullptr, c->s);
     else
-      k_mg_post<T, false><<<tgrid(F.nx, F.ny, Hpost), TBLOCK, 0, c->stream>>>(F, lv[l + 1], F.r, F.xa, F.xb, nu2,
-                                                                              nullptr, c->s);
+      k_mg_post<T, false><<<tgrid(F.nx, F.ny, Hpost, Tile<16>::ty_post), TBLOCK, 0, c->stream>>>(
+          F, lv[l + 1], F.r, F.xa, F.xb, nu2, nullptr, c->s);
   }
   kt_begin(c, kpost);
   k_mg_post<T, true, false, NW0><<<gpost, Tile<NW0>::threads, 0, c->stream>>>(lv[0], lv[1], v.r[c->trk.rpar],

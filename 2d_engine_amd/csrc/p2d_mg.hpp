@@ -45,8 +45,8 @@ struct Tile {
   static constexpr int bnd = NW * 2 * 64;                  // one boundary buffer: [wave][bottom, top][lane]
   static constexpr int lds_pairs = 2 * bnd;                // two buffers (also the restriction's row sums
                                                            // and the block reduction)
-  // The finest level's post-smoothing (halo H = 2 nu <= 2 MAX_NU) fills the strips with
-  // ty_post output rows (36, 24, 12); the coarse levels keep ty for both kernels.
+  // The post-smoothing (halo H = 2 nu <= 2 MAX_NU) fills the strips with ty_post output
+  // rows (36, 24, 12).
   static constexpr int ty_post = KR * NW - 4 * MAX_NU;
   static constexpr int pch = (ty_post + 4 * MAX_NU) / 2 + 3;  // staged coarse rows of the post kernel
   static_assert(KR * NW >= rymax, "a tile region must fit the strips");
@@ -906,7 +906,7 @@ template <class T, bool LEVEL0, bool BUSY = false, int NW = 16>
 __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(4 * tile_blocks<T, LEVEL0 || BUSY>())))
 k_mg_post(Level<T> F, Level<T> Cc, const T* r_lvl, const T* x_in, T* x_out, int nu, double* partial, Scal* s) {
   using Sh = Tile<NW>;
-  constexpr int TY = LEVEL0 ? Sh::ty_post : Sh::ty, PCH = Sh::pch, TBLK = Sh::threads;
+  constexpr int TY = Sh::ty_post, PCH = Sh::pch, TBLK = Sh::threads;
   __shared__ Pair<T> lds[Sh::lds_pairs];
   __shared__ T xcs[PCH][PCW];
   const int H = 2 * nu, HX = tile_hx(H), TX = 128 - 2 * HX, RY = TY + 2 * H;
