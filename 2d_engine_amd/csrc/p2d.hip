@@ -130,13 +130,13 @@ struct p2d_ctx {
   int num_cu = 256;  // compute units of the device (tile kernels with more tiles run two blocks per CU)
   double rho_m = 1, rho_p = 1, sigma = 0;
 
-  // right-hand side, solutions (double-double, double-buffered), staging
+  // right-hand side, solutions (double-double with an FP32 low part, double-buffered), staging
   double* b = nullptr;
   double* ph[2] = {nullptr, nullptr};
-  double* pl[2] = {nullptr, nullptr};
+  float* pl[2] = {nullptr, nullptr};
   int pcur = 0;
   double* muh[2] = {nullptr, nullptr};
-  double* mul[2] = {nullptr, nullptr};
+  float* mul[2] = {nullptr, nullptr};
   double *su = nullptr, *sv = nullptr, *sg1 = nullptr, *sg2 = nullptr, *sp = nullptr;
   // time-step state (allocated by the first p2d_advance): phi, u, v and their stage-start copies
   double *st_phi = nullptr, *st_phin = nullptr, *st_u = nullptr, *st_v = nullptr, *st_un = nullptr, *st_vn = nullptr;
@@ -543,7 +543,7 @@ void cg_tail(p2d_ctx* c, MG<T>& mg, KVec<T>& v) {
 // Residual launch plus its finish (sum r^2, max |r|, sum r into s->red[0..2]).
 template <class T>
 void correct_resid(p2d_ctx* c, const double* b, const double* saw, const double* sas, const double* ph,
-                   const double* pl, const T* e, const T* pd, double* qh, double* ql, T* r) {
+                   const float* pl, const T* e, const T* pd, double* qh, float* ql, T* r) {
   const dim3 g = fgrid(c->nx, c->ny);
   const int kr = sizeof(T) == 4 ? KT_RESID_F32 : KT_RESID_F64;
   kt_begin(c, kr);
@@ -578,7 +578,7 @@ struct Rhs {
 // without waiting: the continue kernel applies the outer convergence test on
 // the device, so a pass costs one host synchronisation.
 template <class T>
-int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double* solh[2], double* soll[2],
+int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double* solh[2], float* soll[2],
             int* cur, double rtol, double atol, double abs2, SolveResult* out) {
   const size_t N = c->N;
   const bool mixed = sizeof(T) == 4;
@@ -665,7 +665,7 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
     if (outer < MAX_PASSES) last[outer] = iters - seg_start;
     T* rdst = c->hs->rpar ? v.r[1] : v.r[0];
     const double* oh = outer > 0 ? solh[*cur] : nullptr;  // the first pass starts from p = 0
-    const double* ol = outer > 0 ? soll[*cur] : nullptr;
+    const float* ol = outer > 0 ? soll[*cur] : nullptr;
     // a pass that ended at its direction's finish leaves that update pending: the
     // refinement applies it (direction p[iter & 1], alpha in the state); x holds the
     // pass's correction only if a head applied an update (the first writes x = a p)
@@ -700,7 +700,7 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
   c->last_passes[slot] = outer;
   if (outer == 0) {  // no refinement ran: p = 0
     k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, solh[*cur], 0.0);
-    k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, soll[*cur], 0.0);
+    k_fill<float><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, soll[*cur], 0.0f);
     CHK(launched_ok());
     r2 = c->hs->red[0];
     rinf = c->hs->red[1];
@@ -718,7 +718,7 @@ int solve_t(p2d_ctx* c, MG<T>& mg, KVec<T>& v, int slot, const Rhs& rhs, double*
   return P2D_OK;
 }
 
-int solve(p2d_ctx* c, const Rhs& rhs, double* solh[2], double* soll[2], int* cur, double rtol, double atol,
+int solve(p2d_ctx* c, const Rhs& rhs, double* solh[2], float* soll[2], int* cur, double rtol, double atol,
           double abs2, SolveResult* out) {
   if (c->cfg.precision == P2D_MIXED) {
     CHK(ensure_mg(c, P2D_PREC_F32));
@@ -1099,7 +1099,7 @@ int p2d_get_pressure(p2d_ctx* c, double* p, int mean_zero, int mem) {
   if (!c || !p || !c->have_pressure) return P2D_BAD_ARG;
   const size_t N = c->N;
   const double* sh = c->ph[c->pcur];
-  const double* sl = c->pl[c->pcur];
+  const float* sl = c->pl[c->pcur];
   k_psum<<<gs_blocks(N), BLOCK, 0, c->stream>>>(N, sh, sl, c->partial, c->s);
   k_fin_store<1><<<1, FINISH_THREADS, 0, c->stream>>>(c->partial, (int)gs_blocks(N), 0u, 3, c->s);
   double* dst = mem == P2D_MEM_HOST ? c->sp : p;
@@ -1671,7 +1671,7 @@ int p2d_time_kernel(p2d_ctx* c, int which, int reps, double* avg_ms) {
   k_pattern<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, c->v64.p[1]);
   k_pattern<float><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, c->v32.x);
   k_pattern<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, c->ph[0]);
-  k_fill<double><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, c->pl[0], 0.0);
+  k_fill<float><<<blocks_for(N), BLOCK, 0, c->stream>>>(N, c->pl[0], 0.0f);
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));

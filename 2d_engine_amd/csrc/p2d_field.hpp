@@ -182,7 +182,8 @@ __global__ void __launch_bounds__(BLOCK) k_divergence(int nx, int ny, double h, 
 }
 
 // --------------------------------------------------------------------------- FP64 refinement step
-// On the double-double pressure: p_new = p_old + e (e may be NULL; ph/pl NULL: p_old = 0;
+// On the double-double pressure (high part FP64, low part FP32: 77 significant bits, the
+// low part rounded once per update): p_new = p_old + e (e may be NULL; ph/pl NULL: p_old = 0;
 // pd: an inner update left pending, e + alpha pd formed in TE first),
 // r = (b' - mean b') - sum_f +-G_f with G_f = beta_f fdiff(p, s a) (west and
 // south faces +, east and north -), stored as R; block partials sum r^2,
@@ -200,9 +201,9 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
                                                          const double* __restrict__ bs,
                                                          const double* __restrict__ saw,
                                                          const double* __restrict__ sas, const double* __restrict__ b,
-                                                         const double* __restrict__ ph, const double* __restrict__ pl,
+                                                         const double* __restrict__ ph, const float* __restrict__ pl,
                                                          const TE* __restrict__ e, const TE* __restrict__ pd,
-                                                         double* __restrict__ qh, double* __restrict__ ql,
+                                                         double* __restrict__ qh, float* __restrict__ ql,
                                                          R* __restrict__ r, double* partial, const Scal* s) {
 #pragma clang fp contract(off)
   // row layout: [1] west ring, [2 + 2c, 3 + 2c] pair c, [FTX + 2] east ring
@@ -226,6 +227,7 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
       ll = 0.0;
     } else if (corr) {
       dd_add(oh, ol, ev, hh, ll);
+      ll = (double)(float)ll;  // the stored low part is FP32: use what is stored
     } else {
       hh = oh;
       ll = ol;
@@ -240,15 +242,15 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
     if (col_in && jj >= 0 && jj < ny) {
       const unsigned k = cell_u(iA, jj, nx);
       const Pair<double> oh = ph ? ld_at<Pair<double>>(ph, k) : Pair<double>{0.0, 0.0};
-      const Pair<double> ol = ph ? ld_at<Pair<double>>(pl, k) : Pair<double>{0.0, 0.0};
+      const Pair<float> ol = ph ? ld_at<Pair<float>>(pl, k) : Pair<float>{0.0f, 0.0f};
       Pair<TE> e2 = e ? ld_at<Pair<TE>>(e, k) : Pair<TE>{(TE)0, (TE)0};
       if (pd) {  // the inner pass's pending update, in TE as the head applies it
         const Pair<TE> d2 = ld_at<Pair<TE>>(pd, k);
         e2.a += ta * d2.a;
         e2.b += ta * d2.b;
       }
-      pnew(oh.a, ol.a, (double)e2.a, h2.a, l2.a);
-      pnew(oh.b, ol.b, (double)e2.b, h2.b, l2.b);
+      pnew(oh.a, (double)ol.a, (double)e2.a, h2.a, l2.a);
+      pnew(oh.b, (double)ol.b, (double)e2.b, h2.b, l2.b);
     }
     sh2[rr][1 + c] = h2;
     sl2[rr][1 + c] = l2;
@@ -261,7 +263,7 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
       const size_t q = ci(ii, jj, nx);
       const TE e0 = e ? e[q] : (TE)0;
       const TE ev = pd ? e0 + ta * pd[q] : e0;
-      pnew(ph ? ph[q] : 0.0, ph ? pl[q] : 0.0, (double)ev, hh, ll);
+      pnew(ph ? ph[q] : 0.0, ph ? (double)pl[q] : 0.0, (double)ev, hh, ll);
     }
     sh[rr][side ? FTX + 2 : 1] = hh;
     sl[rr][side ? FTX + 2 : 1] = ll;
@@ -335,7 +337,7 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
     const double ra = (b2.a - bm) - gA, rbv = (b2.b - bm) - gB;
     if (qh) {
       st_at(qh, k, ch2);
-      st_at(ql, k, cl2);
+      st_at(ql, k, Pair<float>{(float)cl2.a, (float)cl2.b});
     }
     st_at(r, k, Pair<R>{(R)ra, (R)rbv});
     acc[0] += ra * ra;
@@ -353,7 +355,7 @@ __global__ void __launch_bounds__(BLOCK) k_correct_resid(int nx, int ny, const d
 // faces (walls untouched): the same G_f as the residual, so D u_proj = -dt r / h^2
 // up to the rounding of u.
 __global__ void __launch_bounds__(BLOCK) k_face_update(int nx, int ny, double h, double dt,
-                                                       const double* __restrict__ ph, const double* __restrict__ pl,
+                                                       const double* __restrict__ ph, const float* __restrict__ pl,
                                                        const double* __restrict__ saw, const double* __restrict__ sas,
                                                        const double* __restrict__ bw, const double* __restrict__ bs,
                                                        double* __restrict__ u, double* __restrict__ v) {
@@ -363,7 +365,7 @@ __global__ void __launch_bounds__(BLOCK) k_face_update(int nx, int ny, double h,
   if (i >= nx) return;
   for (int j = j0; j < j1; ++j) {
     const size_t k = ci(i, j, nx);
-    const double ch = ph[k], cl = pl[k];
+    const double ch = ph[k], cl = (double)pl[k];
     if (i > 0) {
       const double G = bw[k] * fdiff(ch, cl, ph[k - 1], pl[k - 1], saw ? saw[k] : 0.0);
       u[fxi(i, j, nx)] -= dt * G * ih;
@@ -445,21 +447,21 @@ __global__ void k_clear_err(Scal* s) {
 }
 
 // Sum of p = hi + lo into s->red[3] (for the mean); 1-D grid-stride launch.
-__global__ void __launch_bounds__(BLOCK) k_psum(size_t N, const double* __restrict__ ph, const double* __restrict__ pl,
+__global__ void __launch_bounds__(BLOCK) k_psum(size_t N, const double* __restrict__ ph, const float* __restrict__ pl,
                                                 double* partial, Scal* s) {
   double acc[1] = {0.0};
-  for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < N; k += (size_t)gridDim.x * BLOCK) acc[0] += ph[k] + pl[k];
+  for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < N; k += (size_t)gridDim.x * BLOCK) acc[0] += ph[k] + (double)pl[k];
   block_partial<1>(acc, 0u, partial);
 }
 
 // out = p - mean (mean_zero) or p - p(0,0) (the reference's pin, POISSON_SOLVER.cpp:789-807).
-__global__ void __launch_bounds__(BLOCK) k_shift(size_t N, const double* __restrict__ ph, const double* __restrict__ pl,
+__global__ void __launch_bounds__(BLOCK) k_shift(size_t N, const double* __restrict__ ph, const float* __restrict__ pl,
                                                  double* __restrict__ out, int mean_zero, const Scal* s) {
 #pragma clang fp contract(off)
   const size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x;
   if (k >= N) return;
-  if (mean_zero) out[k] = (ph[k] - s->red[3] / (double)N) + pl[k];
-  else out[k] = (ph[k] - ph[0]) + (pl[k] - pl[0]);
+  if (mean_zero) out[k] = (ph[k] - s->red[3] / (double)N) + (double)pl[k];
+  else out[k] = (ph[k] - ph[0]) + ((double)pl[k] - (double)pl[0]);
 }
 
 // --------------------------------------------------------------------------- adjoint
@@ -495,8 +497,8 @@ __global__ void __launch_bounds__(BLOCK) k_adj_rhs(int nx, int ny, double h, con
 // du = g - G mu, dbeta = -dt du (G p + s a / h) on interior faces; walls 0.
 __global__ void __launch_bounds__(BLOCK) k_adj_faces(int nx, int ny, double h, double dt,
                                                      const double* __restrict__ gu, const double* __restrict__ gv,
-                                                     const double* __restrict__ muh, const double* __restrict__ mul,
-                                                     const double* __restrict__ ph, const double* __restrict__ pl,
+                                                     const double* __restrict__ muh, const float* __restrict__ mul,
+                                                     const double* __restrict__ ph, const float* __restrict__ pl,
                                                      const double* __restrict__ saw, const double* __restrict__ sas,
                                                      double* du, double* dv, double* dbx, double* dby) {
 #pragma clang fp contract(off)
