@@ -48,8 +48,9 @@ KERNELS = [
      "(4 B) and coarse x (1 B), writes z (4 B)"),
     (0, "k_dir<float>", 24,
      "p = z - mean z + beta p_old, q = A p, p.q: reads z p_old beta_x beta_y (16 B), writes p q (8 B)"),
-    (4, "k_correct_resid<float,float>", 64,
-     "FP64 refinement: reads b p_hi p_lo (24 B) x (4 B) beta_x beta_y (16 B), writes p_hi p_lo (16 B) r (4 B)"),
+    (4, "k_correct_resid<float,float>", 56,
+     "FP64 refinement: reads b p_hi (16 B) the FP32 p_lo (4 B) x (4 B) beta_x beta_y (16 B), writes p_hi (8 B) "
+     "p_lo (4 B) r (4 B)"),
 ]
 DOMINANT = "k_mg_pre<float,true>"
 
@@ -57,11 +58,11 @@ DOMINANT = "k_mg_pre<float,true>"
 # a V-cycle is the finest pre + post + dir (41 + 25 + 24) plus, on each coarser level l
 # (N / 4^l cells), pre (reads r beta_x beta_y and the packed weights, writes x and coarse r:
 # 21) and post (reads r x_pre beta_x beta_y, the packed weights and coarse x, writes x: 25),
-# summing to 46 N / 3; a refinement pass is 64; the once-per-projection kernels are the
+# summing to 46 N / 3; a refinement pass is 56; the once-per-projection kernels are the
 # coefficients (reads phi, writes beta_x beta_y jc in FP64: 32), the right-hand side (reads
 # u v beta_x beta_y, writes b and the FP32 residual: 44), the multigrid setup (FP64 beta in,
 # FP32 beta and packed weights out, coarse beta, over the hierarchy: 44), the face update
-# (72) and the divergence check (16).
+# (68, the pressure's low part FP32) and the divergence check (16).
 # An iteration is one full V-cycle with its direction; each pass ends with one more finest
 # pre-smoothing (its update and convergence test, 41), after which the rest of that
 # V-cycle is not launched.  An FP32 pass that converges at its direction finish skips
@@ -69,17 +70,17 @@ DOMINANT = "k_mg_pre<float,true>"
 # refinement the pending direction, 4 more); the count keeps the head: an upper bound.
 VCYCLE_B_PER_CELL = 41 + 25 + 24 + 46.0 / 3.0
 HEAD_B_PER_CELL = 41
-REFINE_B_PER_CELL = 64
-SETUP_B_PER_CELL = 32 + 44 + 44 + 72 + 16
+REFINE_B_PER_CELL = 56
+SETUP_B_PER_CELL = 32 + 44 + 44 + 68 + 16
 
 
 # The FP64 path moves every Krylov vector, coefficient and weight (wx and wy apart) in FP64:
 # finest pre 90 (r q beta_x beta_y wx wy x p read, r x x_pre written, coarse r), post 58,
-# dir 48, coarser levels 2 x 54 N / 3; its refinement reads b p_hi p_lo x beta_x beta_y and
-# writes p_hi p_lo r, all FP64 (72).
+# dir 48, coarser levels 2 x 54 N / 3; its refinement reads b p_hi x beta_x beta_y and writes
+# p_hi r in FP64, and reads and writes the FP32 p_lo (64).
 FP64_VCYCLE_B_PER_CELL = 90 + 58 + 48 + 108.0 / 3.0
 FP64_HEAD_B_PER_CELL = 90
-FP64_REFINE_B_PER_CELL = 72
+FP64_REFINE_B_PER_CELL = 64
 
 
 def projection_bytes(N, stats, fp64=False):
